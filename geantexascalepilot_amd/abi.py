@@ -1,0 +1,157 @@
+"""ctypes mirror of the C ABI declared in include/gxt.h.
+
+The structures here are byte-for-byte the ones in the header; the helper
+functions build the configurations of the five transport workloads (a
+homogeneous lead block, the PbWO4 block, the lead/liquid-argon sampling
+calorimeter in a 4 T field, its FP32 precision form, and the serving mix).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+GEOM_BOX, GEOM_CALO = 0, 1
+MAT_PB, MAT_LAR, MAT_PBWO4 = 0, 1, 2
+FP64, FP32 = 0, 1
+GAMMA, ELECTRON, POSITRON = 0, 1, 2
+
+TAB = dict(LAMBDA_COMPT=0, LAMBDAPRIM_COMPT=1, LAMBDA_CONV=2, LAMBDAPRIM_PHOT=3, LAMBDA_BREM_EM=4,
+           LAMBDA_IONI_EM=5, DEDX_EM=6, RANGE_EM=7, INVRANGE_EM=8, LAMBDA_BREM_EP=9, LAMBDA_IONI_EP=10,
+           DEDX_EP=11, RANGE_EP=12, INVRANGE_EP=13)
+NTAB = 14
+MODEL = dict(KLEIN_NISHINA=0, BETHE_HEITLER=1, SAUTER_GAVRILA=2, SELTZER_BERGER_EM=3, MOLLER=4,
+             SELTZER_BERGER_EP=5, BHABHA=6, ANNIHIL=7)
+
+ERRORS = {-1: "GXT_EINVAL", -2: "GXT_EHIP", -3: "GXT_EDATA", -4: "GXT_ECAPACITY", -5: "GXT_ESTATE",
+          -6: "GXT_ENOTFOUND"}
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("geometry", C.c_int32), ("material", C.c_int32), ("box_xy", C.c_double), ("box_depth", C.c_double),
+        ("n_layers", C.c_int32), ("absorber", C.c_int32), ("gap", C.c_int32), ("pad0", C.c_int32),
+        ("absorber_mm", C.c_double), ("gap_mm", C.c_double), ("lar_density", C.c_double),
+        ("field", C.c_double * 3), ("tracking_cut", C.c_double), ("seed", C.c_uint32),
+        ("precision", C.c_int32), ("zbin_mm", C.c_double), ("rbin_mm", C.c_double), ("n_zbins", C.c_int32),
+        ("n_rbins", C.c_int32), ("pool_capacity", C.c_int64), ("max_steps", C.c_int32),
+        ("max_events", C.c_int32), ("max_jobs", C.c_int32), ("trace_tracks", C.c_int32),
+        ("trace_steps", C.c_int32), ("pad1", C.c_int32), ("data_dir", C.c_char_p),
+    ]
+
+
+class Primary(C.Structure):
+    _fields_ = [("species", C.c_int32), ("event", C.c_int32), ("ekin", C.c_double), ("pos", C.c_double * 3),
+                ("dir", C.c_double * 3), ("substream", C.c_uint64)]
+
+
+class EventTally(C.Structure):
+    _fields_ = [("edep", C.c_double), ("visible", C.c_double), ("leak", C.c_double),
+                ("charged_length", C.c_double), ("n_secondaries", C.c_int64 * 3), ("n_steps", C.c_int64),
+                ("n_pairs", C.c_int64), ("n_annihil", C.c_int64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("track_steps", C.c_int64), ("iterations", C.c_int64), ("max_live", C.c_int64),
+                ("killed_by_limit", C.c_int64), ("wall_ms", C.c_double)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("job_id", C.c_uint64), ("n_events", C.c_int32), ("pad", C.c_int32), ("edep", C.c_double),
+                ("visible", C.c_double), ("submit_ms", C.c_double), ("admit_ms", C.c_double),
+                ("done_ms", C.c_double), ("track_steps", C.c_int64)]
+
+
+class SampleIO(C.Structure):
+    _fields_ = [("n", C.c_int), ("ekin", C.POINTER(C.c_double)), ("dir", C.POINTER(C.c_double)),
+                ("rng", C.POINTER(C.c_uint32)), ("out_ekin", C.POINTER(C.c_double)),
+                ("out_dir", C.POINTER(C.c_double)), ("out_edep", C.POINTER(C.c_double)),
+                ("nsec", C.POINTER(C.c_int32)), ("sec_species", C.POINTER(C.c_int32)),
+                ("sec_ekin", C.POINTER(C.c_double)), ("sec_dir", C.POINTER(C.c_double))]
+
+
+def ptr(a, ctype=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def base_config(**kw):
+    """Defaults shared by every workload: 1 keV tracking cut, 1 cm bins."""
+    c = Config()
+    c.geometry = GEOM_BOX
+    c.material = MAT_PB
+    c.box_xy = 200.0
+    c.box_depth = 200.0
+    c.n_layers = 50
+    c.absorber = MAT_PB
+    c.gap = MAT_LAR
+    c.absorber_mm = 2.3
+    c.gap_mm = 5.7
+    c.lar_density = 1.396
+    c.tracking_cut = 1.0e-3
+    c.seed = 12345
+    c.precision = FP64
+    c.zbin_mm = 10.0
+    c.rbin_mm = 10.0
+    c.n_zbins = 20
+    c.n_rbins = 20
+    c.pool_capacity = 2_000_000
+    c.max_steps = 100000
+    c.max_events = 1000
+    c.max_jobs = 64
+    c.trace_tracks = 0
+    c.trace_steps = 0
+    c._data = DATA_DIR.encode()
+    c.data_dir = c._data
+    for k, v in kw.items():
+        if k == "field":
+            c.field[:] = v
+        else:
+            setattr(c, k, v)
+    return c
+
+
+def config_lead_block(**kw):
+    """1 GeV gammas into a 20x20x20 cm lead block (11.35 g/cm3), seed 12345."""
+    d = dict(geometry=GEOM_BOX, material=MAT_PB, box_xy=200.0, box_depth=200.0, seed=12345, n_zbins=20,
+             n_rbins=10, max_events=1000)
+    d.update(kw)
+    return base_config(**d)
+
+
+def config_pbwo4_block(**kw):
+    """10 GeV electrons into a 30x30x30 cm PbWO4 block (8.28 g/cm3), seed 777."""
+    d = dict(geometry=GEOM_BOX, material=MAT_PBWO4, box_xy=300.0, box_depth=300.0, seed=777, n_zbins=30,
+             n_rbins=15, max_events=10000, pool_capacity=2_000_000)
+    d.update(kw)
+    return base_config(**d)
+
+
+def config_calorimeter(precision=FP64, **kw):
+    """50 x (2.3 mm Pb + 5.7 mm lAr 1.39 g/cm3), 40x40 cm, 4 T along +x."""
+    d = dict(geometry=GEOM_CALO, n_layers=50, absorber=MAT_PB, gap=MAT_LAR, absorber_mm=2.3, gap_mm=5.7,
+             lar_density=1.39, box_xy=400.0, field=(4.0, 0.0, 0.0), seed=2468, n_zbins=40, n_rbins=20,
+             max_events=20000, precision=precision)
+    d.update(kw)
+    return base_config(**d)
+
+
+def primaries_array(n):
+    return (Primary * n)()
+
+
+def tallies_to_numpy(t):
+    n = len(t)
+    out = {k: np.zeros(n) for k in ("edep", "visible", "leak", "charged_length")}
+    out["n_secondaries"] = np.zeros((n, 3), dtype=np.int64)
+    out["n_steps"] = np.zeros(n, dtype=np.int64)
+    out["n_pairs"] = np.zeros(n, dtype=np.int64)
+    out["n_annihil"] = np.zeros(n, dtype=np.int64)
+    for i in range(n):
+        for k in ("edep", "visible", "leak", "charged_length"):
+            out[k][i] = getattr(t[i], k)
+        out["n_secondaries"][i] = list(t[i].n_secondaries)
+        out["n_steps"][i] = t[i].n_steps
+        out["n_pairs"][i] = t[i].n_pairs
+        out["n_annihil"][i] = t[i].n_annihil
+    return out

@@ -1,0 +1,255 @@
+/*
+ * gxt.h -- C ABI of the MI355X-native EM particle-transport step loop (libgxt.so).
+ *
+ * Every entry point returns int: 0 on success, a negative GXT_E* code on
+ * failure (the message is available from gxt_last_error / gxt_ctx_error).
+ * No exception crosses this boundary.  Host buffers are owned by the caller;
+ * all device memory (track pool, RNG states, tables, tallies) is owned by the
+ * library.  A gxt_ctx is driven by one host thread and owns one HIP stream;
+ * the component entry points (gxt_rng_*, gxt_table_*, gxt_sample_*,
+ * gxt_helix_*) use a library-wide default context on device 0.
+ *
+ * Units are CLHEP: mm, MeV, ns; magnetic field in tesla.
+ * Species codes: 0 = gamma, 1 = electron, 2 = positron.
+ */
+#ifndef GXT_H
+#define GXT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define GXT_OK 0
+#define GXT_EINVAL (-1)    /* bad argument                              */
+#define GXT_EHIP (-2)      /* HIP runtime error                         */
+#define GXT_EDATA (-3)     /* physics data file missing or malformed    */
+#define GXT_ECAPACITY (-4) /* track pool or job table full              */
+#define GXT_ESTATE (-5)    /* call not valid in the current state       */
+#define GXT_ENOTFOUND (-6) /* unknown job id                            */
+
+/* ---- configuration ----------------------------------------------------- */
+#define GXT_GEOM_BOX 0  /* homogeneous block: |x|,|y| <= xy/2, 0 <= z <= depth */
+#define GXT_GEOM_CALO 1 /* z-stacked layers of absorber + gap                  */
+
+#define GXT_MAT_PB 0    /* G4_Pb 11.35 g/cm3 (reference tables, couple 1)    */
+#define GXT_MAT_LAR 1   /* G4_lAr (reference tables, couple 2, density-scaled) */
+#define GXT_MAT_PBWO4 2 /* PbWO4 8.28 g/cm3 (generated tables)                */
+
+#define GXT_FP64 0
+#define GXT_FP32 1 /* precision form: FP32 tables/sampling/helix, FP64 position+tallies */
+
+typedef struct gxt_config {
+  int32_t geometry;         /* GXT_GEOM_*                                           */
+  int32_t material;         /* box material (GXT_MAT_*)                             */
+  double box_xy;            /* full transverse size of the block or layers (mm)     */
+  double box_depth;         /* box depth along z (mm)                               */
+  int32_t n_layers;         /* calorimeter layers                                   */
+  int32_t absorber;         /* calorimeter absorber material (GXT_MAT_*)            */
+  int32_t gap;              /* calorimeter gap material (GXT_MAT_*)                 */
+  int32_t pad0;
+  double absorber_mm;       /* absorber thickness per layer (mm)                    */
+  double gap_mm;            /* gap thickness per layer (mm)                         */
+  double lar_density;       /* g/cm3 used for lAr (tables scaled from 1.396)        */
+  double field[3];          /* uniform magnetic field (tesla)                       */
+  double tracking_cut;      /* MeV; tracks below it deposit locally                 */
+  uint32_t seed;            /* MRG32k3a seed, replicated over the 6 state words     */
+  int32_t precision;        /* GXT_FP64 / GXT_FP32                                  */
+  double zbin_mm;           /* longitudinal tally bin (mm)                          */
+  double rbin_mm;           /* radial tally bin (mm)                                */
+  int32_t n_zbins;
+  int32_t n_rbins;
+  int64_t pool_capacity;    /* live-track pool size (tracks)                        */
+  int32_t max_steps;        /* per-track step limit (track killed beyond it)        */
+  int32_t max_events;       /* per-event tally slots                                */
+  int32_t max_jobs;         /* concurrent job slots                                 */
+  int32_t trace_tracks;     /* >0: record the first trace_steps steps of tracks     */
+  int32_t trace_steps;
+  int32_t pad1;
+  const char *data_dir;     /* directory holding the physics data files             */
+} gxt_config;
+
+/* One primary particle. */
+typedef struct gxt_primary {
+  int32_t species;      /* 0 gamma, 1 e-, 2 e+                       */
+  int32_t event;        /* event index inside its job                 */
+  double ekin;          /* kinetic energy (MeV)                       */
+  double pos[3];        /* mm                                         */
+  double dir[3];        /* unit vector                                */
+  uint64_t substream;   /* MRG32k3a substream index of this primary   */
+} gxt_primary;
+
+/* Per-event tallies (FP64). */
+typedef struct gxt_event_tally {
+  double edep;          /* total energy deposited in the setup (MeV)  */
+  double visible;       /* deposited in the gap (calorimeter) or the block (box) */
+  double leak;          /* kinetic energy leaving the setup           */
+  double charged_length;/* summed e-/e+ step length (mm)              */
+  int64_t n_secondaries[3]; /* secondaries created per species        */
+  int64_t n_steps;      /* track-steps of this event                  */
+  int64_t n_pairs;      /* gamma conversions                          */
+  int64_t n_annihil;    /* positron annihilations (flight + rest)     */
+} gxt_event_tally;
+
+/* Run statistics. */
+typedef struct gxt_stats {
+  int64_t track_steps;      /* total track-steps                       */
+  int64_t iterations;       /* step-loop iterations                    */
+  int64_t max_live;         /* peak live tracks                        */
+  int64_t killed_by_limit;  /* tracks killed by max_steps              */
+  double wall_ms;           /* device time of the last gxt_run          */
+} gxt_stats;
+
+/* Completed job, as returned by gxt_poll. */
+typedef struct gxt_job_result {
+  uint64_t job_id;
+  int32_t n_events;
+  int32_t pad;
+  double edep;            /* summed over the job's events (MeV)      */
+  double visible;
+  double submit_ms;       /* host clock, ms since context creation  */
+  double admit_ms;
+  double done_ms;
+  int64_t track_steps;
+} gxt_job_result;
+
+typedef struct gxt_ctx gxt_ctx;
+
+/* ---- context ----------------------------------------------------------- */
+/* Replaces ProxyDeviceManager<T>::ProxyDeviceManager + Initialize
+ * (proxy/ProxyDeviceManager.hpp:33-41, proxy/src/ProxyDeviceManager.cpp:41-69):
+ * loads the physics tables (ProxyDataManager.cu:27-122), uploads them and
+ * allocates the track pool. */
+int gxt_create(const gxt_config *cfg, gxt_ctx **out);
+int gxt_destroy(gxt_ctx *ctx);
+const char *gxt_last_error(void);
+const char *gxt_ctx_error(gxt_ctx *ctx);
+
+/* Fill a gxt_config with the defaults (tracking cut 1 keV, 1 cm bins, ...). */
+int gxt_default_config(gxt_config *cfg);
+
+/* ---- primary source ---------------------------------------------------- */
+/* Pencil beam / front-face spot gun (replaces ProxyEgammaGun::GenerateOneEvent,
+ * proxy/src/ProxyEgammaGun.cpp:43-85).  Primary i gets substream first+i;
+ * when emax > emin or spot_mm > 0 its energy and (x,y) come from three draws
+ * of substream n_total + first + i of the same seed, in that order. */
+int gxt_make_primaries(uint32_t seed, int species, double emin, double emax, double spot_mm,
+                       int n, uint64_t first, uint64_t n_total, gxt_primary *out);
+
+/* ---- batch transport --------------------------------------------------- */
+/* Queue a job of n primaries (replaces ProxyDeviceManager::UploadTrackState,
+ * ProxyDeviceManager.cpp:152-204).  Non-blocking; the job is admitted into
+ * the shared pool when its estimated footprint fits. */
+int gxt_submit(gxt_ctx *ctx, const gxt_primary *prim, int n, uint64_t job_id);
+/* Run the step loop until every submitted job has finished (replaces
+ * ProxyDeviceManager::DoStep/DoGPIL, ProxyDeviceManager.cpp:206-276, and the
+ * host wave loop DoStep<>, proxy/test/test_BasicCpuTransport.cpp:517-550).
+ * max_iterations <= 0 means no limit. */
+int gxt_run(gxt_ctx *ctx, int64_t max_iterations);
+/* Completed jobs since the last call (up to max). */
+int gxt_poll(gxt_ctx *ctx, gxt_job_result *out, int max, int *n_out);
+/* Tallies (replaces ProxyDeviceManager::DownloadTrackData,
+ * ProxyDeviceManager.cpp:206-259).  Arrays sized n_zbins, n_rbins and the
+ * number of volumes (1 for the box, 2*n_layers for the calorimeter). */
+int gxt_get_profiles(gxt_ctx *ctx, double *edep_z, double *edep_r, double *edep_volume);
+int gxt_get_event_tallies(gxt_ctx *ctx, uint64_t job_id, gxt_event_tally *out, int n);
+/* Energy histogram of created secondaries: 3 species x 64 bins of log10(E/MeV)
+ * over [-3, 5). */
+int gxt_get_spectra(gxt_ctx *ctx, int64_t *counts);
+int gxt_get_stats(gxt_ctx *ctx, gxt_stats *out);
+int gxt_reset_tallies(gxt_ctx *ctx);
+/* Step trace of traced tracks: records of 12 doubles
+ * {track uid (bits), step, species, process, ekin, x, y, z, dx, dy, dz, edep}.
+ * process: -1 transport only, -2 boundary, -3 escaped, otherwise the discrete
+ * process index of the species. */
+int gxt_get_trace(gxt_ctx *ctx, double *out, int64_t max_records, int64_t *n_out);
+
+/* ---- serving form ------------------------------------------------------ */
+/* Start/stop the serving loop: a host thread that keeps the device step loop
+ * running, admits queued jobs as pool space frees up, and completes jobs when
+ * their last track dies.  gxt_submit stays non-blocking while serving. */
+int gxt_serve_start(gxt_ctx *ctx);
+int gxt_serve_stop(gxt_ctx *ctx);
+/* Host-clock milliseconds since the context was created. */
+double gxt_now_ms(gxt_ctx *ctx);
+
+/* ---- component entry points (batched, SoA, host buffers) --------------- */
+/* MRG32k3a substream states: out[6*i..6*i+5] = state of substream idx[i] of the
+ * stream seeded with seed[0..5] (replaces MRG32k3a::SetNextSubstream /
+ * Initialize, random/src/MRG32k3a.cpp:61-67,73-105). */
+int gxt_rng_init(const uint32_t seed[6], const uint64_t *substreams, int n, uint32_t *states);
+/* k uniforms from each of n states; out[i*k + j]; states advanced in place
+ * (replaces MRG32k3a::Kernel, random/src/MRG32k3a.cpp:123-163). */
+int gxt_rng_draw(uint32_t *states, int n, int k, double *out);
+
+/* Physics-table lookup on the device (replaces ProxyPhysicsVector::Value and
+ * InvRangeValue, proxy/src/ProxyPhysicsVector.cu:469-524, reached through
+ * ProxyDataManager::GetTable()->Value(couple, E)).  table: GXT_TAB_*. */
+#define GXT_TAB_LAMBDA_COMPT 0
+#define GXT_TAB_LAMBDAPRIM_COMPT 1
+#define GXT_TAB_LAMBDA_CONV 2
+#define GXT_TAB_LAMBDAPRIM_PHOT 3
+#define GXT_TAB_LAMBDA_BREM_EM 4
+#define GXT_TAB_LAMBDA_IONI_EM 5
+#define GXT_TAB_DEDX_EM 6
+#define GXT_TAB_RANGE_EM 7
+#define GXT_TAB_INVRANGE_EM 8
+#define GXT_TAB_LAMBDA_BREM_EP 9
+#define GXT_TAB_LAMBDA_IONI_EP 10
+#define GXT_TAB_DEDX_EP 11
+#define GXT_TAB_RANGE_EP 12
+#define GXT_TAB_INVRANGE_EP 13
+#define GXT_NTAB 14
+int gxt_table_value(gxt_ctx *ctx, int material_slot, int table, int precision, const double *x,
+                    double *out, int n);
+/* Macroscopic cross-section (1/mm) of discrete process p of a species at the
+ * given kinetic energies, as the step loop's PostStep GPIL uses it
+ * (replaces ProxyEmProcess::GetLambda, proxy/ProxyEmProcess.hpp:89-91). */
+int gxt_macro_xs(gxt_ctx *ctx, int material_slot, int species, int process, int precision,
+                 const double *ekin, double *out, int n);
+
+/* Final-state samplers (replace ProxyEmModel<T>::SampleSecondaries,
+ * proxy/ProxyEmModel.hpp:57-82, for T = KleinNishina, BetheHeitler,
+ * SauterGavrila, SeltzerBerger, MollerScattering, and add Bhabha and e+
+ * annihilation).  Inputs per track: kinetic energy, direction (3 SoA arrays),
+ * RNG state (6 words, advanced in place).  Outputs per track: primary's new
+ * kinetic energy and direction (ekin 0 = killed), local deposit, and up to
+ * two secondaries (species, ekin, direction); nsec[i] = number produced.
+ * model: GXT_MODEL_*. */
+#define GXT_MODEL_KLEIN_NISHINA 0
+#define GXT_MODEL_BETHE_HEITLER 1
+#define GXT_MODEL_SAUTER_GAVRILA 2
+#define GXT_MODEL_SELTZER_BERGER_EM 3
+#define GXT_MODEL_MOLLER 4
+#define GXT_MODEL_SELTZER_BERGER_EP 5
+#define GXT_MODEL_BHABHA 6
+#define GXT_MODEL_ANNIHIL 7
+typedef struct gxt_sample_io {
+  int n;
+  const double *ekin;        /* [n]      */
+  const double *dir;         /* [3n] SoA: dx[n], dy[n], dz[n] */
+  uint32_t *rng;             /* [6n] AoS: state of track i at rng[6i] */
+  double *out_ekin;          /* [n]      */
+  double *out_dir;           /* [3n] SoA */
+  double *out_edep;          /* [n]      */
+  int32_t *nsec;             /* [n]      */
+  int32_t *sec_species;      /* [2n]: secondary j of track i at [2i+j] */
+  double *sec_ekin;          /* [2n]     */
+  double *sec_dir;           /* [6n]: 3 components of secondary j of track i at [6i+3j] */
+} gxt_sample_io;
+int gxt_sample(gxt_ctx *ctx, int model, int material_slot, int precision, gxt_sample_io *io);
+
+/* Constant-field helix step (replaces ConstFieldHelixStepper::DoStep /
+ * DoStepArr, magneticfield/ConstFieldHelixStepper.hpp:57-86,164-302, in
+ * mm/MeV/tesla instead of cm/GeV/kilogauss).  pos, dir: [3n] SoA, updated in place. */
+int gxt_helix_step(const double field[3], int precision, const double *charge,
+                   const double *momentum, const double *step, double *pos, double *dir, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GXT_H */

@@ -1,0 +1,63 @@
+// gxref.h -- CPU oracle of the EM particle-transport step loop.
+//
+// TEST INFRASTRUCTURE ONLY.  This is a scalar FP64 C++ restatement of the
+// reference's step loop and leaf algorithms (GeantExascalePilot,
+// source/Geant/{random,proxy,magneticfield,core,track}) under the physics
+// contract documented in the repository design notes.  Only tests/, the
+// smoke check and bench.py's CPU-baseline leg may load it, and only as the
+// checker or as the reported CPU baseline; the product library libgxt.so
+// never links, loads or calls it.
+//
+// The C entry points mirror include/gxt.h with a gxref_ prefix so one test
+// harness drives both.
+#ifndef GXREF_H
+#define GXREF_H
+
+#include "../include/gxt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gxref_ctx gxref_ctx;
+
+int gxref_create(const gxt_config *cfg, gxref_ctx **out);
+int gxref_destroy(gxref_ctx *ctx);
+const char *gxref_last_error(void);
+
+int gxref_make_primaries(uint32_t seed, int species, double emin, double emax, double spot_mm, int n,
+                         uint64_t first, uint64_t n_total, gxt_primary *out);
+
+/* Transport n primaries of one job (events numbered by gxt_primary.event)
+ * with nthreads host threads (events handed out dynamically). */
+int gxref_run(gxref_ctx *ctx, const gxt_primary *prim, int n, int nthreads);
+int gxref_get_profiles(gxref_ctx *ctx, double *edep_z, double *edep_r, double *edep_volume);
+int gxref_get_event_tallies(gxref_ctx *ctx, gxt_event_tally *out, int n);
+int gxref_get_spectra(gxref_ctx *ctx, int64_t *counts);
+int gxref_get_stats(gxref_ctx *ctx, gxt_stats *out);
+int gxref_get_trace(gxref_ctx *ctx, double *out, int64_t max_records, int64_t *n_out);
+
+int gxref_rng_init(const uint32_t seed[6], const uint64_t *substreams, int n, uint32_t *states);
+int gxref_rng_draw(uint32_t *states, int n, int k, double *out);
+/* Jump matrices A^(2^(76+j)) for j in [0,64) of both components, row-major,
+ * out[(c*64 + j)*9 + r*3 + col]. */
+int gxref_rng_jump_matrices(uint64_t *out);
+
+int gxref_table_value(gxref_ctx *ctx, int material_slot, int table, const double *x, double *out, int n);
+/* Second derivatives of a table (for checks against an independent spline). */
+int gxref_table_nodes(gxref_ctx *ctx, int material_slot, int table, double *x, double *y, double *d2,
+                      int *n);
+int gxref_macro_xs(gxref_ctx *ctx, int material_slot, int species, int process, const double *ekin,
+                   double *out, int n);
+int gxref_sample(gxref_ctx *ctx, int model, int material_slot, gxt_sample_io *io);
+int gxref_helix_step(const double field[3], const double *charge, const double *momentum,
+                     const double *step, double *pos, double *dir, int n);
+int gxref_rotate_to_lab(double *u, double *v, double *w, const double *u1, const double *u2,
+                        const double *u3, int n);
+/* Sandia photo-absorption macroscopic cross-section (1/mm). */
+int gxref_sandia_xs(gxref_ctx *ctx, int material_slot, const double *ekin, double *out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
