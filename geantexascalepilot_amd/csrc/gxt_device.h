@@ -1,0 +1,911 @@
+// gxt_device.h -- device functions of the step loop for gfx950.
+//
+// Physics state is templated on R: double for the FP64 form, float for the
+// precision form (table interpolation, rejection sampling, helix in FP32;
+// positions, safety, boundary distances and tallies stay FP64).  The
+// algorithms restate the reference's leaf functions (file:line citations are
+// relative to GeantExascalePilot/source/Geant/) with the quirk fixes Q1-Q20
+// of the physics contract; the CPU oracle implements the same contract.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gxt_types.h"
+
+namespace gxt {
+
+constexpr double kMe = 0.510998910;
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 2.0 * kPi;
+constexpr double kFineStructure = 1.0 / 137.0;
+constexpr double kB2C = -0.299792458;  // MeV/c per (tesla mm)
+constexpr double kHuge = 1.0e300;
+constexpr double kAvogadro = 6.02214179e+23;
+constexpr double kClassicRe = 2.8179402894e-12;
+
+// ------------------------------------------------------------------ math
+__device__ __forceinline__ double m_log(double x) { return log(x); }
+__device__ __forceinline__ float m_log(float x) { return logf(x); }
+__device__ __forceinline__ double m_exp(double x) { return exp(x); }
+__device__ __forceinline__ float m_exp(float x) { return expf(x); }
+__device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float m_sqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double m_pow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ float m_pow(float x, float y) { return powf(x, y); }
+__device__ __forceinline__ double m_sin(double x) { return sin(x); }
+__device__ __forceinline__ float m_sin(float x) { return sinf(x); }
+__device__ __forceinline__ double m_cos(double x) { return cos(x); }
+__device__ __forceinline__ float m_cos(float x) { return cosf(x); }
+__device__ __forceinline__ double m_fabs(double x) { return fabs(x); }
+__device__ __forceinline__ float m_fabs(float x) { return fabsf(x); }
+__device__ __forceinline__ double m_log10(double x) { return log10(x); }
+
+template <class R> __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
+template <class R> __device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
+
+// ------------------------------------------------------------------ MRG32k3a
+// L'Ecuyer's recurrence (random/MRG.hpp:35-45, random/src/MRG32k3a.cpp:123-163)
+// on FP64 registers.  a*s < 2^53 is exact; q = floor(p * (1/m)) is exact up to
+// one unit, which the two fix-ups absorb, so the residue is the exact
+// integer residue and the output (one FP64 multiply by norm) is bit-identical
+// to the reference's host branch.
+constexpr double kM1 = 4294967087.0, kM2 = 4294944443.0;
+constexpr double kInvM1 = 1.0 / 4294967087.0, kInvM2 = 1.0 / 4294944443.0;
+constexpr double kA12 = 1403580.0, kA13n = 810728.0, kA21 = 527612.0, kA23n = 1370589.0;
+constexpr double kNorm = 1.0 / (4294967087.0 + 1.0);
+
+struct Rng {
+  double s[6];
+  __device__ __forceinline__ void load(const uint32_t *plane, int64_t cap, int64_t i) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] = (double)plane[k * cap + i];
+  }
+  __device__ __forceinline__ void store(uint32_t *plane, int64_t cap, int64_t i) const {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) plane[k * cap + i] = (uint32_t)s[k];
+  }
+  __device__ __forceinline__ double uniform() {
+    double p1 = kA12 * s[1] - kA13n * s[0];
+    double k1 = floor(p1 * kInvM1);
+    p1 -= k1 * kM1;
+    if (p1 < 0.0) p1 += kM1;
+    if (p1 >= kM1) p1 -= kM1;
+    s[0] = s[1];
+    s[1] = s[2];
+    s[2] = p1;
+    double p2 = kA21 * s[5] - kA23n * s[3];
+    double k2 = floor(p2 * kInvM2);
+    p2 -= k2 * kM2;
+    if (p2 < 0.0) p2 += kM2;
+    if (p2 >= kM2) p2 -= kM2;
+    s[3] = s[4];
+    s[4] = s[5];
+    s[5] = p2;
+    return (p1 > p2) ? (p1 - p2) * kNorm : (p1 - p2 + kM1) * kNorm;
+  }
+  template <class R> __device__ __forceinline__ R u() { return (R)uniform(); }
+};
+
+// (M v) mod m for 32-bit residues: every product < 2^64.
+__device__ __forceinline__ void matvec_mod(const uint32_t *M, uint64_t v[3], uint64_t m) {
+  uint64_t r[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acc += ((uint64_t)M[i * 3 + k] * v[k]) % m;
+    r[i] = acc % m;
+  }
+  v[0] = r[0];
+  v[1] = r[1];
+  v[2] = r[2];
+}
+
+// Substream idx of the stream seeded with seed: (A^(2^76))^idx seed, by
+// binary powering over the jump matrices A^(2^(76+j)).
+__device__ inline void substream_state(const DevParams *P, uint64_t idx, double out[6]) {
+  uint64_t v1[3] = {P->seed[0], P->seed[1], P->seed[2]};
+  uint64_t v2[3] = {P->seed[3], P->seed[4], P->seed[5]};
+  for (int j = 0; j < 64; ++j) {
+    if ((idx >> j) & 1ULL) {
+      matvec_mod(P->jump[0][j], v1, 4294967087ULL);
+      matvec_mod(P->jump[1][j], v2, 4294944443ULL);
+    }
+  }
+  out[0] = (double)v1[0];
+  out[1] = (double)v1[1];
+  out[2] = (double)v1[2];
+  out[3] = (double)v2[0];
+  out[4] = (double)v2[1];
+  out[5] = (double)v2[2];
+}
+
+// Substream index of a secondary: two draws of the creating track, 32 bits each.
+__device__ __forceinline__ uint64_t child_substream(Rng &r) {
+  const uint64_t hi = (uint64_t)(r.uniform() * 4294967296.0);
+  const uint64_t lo = (uint64_t)(r.uniform() * 4294967296.0);
+  return (hi << 32) | lo;
+}
+
+// ------------------------------------------------------------------ tables
+template <class R> struct Nodes;
+template <> struct Nodes<double> {
+  const double *x, *y, *d2;
+  __device__ Nodes(const DevParams *P) : x(P->xd), y(P->yd), d2(P->d2d) {}
+};
+template <> struct Nodes<float> {
+  const float *x, *y, *d2;
+  __device__ Nodes(const DevParams *P) : x(P->xf), y(P->yf), d2(P->d2f) {}
+};
+
+template <class R> __device__ __forceinline__ R tdl(const DevTable &t);
+template <> __device__ __forceinline__ double tdl<double>(const DevTable &t) { return t.dl; }
+template <> __device__ __forceinline__ float tdl<float>(const DevTable &t) { return t.dlf; }
+template <class R> __device__ __forceinline__ R tbase(const DevTable &t);
+template <> __device__ __forceinline__ double tbase<double>(const DevTable &t) { return t.base; }
+template <> __device__ __forceinline__ float tbase<float>(const DevTable &t) { return t.basef; }
+
+// Cubic spline on a bin (ProxyPhysicsVector.cu:396-429).
+template <class R>
+__device__ __forceinline__ R spline(const Nodes<R> &N, int o, R e) {
+  const R x1 = N.x[o], x2 = N.x[o + 1];
+  const R delta = x2 - x1;
+  const R a = (x2 - e) / delta;
+  const R b = (e - x1) / delta;
+  return a * N.y[o] + b * N.y[o + 1] +
+         ((a * a * a - a) * N.d2[o] + (b * b * b - b) * N.d2[o + 1]) * delta * delta / R(6.0);
+}
+
+// ProxyPhysicsVector::Value on a log-binned vector (ProxyPhysicsVector.cu:388-394,469-483).
+template <class R>
+__device__ __forceinline__ R pvec_value(const Nodes<R> &N, const DevTable &t, R e) {
+  const int o = t.off, n = t.n;
+  if (e <= N.x[o]) return N.y[o];
+  if (e >= N.x[o + n - 1]) return N.y[o + n - 1];
+  int bin = (int)(m_log(e) / tdl<R>(t) - tbase<R>(t));
+  if (bin > n - 2) bin = n - 2;
+  if (bin < 0) bin = 0;
+  if (bin > 0 && e < N.x[o + bin]) --bin;
+  return spline(N, o + bin, e);
+}
+
+// InvRangeValue (ProxyPhysicsVector.cu:491-524): binary search (Q7).
+template <class R>
+__device__ __forceinline__ R pvec_inv_value(const Nodes<R> &N, const DevTable &t, R r) {
+  const int o = t.off, n = t.n;
+  if (r <= N.x[o]) return N.y[o];
+  if (r >= N.x[o + n - 1]) return N.y[o + n - 1];
+  int n1 = 0, n2 = n / 2, n3 = n - 1;
+  while (n1 != n3 - 1) {
+    if (r > N.x[o + n2])
+      n1 = n2;
+    else
+      n3 = n2;
+    n2 = n1 + (n3 - n1 + 1) / 2;
+  }
+  return spline(N, o + n1, r);
+}
+
+// Seltzer-Berger grid: x[32], y[57], v[57][32] at offset sb.
+template <class R> __device__ __forceinline__ const R *sb_base(const DevParams *P, int off);
+template <> __device__ __forceinline__ const double *sb_base<double>(const DevParams *P, int off) { return P->sbd + off; }
+template <> __device__ __forceinline__ const float *sb_base<float>(const DevParams *P, int off) { return P->sbf + off; }
+
+// Bin search of Proxy2DVector.cpp:93-139.
+template <class R>
+__device__ __forceinline__ int sb_bin(const R *a, int n, R z) {
+  if (z < a[1]) return 0;
+  if (z >= a[n - 2]) return n - 2;
+  int lo = 0, hi = n - 2;
+  while (lo <= hi) {
+    const int mid = (lo + hi) / 2;
+    if (z < a[mid])
+      hi = mid - 1;
+    else
+      lo = mid + 1;
+  }
+  return hi;
+}
+
+// Proxy2DVector::Value (Proxy2DVector.cpp:37-67).
+template <class R>
+__device__ __forceinline__ R sb_value(const R *g, R x, R y) {
+  const R *gx = g, *gy = g + 32, *gv = g + 32 + 57;
+  if (x < gx[0]) x = gx[0];
+  else if (x > gx[31]) x = gx[31];
+  if (y < gy[0]) y = gy[0];
+  else if (y > gy[56]) y = gy[56];
+  const int ix = sb_bin(gx, 32, x), iy = sb_bin(gy, 57, y);
+  const R x1 = gx[ix], x2 = gx[ix + 1], y1 = gy[iy], y2 = gy[iy + 1];
+  const R v11 = gv[iy * 32 + ix], v12 = gv[iy * 32 + ix + 1];
+  const R v21 = gv[(iy + 1) * 32 + ix], v22 = gv[(iy + 1) * 32 + ix + 1];
+  return ((y2 - y) * (v11 * (x2 - x) + v12 * (x - x1)) + ((y - y1) * (v21 * (x2 - x) + v22 * (x - x1)))) /
+         ((x2 - x1) * (y2 - y1));
+}
+
+// Sandia photo-absorption per atom (mm2), interval = last edge <= E.
+__device__ inline double sandia_atom(const DevParams *P, const DevElement &el, double ekin) {
+  const double ek = ekin * 1.0e3;
+  const double *rows = P->sandia + el.sandia;
+  int row = el.nint - 1;
+  while (row > 0 && ek < rows[5 * row]) --row;
+  const double *c = rows + 5 * row;
+  const double inv = 1.0 / ek;
+  const double s = inv * (c[1] + inv * (c[2] + inv * (c[3] + inv * c[4])));
+  return (el.Z / el.zoa) / kAvogadro * s * 100.0;
+}
+
+__device__ inline double sandia_macro(const DevParams *P, const DevMaterial &m, double ekin) {
+  double s = 0;
+  for (int i = 0; i < m.nel; ++i) s += m.el[i].natoms * sandia_atom(P, m.el[i], ekin);
+  return s;
+}
+
+// Heitler two-photon annihilation cross-section per electron (mm2).
+template <class R>
+__device__ __forceinline__ R heitler_xs(R ekin) {
+  const R tau = ekin / R(kMe);
+  const R gam = tau + R(1.0);
+  const R gamma2 = gam * gam;
+  const R bg2 = tau * (tau + R(2.0));
+  const R bg = m_sqrt(bg2);
+  return R(kPi * kClassicRe * kClassicRe) *
+         ((gamma2 + R(4.0) * gam + R(1.0)) * m_log(gam + bg) - (gam + R(3.0)) * bg) / (bg2 * (gam + R(1.0)));
+}
+
+// Macroscopic cross-section (1/mm) of discrete process p of a species
+// (ProxyEmProcess::GetLambda and the processes' GetLambda; Q1, Q4, Q5).
+template <class R>
+__device__ inline R macro_xs(const DevParams *P, const Nodes<R> &N, const DevMaterial &m, int species, int p, R e) {
+  if (species == 0) {
+    if (p == 0) {
+      if ((double)e < m.compt_split) return pvec_value(N, m.tab[GXT_TAB_LAMBDA_COMPT], e);
+      return pvec_value(N, m.tab[GXT_TAB_LAMBDAPRIM_COMPT], e) / e;
+    }
+    if (p == 1) return pvec_value(N, m.tab[GXT_TAB_LAMBDA_CONV], e);
+    if ((double)e >= m.phot_emin) return pvec_value(N, m.tab[GXT_TAB_LAMBDAPRIM_PHOT], e) / e;
+    return (R)sandia_macro(P, m, (double)e);
+  }
+  const bool pos = species == 2;
+  if (p == 0) return pvec_value(N, m.tab[pos ? GXT_TAB_LAMBDA_BREM_EP : GXT_TAB_LAMBDA_BREM_EM], e);
+  if (p == 1) return pvec_value(N, m.tab[pos ? GXT_TAB_LAMBDA_IONI_EP : GXT_TAB_LAMBDA_IONI_EM], e);
+  return R(m.ne) * heitler_xs(e);
+}
+
+__device__ __forceinline__ int nproc(int species) { return species == 1 ? 2 : 3; }
+
+// ------------------------------------------------------------------ vectors
+template <class R>
+__device__ __forceinline__ void unit3(R &a, R &b, R &c) {
+  const R m2 = a * a + b * b + c * c;
+  if (m2 > R(0)) {
+    const R inv = R(1) / m_sqrt(m2);
+    a *= inv;
+    b *= inv;
+    c *= inv;
+  }
+}
+
+// Math::RotateToLabFrame (core/math_wrappers.hpp:194-212).
+template <class R>
+__device__ __forceinline__ void rotate_to_lab(R &u, R &v, R &w, R u1, R u2, R u3) {
+  R up = u1 * u1 + u2 * u2;
+  if (up > R(0)) {
+    up = m_sqrt(up);
+    const R px = u, py = v, pz = w;
+    u = (u3 * px * u1 - u2 * py) * (R(1) / up) + u1 * pz;
+    v = (u3 * px * u2 + u1 * py) * (R(1) / up) + u2 * pz;
+    w = -up * px + u3 * pz;
+  } else if (u3 < R(0)) {
+    u = -u;
+    w = -w;
+  }
+}
+
+// ------------------------------------------------------------------ samplers
+template <class R>
+struct Sec {
+  int species;
+  R e;
+  R d[3];
+};
+
+template <class R>
+struct Interaction {
+  R ekin;  // primary after the interaction (0 = killed)
+  R d[3];
+  R edep;
+  int nsec;
+  Sec<R> sec[2];
+};
+
+template <class R>
+__device__ __forceinline__ void ia_init(Interaction<R> &o, R e, const R d[3], R ekin_out) {
+  o.nsec = 0;
+  o.edep = R(0);
+  o.ekin = ekin_out;
+  o.d[0] = d[0];
+  o.d[1] = d[1];
+  o.d[2] = d[2];
+  (void)e;
+}
+
+// Element choice for compounds (Q2): kind 0 = n Z(Z+1), kind 1 = partial Sandia.
+template <class R>
+__device__ inline int select_element(const DevParams *P, const DevMaterial &m, int kind, R e, Rng &rng) {
+  if (m.nel == 1) return 0;
+  double w[kMaxElem], tot = 0;
+  for (int i = 0; i < m.nel; ++i) {
+    w[i] = kind == 0 ? m.el[i].wbrem : m.el[i].natoms * sandia_atom(P, m.el[i], (double)e);
+    tot += w[i];
+  }
+  const double r = rng.uniform() * tot;
+  double acc = 0;
+  for (int i = 0; i < m.nel - 1; ++i) {
+    acc += w[i];
+    if (r < acc) return i;
+  }
+  return m.nel - 1;
+}
+
+// Klein-Nishina (proxy/ProxyKleinNishina.hpp:96-166).
+template <class R>
+__device__ inline void sample_compton(R e0, const R d0[3], Rng &rng, Interaction<R> &o) {
+  const R lowE = R(100.0e-6);
+  ia_init(o, e0, d0, e0);
+  if (e0 < lowE) return;
+  const R E0_m = e0 / R(kMe);
+  const R eps0 = R(1) / (R(1) + R(2) * E0_m);
+  const R eps0sq = eps0 * eps0;
+  const R alpha1 = -m_log(eps0);
+  const R alpha2 = R(0.5) * (R(1) - eps0sq);
+  R eps, epssq, onecost, sint2, greject;
+  do {
+    if (alpha1 / (alpha1 + alpha2) > rng.template u<R>()) {
+      eps = m_exp(-alpha1 * rng.template u<R>());
+      epssq = eps * eps;
+    } else {
+      epssq = eps0sq + (R(1) - eps0sq) * rng.template u<R>();
+      eps = m_sqrt(epssq);
+    }
+    onecost = (R(1) - eps) / (eps * E0_m);
+    sint2 = onecost * (R(2) - onecost);
+    greject = R(1) - eps * sint2 / (R(1) + epssq);
+  } while (greject < rng.template u<R>());
+  const R sinT = sint2 < R(0) ? R(0) : m_sqrt(sint2);
+  const R phi = R(kTwoPi) * rng.template u<R>();
+  const R e1 = eps * e0;
+  R g0 = sinT * m_cos(phi), g1 = sinT * m_sin(phi), g2 = R(1) - onecost;
+  rotate_to_lab(g0, g1, g2, d0[0], d0[1], d0[2]);
+  const R te = e0 - e1;
+  if (e1 > lowE) {
+    o.ekin = e1;
+    o.d[0] = g0;
+    o.d[1] = g1;
+    o.d[2] = g2;
+    Sec<R> &el = o.sec[o.nsec++];
+    el.species = 1;
+    el.e = te;
+    el.d[0] = e0 * d0[0] - e1 * g0;
+    el.d[1] = e0 * d0[1] - e1 * g1;
+    el.d[2] = e0 * d0[2] - e1 * g2;
+    unit3(el.d[0], el.d[1], el.d[2]);
+  } else {
+    o.ekin = R(0);
+    o.edep = e1 + te;
+  }
+}
+
+template <class R> __device__ __forceinline__ R screen1(R x) {
+  return x > R(1) ? R(42.24) - R(8.368) * m_log(x + R(0.952)) : R(42.392) - x * (R(7.796) - R(1.961) * x);
+}
+template <class R> __device__ __forceinline__ R screen2(R x) {
+  return x > R(1) ? R(42.24) - R(8.368) * m_log(x + R(0.952)) : R(41.405) - x * (R(5.828) - R(0.8945) * x);
+}
+template <class R> __device__ __forceinline__ R coulomb_factor(R Z) {
+  const R k1 = R(0.0083), k2 = R(0.20206), k3 = R(0.0020), k4 = R(0.0369);
+  const R az1 = R(kFineStructure) * Z;
+  const R az2 = az1 * az1;
+  const R az4 = az2 * az2;
+  return (k1 * az4 + k2 + R(1) / (R(1) + az2)) * az2 - (k3 * az4 + k4) * az4;
+}
+
+// Bethe-Heitler (proxy/ProxyBetheHeitler.hpp:64-182; Q2, Q19).
+template <class R>
+__device__ inline void sample_pair(const DevParams *P, const DevMaterial &m, R e0, const R d0[3], Rng &rng,
+                                   Interaction<R> &o) {
+  ia_init(o, e0, d0, e0);
+  const R eps0 = R(kMe) / e0;
+  if (eps0 > R(0.5)) return;
+  const int ie = select_element(P, m, 0, e0, rng);
+  const R Z = (R)m.el[ie].Z;
+  R eps;
+  if (e0 < R(2.0)) {
+    eps = eps0 + (R(0.5) - eps0) * rng.template u<R>();
+  } else {
+    const R logZ3 = m_log(Z) / R(3.0);
+    R FZ = R(8.0) * logZ3;
+    if (e0 > R(50.0)) FZ += R(8.0) * coulomb_factor(Z);
+    const R Z3 = m_pow(Z, R(1.0) / R(3.0));
+    const R screenfac = R(136.0) * eps0 / Z3;
+    const R screenmax = m_exp((R(42.24) - FZ) / R(8.368)) - R(0.952);
+    const R screenmin = rmin(R(4.0) * screenfac, screenmax);
+    const R eps1 = R(0.5) - R(0.5) * m_sqrt(R(1) - screenmin / screenmax);
+    const R epsmin = rmax(eps0, eps1);
+    const R epsrange = R(0.5) - epsmin;
+    const R F10 = screen1(screenmin) - FZ;
+    const R F20 = screen2(screenmin) - FZ;
+    const R NormF1 = rmax(F10 * epsrange * epsrange, R(0));
+    const R NormF2 = rmax(R(1.5) * F20, R(0));
+    R greject;
+    do {
+      if (NormF1 / (NormF1 + NormF2) > rng.template u<R>()) {
+        eps = R(0.5) - epsrange * m_pow(rng.template u<R>(), R(0.333333));
+        const R sv = screenfac / (eps * (R(1) - eps));
+        greject = (screen1(sv) - FZ) / F10;
+      } else {
+        eps = epsmin + epsrange * rng.template u<R>();
+        const R sv = screenfac / (eps * (R(1) - eps));
+        greject = (screen2(sv) - FZ) / F20;
+      }
+    } while (greject < rng.template u<R>());
+  }
+  const R eTot = (rng.template u<R>() > R(0.5)) ? (R(1) - eps) * e0 : eps * e0;
+  const R pTot = e0 - eTot;
+  const R r1 = rng.template u<R>();
+  const R r2 = rng.template u<R>();
+  const R aa0 = -m_log(r1 * r2);
+  const R u = (R(0.25) > rng.template u<R>()) ? aa0 / R(0.625) : aa0 / R(1.875);
+  const R phi = R(kTwoPi) * rng.template u<R>();
+  const R cosp = m_cos(phi), sinp = m_sin(phi);
+  const R the = u * R(kMe) / eTot, thp = u * R(kMe) / pTot;
+  const R ste = m_sin(the), cte = m_cos(the);
+  const R stp = m_sin(thp), ctp = m_cos(thp);
+  Sec<R> &el = o.sec[0];
+  el.species = 1;
+  el.e = eTot - R(kMe);
+  el.d[0] = ste * cosp;
+  el.d[1] = ste * sinp;
+  el.d[2] = cte;
+  rotate_to_lab(el.d[0], el.d[1], el.d[2], d0[0], d0[1], d0[2]);
+  Sec<R> &po = o.sec[1];
+  po.species = 2;
+  po.e = pTot - R(kMe);
+  po.d[0] = -stp * cosp;
+  po.d[1] = -stp * sinp;
+  po.d[2] = ctp;
+  rotate_to_lab(po.d[0], po.d[1], po.d[2], d0[0], d0[1], d0[2]);
+  o.nsec = 2;
+  o.ekin = R(0);
+}
+
+// Sauter-Gavrila (proxy/ProxySauterGavrila.hpp:66-130; Q2, Q12, Q17).
+template <class R>
+__device__ inline void sample_photo(const DevParams *P, const DevMaterial &m, R e0, const R d0[3], Rng &rng,
+                                    Interaction<R> &o) {
+  ia_init(o, e0, d0, R(0));
+  const int ie = select_element(P, m, 1, e0, rng);
+  const DevElement &el = m.el[ie];
+  R bind = R(0);
+  for (int i = 0; i < el.nshell; ++i) {
+    const R b = (R)P->shells[el.shell + i];
+    if (e0 >= b) {
+      bind = b;
+      break;
+    }
+  }
+  const R te = e0 - bind;
+  const R tau = e0 / R(kMe);
+  R cost;
+  if (tau > R(50.0)) {
+    cost = R(1);
+  } else {
+    const R gamma = tau + R(1);
+    const R beta = m_sqrt(tau * (tau + R(2))) / gamma;
+    const R A = (R(1) - beta) / beta;
+    const R Ap2 = A + R(2);
+    const R B = R(0.5) * beta * gamma * (gamma - R(1)) * (gamma - R(2));
+    const R grej = R(2) * (R(1) + A * B) / A;
+    R z, g;
+    do {
+      const R q = rng.template u<R>();
+      z = R(2) * A * (R(2) * q + Ap2 * m_sqrt(q)) / (Ap2 * Ap2 - R(4) * q);
+      g = (R(2) - z) * (R(1) / (A + z) + B);
+    } while (g < rng.template u<R>() * grej);
+    cost = R(1) - z;
+  }
+  const R sint = m_sqrt((R(1) + cost) * (R(1) - cost));
+  const R phi = R(kTwoPi) * rng.template u<R>();
+  Sec<R> &e = o.sec[0];
+  e.species = 1;
+  e.e = te;
+  e.d[0] = sint * m_cos(phi);
+  e.d[1] = sint * m_sin(phi);
+  e.d[2] = cost;
+  rotate_to_lab(e.d[0], e.d[1], e.d[2], d0[0], d0[1], d0[2]);
+  o.nsec = 1;
+  o.edep = bind;
+}
+
+// Seltzer-Berger (proxy/ProxySeltzerBerger.hpp:204-315; Q2, Q3, Q15, Q16, Q20).
+template <class R>
+__device__ inline void sample_brems(const DevParams *P, const DevMaterial &m, R t0, const R d0[3], Rng &rng,
+                                    Interaction<R> &o) {
+  ia_init(o, t0, d0, t0);
+  const R cut = rmin((R)m.cut_gamma, t0);
+  const R emax = t0;
+  if (cut >= emax) return;
+  const int ie = select_element(P, m, 0, t0, rng);
+  const R *sb = sb_base<R>(P, m.el[ie].sb);
+  const R etot = t0 + R(kMe);
+  const R dcorr = (R)m.migdal * etot * etot;
+  const R xmin = m_log(cut * cut + dcorr);
+  const R xmax = m_log(emax * emax + dcorr);
+  const R y = m_log(t0);
+  const R x0 = cut / t0;
+  R vmax = sb_value(sb, x0, y) * R(1.02);
+  const R epeak = R(300.0), elow = R(10.0e-3);
+  if (x0 < R(0.97) && (t0 > epeak || t0 < elow)) {
+    const R ylim = rmin(sb_value(sb, R(0.97), R(4.0) * m_log(R(10.0))), R(1.1) * sb_value(sb, R(0.97), y));
+    if (ylim > vmax) vmax = ylim;
+  }
+  if (x0 < R(0.05)) vmax *= R(1.2);
+  R k, v;
+  do {
+    const R r = rng.template u<R>();
+    R x = m_exp(xmin + r * (xmax - xmin)) - dcorr;
+    if (x < R(0)) x = R(0);
+    k = m_sqrt(x);
+    v = sb_value(sb, k / t0, y);
+  } while (v < vmax * rng.template u<R>());
+  const R c = R(4.0) - R(8.0) * rng.template u<R>();
+  const R signc = c < R(0) ? R(-1) : R(1);
+  const R a = m_fabs(c);
+  const R delta = R(0.5) * (m_sqrt(R(4.0) + a * a) + a);
+  const R cofA = -signc * m_pow(delta, R(1.0) / R(3.0));
+  R cost = cofA - R(1) / cofA;
+  const R tau = t0 / R(kMe);
+  const R beta = m_sqrt(tau * (tau + R(2))) / (tau + R(1));
+  cost = (cost + beta) / (R(1) + cost * beta);
+  const R sint = m_sqrt((R(1) - cost) * (R(1) + cost));
+  const R phi = R(kTwoPi) * rng.template u<R>();
+  Sec<R> &g = o.sec[0];
+  g.species = 0;
+  g.e = k;
+  g.d[0] = sint * m_cos(phi);
+  g.d[1] = sint * m_sin(phi);
+  g.d[2] = cost;
+  rotate_to_lab(g.d[0], g.d[1], g.d[2], d0[0], d0[1], d0[2]);
+  o.nsec = 1;
+  const R ptot = m_sqrt(t0 * (etot + R(kMe)));
+  o.ekin = t0 - k;
+  o.d[0] = ptot * d0[0] - k * g.d[0];
+  o.d[1] = ptot * d0[1] - k * g.d[1];
+  o.d[2] = ptot * d0[2] - k * g.d[2];
+  unit3(o.d[0], o.d[1], o.d[2]);
+}
+
+// Moller / Bhabha (proxy/ProxyMollerScattering.hpp:92-159; Q6; Bhabha branch A12).
+template <class R>
+__device__ inline void sample_ioni(const DevMaterial &m, bool positron, R t0, const R d0[3], Rng &rng,
+                                   Interaction<R> &o) {
+  ia_init(o, t0, d0, t0);
+  const R tmin = (R)m.cut_e;
+  const R tmax = positron ? t0 : R(0.5) * t0;
+  if (tmin >= tmax) return;
+  const R energy = t0 + R(kMe);
+  const R xmin = tmin / t0, xmax = tmax / t0;
+  const R gam = energy / R(kMe);
+  const R gamma2 = gam * gam;
+  R x, z, grej;
+  if (!positron) {
+    const R gg = (R(2) * gam - R(1)) / gamma2;
+    R y = R(1) - xmax;
+    grej = R(1) - gg * xmax + xmax * xmax * (R(1) - gg + (R(1) - gg * y) / (y * y));
+    do {
+      const R r = rng.template u<R>();
+      x = xmin * xmax / (xmin * (R(1) - r) + xmax * r);
+      y = R(1) - x;
+      z = R(1) - gg * x + x * x * (R(1) - gg + (R(1) - gg * y) / (y * y));
+    } while (grej * rng.template u<R>() > z);
+  } else {
+    const R beta2 = R(1) - R(1) / gamma2;
+    const R yb = R(1) / (R(1) + gam);
+    const R y2 = yb * yb;
+    const R y12 = R(1) - R(2) * yb;
+    const R b1 = R(2) - y2;
+    const R b2 = y12 * (R(3) + y2);
+    const R y122 = y12 * y12;
+    const R b4 = y122 * y12;
+    const R b3 = b4 + y122;
+    const R xm2 = xmax * xmax;
+    grej = R(1) + (xm2 * xm2 * b4 - xmin * xmin * xmin * b3 + xm2 * b2 - xmin * b1) * beta2;
+    do {
+      const R r = rng.template u<R>();
+      x = xmin * xmax / (xmin * (R(1) - r) + xmax * r);
+      const R x2 = x * x;
+      z = R(1) + (x2 * x2 * b4 - x * x2 * b3 + x2 * b2 - x * b1) * beta2;
+    } while (grej * rng.template u<R>() > z);
+  }
+  const R dT = x * t0;
+  const R ptot = m_sqrt(t0 * (t0 + R(2) * R(kMe)));
+  const R pd = m_sqrt(dT * (dT + R(2) * R(kMe)));
+  R cost = dT * (energy + R(kMe)) / (pd * ptot);
+  if (cost > R(1)) cost = R(1);
+  const R sint = m_sqrt((R(1) - cost) * (R(1) + cost));
+  const R phi = R(kTwoPi) * rng.template u<R>();
+  Sec<R> &d = o.sec[0];
+  d.species = 1;
+  d.e = dT;
+  d.d[0] = sint * m_cos(phi);
+  d.d[1] = sint * m_sin(phi);
+  d.d[2] = cost;
+  rotate_to_lab(d.d[0], d.d[1], d.d[2], d0[0], d0[1], d0[2]);
+  o.nsec = 1;
+  o.ekin = t0 - dT;
+  o.d[0] = ptot * d0[0] - pd * d.d[0];
+  o.d[1] = ptot * d0[1] - pd * d.d[1];
+  o.d[2] = ptot * d0[2] - pd * d.d[2];
+  unit3(o.d[0], o.d[1], o.d[2]);
+}
+
+// e+ annihilation in flight (Heitler; A12).
+template <class R>
+__device__ inline void sample_annihil(R t0, const R d0[3], Rng &rng, Interaction<R> &o) {
+  ia_init(o, t0, d0, R(0));
+  const R tau = t0 / R(kMe);
+  const R gam = tau + R(1);
+  const R tau2 = tau + R(2);
+  const R sqgrate = m_sqrt(tau / tau2) * R(0.5);
+  const R sqg2m1 = m_sqrt(tau * tau2);
+  const R epsmin = R(0.5) - sqgrate;
+  const R epsmax = R(0.5) + sqgrate;
+  const R epsqot = epsmax / epsmin;
+  R eps, greject;
+  do {
+    eps = epsmin * m_exp(m_log(epsqot) * rng.template u<R>());
+    greject = R(1) - eps + (R(2) * gam * eps - R(1)) / (eps * tau2 * tau2);
+  } while (greject < rng.template u<R>());
+  R cost = (eps * tau2 - R(1)) / (eps * sqg2m1);
+  if (cost > R(1)) cost = R(1);
+  if (cost < R(-1)) cost = R(-1);
+  const R sint = m_sqrt((R(1) + cost) * (R(1) - cost));
+  const R phi = R(kTwoPi) * rng.template u<R>();
+  const R etot = t0 + R(2) * R(kMe);
+  Sec<R> &g1 = o.sec[0];
+  g1.species = 0;
+  g1.e = eps * etot;
+  g1.d[0] = sint * m_cos(phi);
+  g1.d[1] = sint * m_sin(phi);
+  g1.d[2] = cost;
+  rotate_to_lab(g1.d[0], g1.d[1], g1.d[2], d0[0], d0[1], d0[2]);
+  Sec<R> &g2 = o.sec[1];
+  g2.species = 0;
+  g2.e = (R(1) - eps) * etot;
+  const R pp = m_sqrt(t0 * (t0 + R(2) * R(kMe)));
+  g2.d[0] = pp * d0[0] - g1.e * g1.d[0];
+  g2.d[1] = pp * d0[1] - g1.e * g1.d[1];
+  g2.d[2] = pp * d0[2] - g1.e * g1.d[2];
+  unit3(g2.d[0], g2.d[1], g2.d[2]);
+  o.nsec = 2;
+}
+
+template <class R>
+__device__ inline void sample_model(const DevParams *P, const DevMaterial &m, int model, R e, const R d[3], Rng &rng,
+                                    Interaction<R> &o) {
+  switch (model) {
+    case GXT_MODEL_KLEIN_NISHINA: sample_compton(e, d, rng, o); break;
+    case GXT_MODEL_BETHE_HEITLER: sample_pair(P, m, e, d, rng, o); break;
+    case GXT_MODEL_SAUTER_GAVRILA: sample_photo(P, m, e, d, rng, o); break;
+    case GXT_MODEL_SELTZER_BERGER_EM:
+    case GXT_MODEL_SELTZER_BERGER_EP: sample_brems(P, m, e, d, rng, o); break;
+    case GXT_MODEL_MOLLER: sample_ioni(m, false, e, d, rng, o); break;
+    case GXT_MODEL_BHABHA: sample_ioni(m, true, e, d, rng, o); break;
+    default: sample_annihil(e, d, rng, o); break;
+  }
+}
+
+// queue index -> species, process, model
+__device__ __forceinline__ int queue_species(int q) { return q < 3 ? 0 : (q < 5 ? 1 : 2); }
+__device__ __forceinline__ int queue_proc(int q) { return q < 3 ? q : (q < 5 ? q - 3 : q - 5); }
+__device__ __forceinline__ int queue_of(int species, int p) { return species == 0 ? p : (species == 1 ? 3 + p : 5 + p); }
+__device__ __forceinline__ int queue_model(int q) {
+  // gamma: KN, BH, SG; e-: SB, Moller; e+: SB, Bhabha, annihilation
+  return q == 0 ? GXT_MODEL_KLEIN_NISHINA
+                : q == 1 ? GXT_MODEL_BETHE_HEITLER
+                         : q == 2 ? GXT_MODEL_SAUTER_GAVRILA
+                                  : q == 3 ? GXT_MODEL_SELTZER_BERGER_EM
+                                           : q == 4 ? GXT_MODEL_MOLLER
+                                                    : q == 5 ? GXT_MODEL_SELTZER_BERGER_EP
+                                                             : q == 6 ? GXT_MODEL_BHABHA : GXT_MODEL_ANNIHIL;
+}
+
+// ------------------------------------------------------------------ continuous loss
+// ProxyIonization::AlongStepGPIL (ProxyIonization.hpp:108-128; Q6, Q18).
+template <class R>
+__device__ __forceinline__ R along_limit(const Nodes<R> &N, const DevMaterial &m, bool pos, R e, R mine) {
+  R range = pvec_value(N, m.tab[pos ? GXT_TAB_RANGE_EP : GXT_TAB_RANGE_EM], e);
+  if (e < mine) range *= e / mine;
+  const R finR = rmin(R(10.0), (R)m.cut_length);
+  return range > finR ? range * R(0.2) + finR * R(0.8) * (R(2.0) - finR / range) : range;
+}
+
+// ProxyIonization::AlongStepDoIt (ProxyIonization.hpp:130-168; Q7).
+template <class R>
+__device__ __forceinline__ R along_doit(const Nodes<R> &N, const DevMaterial &m, bool pos, R e, R step, R mine,
+                                        R &eloss) {
+  const R range = pvec_value(N, m.tab[pos ? GXT_TAB_RANGE_EP : GXT_TAB_RANGE_EM], e);
+  if (step >= range || e <= mine) {
+    eloss = e;
+    return R(0);
+  }
+  const R dedx = pvec_value(N, m.tab[pos ? GXT_TAB_DEDX_EP : GXT_TAB_DEDX_EM], e);
+  R loss = step * dedx;
+  if (loss > e * R(0.01))
+    loss = e - pvec_inv_value(N, m.tab[pos ? GXT_TAB_INVRANGE_EP : GXT_TAB_INVRANGE_EM], range - step);
+  R fin = e - loss;
+  if (fin < mine) {
+    loss += fin;
+    fin = R(0);
+  }
+  eloss = loss < R(0) ? R(0) : loss;
+  return fin;
+}
+
+// ------------------------------------------------------------------ geometry
+// Analytic slab/box navigator (replaces the VecGeom navigation of
+// geometry/NavigationInterface.hpp:42-98).  Positions and distances are FP64.
+__device__ inline int locate(const DevGeometry &g, const double p[3], double d0, double d1, double d2) {
+  const double ax = fabs(p[0]), ay = fabs(p[1]);
+  if (ax > g.hx || ay > g.hy) return -1;
+  if (ax == g.hx && p[0] * d0 > 0) return -1;
+  if (ay == g.hy && p[1] * d1 > 0) return -1;
+  const int nv = g.nvol;
+  const double z = p[2];
+  if (z < g.z[0] || z > g.z[nv]) return -1;
+  if (z == g.z[0] && d2 < 0) return -1;
+  if (z == g.z[nv] && d2 > 0) return -1;
+  int lo = 0, hi = nv - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (g.z[mid] <= z)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  if (z == g.z[lo] && lo > 0 && d2 < 0) --lo;
+  return lo;
+}
+
+__device__ __forceinline__ double dist_out(const DevGeometry &g, int v, const double p[3], double d0, double d1,
+                                           double d2, int &face) {
+  double s = kHuge;
+  face = -1;
+  if (d2 > 0) {
+    s = (g.z[v + 1] - p[2]) / d2;
+    face = 1;
+  } else if (d2 < 0) {
+    s = (g.z[v] - p[2]) / d2;
+    face = 0;
+  }
+  if (d0 != 0) {
+    const double sx = ((d0 > 0 ? g.hx : -g.hx) - p[0]) / d0;
+    if (sx < s) {
+      s = sx;
+      face = 2;
+    }
+  }
+  if (d1 != 0) {
+    const double sy = ((d1 > 0 ? g.hy : -g.hy) - p[1]) / d1;
+    if (sy < s) {
+      s = sy;
+      face = 3;
+    }
+  }
+  return s < 0 ? 0.0 : s;
+}
+
+__device__ __forceinline__ double safety(const DevGeometry &g, int v, const double p[3]) {
+  double s = fmin(p[2] - g.z[v], g.z[v + 1] - p[2]);
+  s = fmin(s, g.hx - fabs(p[0]));
+  s = fmin(s, g.hy - fabs(p[1]));
+  return s < 0 ? 0.0 : s;
+}
+
+// ConstFieldHelixStepper::DoStep (magneticfield/ConstFieldHelixStepper.hpp:164-239)
+// in mm/MeV/tesla; the direction is renormalised (FieldPropagationHandler.cpp:323-330).
+// The arithmetic is in R; the position update is accumulated in FP64.
+template <class R>
+__device__ inline void helix_step(const double B[3], double bmag, R charge, R mom, R step, double pos[3], R dir[3]) {
+  const R ub[3] = {(R)(B[0] / bmag), (R)(B[1] / bmag), (R)(B[2] / bmag)};
+  const R uvb = dir[0] * ub[0] + dir[1] * ub[1] + dir[2] * ub[2];
+  R dt2 = dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2] - uvb * uvb;
+  if (dt2 < R(0)) dt2 = R(0);
+  const R sinvb = m_sqrt(dt2) + R(1.0e-30);
+  const R Rad = mom * sinvb / (R(kB2C) * charge * (R)bmag);
+  R vx[3] = {dir[0] - uvb * ub[0], dir[1] - uvb * ub[1], dir[2] - uvb * ub[2]};
+  unit3(vx[0], vx[1], vx[2]);
+  const R cvb[3] = {vx[1] * ub[2] - vx[2] * ub[1], vx[2] * ub[0] - vx[0] * ub[2], vx[0] * ub[1] - vx[1] * ub[0]};
+  const R phi = -step * charge * (R)bmag * R(kB2C) / mom;
+  const R sphi = m_sin(phi), cphi = m_cos(phi);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    pos[i] = pos[i] + (double)(Rad * (cphi - R(1)) * cvb[i]) - (double)(Rad * sphi * vx[i]) +
+             (double)(step * uvb * ub[i]);
+    dir[i] = uvb * ub[i] + cphi * sinvb * vx[i] + sphi * sinvb * cvb[i];
+  }
+  unit3(dir[0], dir[1], dir[2]);
+}
+
+enum { kPhysics = 0, kBoundary = 1, kEscaped = 2, kLimit = 3 };
+
+// Propagation inside the current volume to pstep or the boundary
+// (FieldPropagationHandler.cpp:87-360; LinearPropagationHandler.cpp:13-63;
+// driver loop test_BasicCpuTransport.cpp:319-336).
+template <class R>
+__device__ inline int propagate(const DevParams *P, int species, R e, double pos[3], R dir[3], int &vol, double pstep,
+                                double &step) {
+  const DevGeometry &g = P->geo;
+  if (species == 0 || P->bmag == 0.0) {
+    int face;
+    const double d0 = dir[0], d1 = dir[1], d2 = dir[2];
+    const double sn = dist_out(g, vol, pos, d0, d1, d2, face);
+    if (pstep < sn) {
+      pos[0] += pstep * d0;
+      pos[1] += pstep * d1;
+      pos[2] += pstep * d2;
+      step = pstep;
+      return kPhysics;
+    }
+    pos[0] += sn * d0;
+    pos[1] += sn * d1;
+    pos[2] += sn * d2;
+    step = sn;
+    if (face == 0 || face == 1) {
+      const int nv = face == 1 ? vol + 1 : vol - 1;
+      pos[2] = g.z[face == 1 ? vol + 1 : vol];
+      if (nv < 0 || nv >= g.nvol) return kEscaped;
+      vol = nv;
+      return kBoundary;
+    }
+    return kEscaped;
+  }
+  const R q = species == 1 ? R(-1) : R(1);
+  const R mom = m_sqrt(e * (e + R(2) * R(kMe)));
+  const R ub[3] = {(R)(P->B[0] / P->bmag), (R)(P->B[1] / P->bmag), (R)(P->B[2] / P->bmag)};
+  double remaining = pstep;
+  step = 0.0;
+  for (int it = 0; it < 64; ++it) {
+    const double saf = safety(g, vol, pos);
+    if (remaining <= saf) {
+      helix_step<R>(P->B, P->bmag, q, mom, (R)remaining, pos, dir);
+      step += remaining;
+      return kPhysics;
+    }
+    int face;
+    const double slin = dist_out(g, vol, pos, (double)dir[0], (double)dir[1], (double)dir[2], face);
+    const R pl = mom * (dir[0] * ub[0] + dir[1] * ub[1] + dir[2] * ub[2]);
+    R pt2 = mom * mom - pl * pl;
+    if (pt2 < R(0)) pt2 = R(0);
+    const R pt = m_sqrt(pt2);
+    const double sl = pt > R(0) ? (double)(R(2) * m_sqrt(R(0.1) / (R(0.299792458) * (R)P->bmag / pt))) : kHuge;
+    double s;
+    if (slin <= sl && slin <= remaining)
+      s = fmax(slin, 1.0e-6);
+    else
+      s = fmin(remaining, fmax(sl, saf));
+    helix_step<R>(P->B, P->bmag, q, mom, (R)s, pos, dir);
+    step += s;
+    remaining -= s;
+    const int v2 = locate(g, pos, (double)dir[0], (double)dir[1], (double)dir[2]);
+    if (v2 != vol) {
+      if (v2 < 0) return kEscaped;
+      vol = v2;
+      return kBoundary;
+    }
+    if (remaining <= 0.0) return kPhysics;
+  }
+  return kLimit;
+}
+
+}  // namespace gxt

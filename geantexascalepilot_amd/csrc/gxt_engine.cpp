@@ -1,0 +1,983 @@
+// gxt_engine.cpp -- host driver of the step loop and the C ABI of libgxt.so.
+//
+// One gxt_ctx owns one HIP stream, the device copy of the physics data, a
+// double-buffered SoA track pool, the interaction queues and the tallies.
+// Jobs (groups of primaries) are admitted into the shared pool by estimated
+// footprint; the device step loop runs several iterations between host
+// synchronisations; a job completes when its live-track counter reaches 0.
+// This replaces ProxyDeviceManager (proxy/src/ProxyDeviceManager.cpp:41-276)
+// and the host wave loop DoStep<> (proxy/test/test_BasicCpuTransport.cpp:517-550).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/gxt.h"
+#include "gxt_data.h"
+#include "gxt_kernels.h"
+
+using namespace gxt;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_err(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return set_err(GXT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+hipError_t dalloc(T **p, size_t n) {
+  if (n == 0) n = 1;
+  return hipMalloc((void **)p, n * sizeof(T));
+}
+
+struct DeviceData {
+  DevParams *P = nullptr;
+  double *xd = nullptr, *yd = nullptr, *d2d = nullptr, *sbd = nullptr, *sandia = nullptr, *shells = nullptr;
+  float *xf = nullptr, *yf = nullptr, *d2f = nullptr, *sbf = nullptr;
+  void release() {
+    (void)hipFree(P);
+    (void)hipFree(xd);
+    (void)hipFree(yd);
+    (void)hipFree(d2d);
+    (void)hipFree(sbd);
+    (void)hipFree(sandia);
+    (void)hipFree(shells);
+    (void)hipFree(xf);
+    (void)hipFree(yf);
+    (void)hipFree(d2f);
+    (void)hipFree(sbf);
+    *this = DeviceData();
+  }
+};
+
+// Upload tables, grids and parameters; fills hd.params' device pointers.
+int upload(HostData &hd, DeviceData &dd) {
+  auto tof = [](const std::vector<double> &v) {
+    std::vector<float> f(v.size());
+    for (size_t i = 0; i < v.size(); ++i) f[i] = (float)v[i];
+    return f;
+  };
+  auto put = [](auto **dst, const auto &v) -> hipError_t {
+    hipError_t e = dalloc(dst, v.size());
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice);
+  };
+  HIP_TRY(put(&dd.xd, hd.x));
+  HIP_TRY(put(&dd.yd, hd.y));
+  HIP_TRY(put(&dd.d2d, hd.d2));
+  HIP_TRY(put(&dd.xf, tof(hd.x)));
+  HIP_TRY(put(&dd.yf, tof(hd.y)));
+  HIP_TRY(put(&dd.d2f, tof(hd.d2)));
+  HIP_TRY(put(&dd.sbd, hd.sb));
+  HIP_TRY(put(&dd.sbf, tof(hd.sb)));
+  HIP_TRY(put(&dd.sandia, hd.sandia));
+  HIP_TRY(put(&dd.shells, hd.shells));
+  DevParams &p = hd.params;
+  p.xd = dd.xd;
+  p.yd = dd.yd;
+  p.d2d = dd.d2d;
+  p.xf = dd.xf;
+  p.yf = dd.yf;
+  p.d2f = dd.d2f;
+  p.sbd = dd.sbd;
+  p.sbf = dd.sbf;
+  p.sandia = dd.sandia;
+  p.shells = dd.shells;
+  HIP_TRY(dalloc(&dd.P, 1));
+  HIP_TRY(hipMemcpy(dd.P, &p, sizeof(p), hipMemcpyHostToDevice));
+  return GXT_OK;
+}
+
+template <class R>
+struct DevPool {
+  Pool<R> p{};
+  int alloc(int64_t cap, bool trace) {
+    p.cap = cap;
+    HIP_TRY(dalloc(&p.x, cap));
+    HIP_TRY(dalloc(&p.y, cap));
+    HIP_TRY(dalloc(&p.z, cap));
+    HIP_TRY(dalloc(&p.dx, cap));
+    HIP_TRY(dalloc(&p.dy, cap));
+    HIP_TRY(dalloc(&p.dz, cap));
+    HIP_TRY(dalloc(&p.e, cap));
+    HIP_TRY(dalloc(&p.nil0, cap));
+    HIP_TRY(dalloc(&p.nil1, cap));
+    HIP_TRY(dalloc(&p.nil2, cap));
+    HIP_TRY(dalloc(&p.rng, 6 * cap));
+    HIP_TRY(dalloc(&p.event, cap));
+    HIP_TRY(dalloc(&p.meta, cap));
+    HIP_TRY(dalloc(&p.nsteps, cap));
+    if (trace) {
+      HIP_TRY(dalloc(&p.uid, cap));
+      HIP_TRY(dalloc(&p.tedep, cap));
+    }
+    return GXT_OK;
+  }
+  void release() {
+    void *ptrs[] = {p.x, p.y, p.z, p.dx, p.dy, p.dz, p.e, p.nil0, p.nil1, p.nil2, p.rng, p.event, p.meta,
+                    p.nsteps, p.uid, p.tedep};
+    for (void *q : ptrs)
+      if (q) (void)hipFree(q);
+    p = Pool<R>{};
+  }
+};
+
+double now_ms(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+struct Job {
+  uint64_t id = 0;
+  std::vector<gxt_primary> prims;
+  int n_events = 0;
+  int slot = -1;
+  int ev_base = -1;
+  double footprint = 0;
+  double submit_ms = 0, admit_ms = 0, done_ms = 0;
+  std::vector<gxt_event_tally> tallies;
+  int64_t steps = 0;
+};
+
+}  // namespace
+
+struct gxt_ctx {
+  gxt_config cfg{};
+  std::string data_dir;
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  HostData hd;
+  DeviceData dd;
+  int precision = GXT_FP64;
+  int64_t cap = 0;
+  bool trace = false;
+  DevPool<double> pd[2];
+  DevPool<float> pf[2];
+  int parity = 0;
+  int grid = 1024;
+  int32_t *queues = nullptr;
+  Counters *cnt = nullptr;
+  Counters *hcnt = nullptr;  // pinned
+  DevEvent *ev = nullptr;
+  int *job_live = nullptr;
+  int *hjob_live = nullptr;  // pinned
+  double *rows = nullptr;
+  unsigned long long *spec = nullptr;
+  int64_t row_stride = 0;
+  double *trace_buf = nullptr;
+  int64_t trace_cap = 0;
+  gxt_primary *stage = nullptr;
+  int64_t stage_cap = 0;
+  // jobs
+  std::mutex mu;
+  std::deque<std::unique_ptr<Job>> pending;
+  std::vector<std::unique_ptr<Job>> active;   // index = job slot
+  std::vector<std::unique_ptr<Job>> done_unpolled;
+  std::map<uint64_t, std::unique_ptr<Job>> finished;
+  std::vector<char> ev_used;
+  double footprint_used = 0;
+  double tracks_per_gev = 2500.0;
+  // stats
+  int64_t iterations = 0;
+  int64_t max_live = 0;
+  double wall_ms = 0;
+  std::chrono::steady_clock::time_point t0;
+  // serving
+  std::thread server;
+  std::atomic<bool> serving{false};
+  std::atomic<bool> stop{false};
+  int serve_rc = GXT_OK;
+
+  StepArgs args() {
+    StepArgs a;
+    a.P = dd.P;
+    a.cnt = cnt;
+    a.ev = ev;
+    a.job_live = job_live;
+    a.queues = queues;
+    a.rows.bins = rows;
+    a.rows.spec = spec;
+    a.rows.stride = row_stride;
+    a.trace = trace_buf;
+    a.trace_cap = trace_cap;
+    return a;
+  }
+
+  int fail(int code, const std::string &m) {
+    err = m;
+    return set_err(code, m);
+  }
+};
+
+namespace {
+
+double job_footprint(const gxt_ctx *c, const std::vector<gxt_primary> &p) {
+  double f = 0;
+  for (const gxt_primary &q : p) f += 16.0 + c->tracks_per_gev * q.ekin * 1.0e-3;
+  return f;
+}
+
+int alloc_events(gxt_ctx *c, int n) {
+  const int m = (int)c->ev_used.size();
+  int run = 0;
+  for (int i = 0; i < m; ++i) {
+    run = c->ev_used[i] ? 0 : run + 1;
+    if (run == n) {
+      const int b = i - n + 1;
+      for (int k = b; k <= i; ++k) c->ev_used[k] = 1;
+      return b;
+    }
+  }
+  return -1;
+}
+
+// Admit pending jobs whose footprint fits; inject their primaries.
+int admit(gxt_ctx *c) {
+  std::vector<Job *> admitted;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    while (!c->pending.empty()) {
+      Job *j = c->pending.front().get();
+      const double budget = 0.85 * (double)c->cap;
+      const bool nothing_active = c->footprint_used == 0.0;
+      if (!nothing_active && c->footprint_used + j->footprint > budget) break;
+      int slot = -1;
+      for (int s = 0; s < (int)c->active.size(); ++s)
+        if (!c->active[s]) {
+          slot = s;
+          break;
+        }
+      if (slot < 0) break;
+      const int eb = alloc_events(c, j->n_events);
+      if (eb < 0) break;
+      j->slot = slot;
+      j->ev_base = eb;
+      j->admit_ms = now_ms(c->t0);
+      c->footprint_used += j->footprint;
+      c->active[slot] = std::move(c->pending.front());
+      c->pending.pop_front();
+      admitted.push_back(j);
+    }
+  }
+  if (admitted.empty()) return GXT_OK;
+  int64_t total = 0;
+  for (Job *j : admitted) total += (int64_t)j->prims.size();
+  if (total > c->stage_cap) {
+    (void)hipFree(c->stage);
+    c->stage_cap = std::max<int64_t>(total, 4096);
+    HIP_TRY(dalloc(&c->stage, c->stage_cap));
+  }
+  std::vector<gxt_primary> buf;
+  buf.reserve(total);
+  std::vector<std::vector<DevEvent>> evbufs;  // alive until the stream synchronises
+  for (Job *j : admitted) {
+    evbufs.emplace_back(j->n_events);
+    std::vector<DevEvent> &evs = evbufs.back();
+    std::memset(evs.data(), 0, evs.size() * sizeof(DevEvent));
+    for (DevEvent &e : evs) e.job = j->slot;
+    HIP_TRY(hipMemcpyAsync(c->ev + j->ev_base, evs.data(), evs.size() * sizeof(DevEvent), hipMemcpyHostToDevice,
+                           c->stream));
+    c->hjob_live[j->slot] = (int)j->prims.size();
+    HIP_TRY(hipMemcpyAsync(c->job_live + j->slot, &c->hjob_live[j->slot], sizeof(int), hipMemcpyHostToDevice,
+                           c->stream));
+    for (gxt_primary p : j->prims) {
+      p.event += j->ev_base;
+      buf.push_back(p);
+    }
+  }
+  HIP_TRY(hipMemcpyAsync(c->stage, buf.data(), buf.size() * sizeof(gxt_primary), hipMemcpyHostToDevice, c->stream));
+  const StepArgs a = c->args();
+  hipError_t e;
+  if (c->precision == GXT_FP32) {
+    PoolArgs<float> pa{c->pf[c->parity].p, c->pf[1 - c->parity].p};
+    e = launch_inject<float>(a, pa, c->stage, (int)total, c->stream);
+  } else {
+    PoolArgs<double> pa{c->pd[c->parity].p, c->pd[1 - c->parity].p};
+    e = launch_inject<double>(a, pa, c->stage, (int)total, c->stream);
+  }
+  if (e != hipSuccess) return c->fail(GXT_EHIP, std::string("inject: ") + hipGetErrorString(e));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GXT_OK;
+}
+
+int iterate(gxt_ctx *c, int k) {
+  const StepArgs a = c->args();
+  for (int i = 0; i < k; ++i) {
+    hipError_t e;
+    if (c->precision == GXT_FP32) {
+      PoolArgs<float> pa{c->pf[c->parity].p, c->pf[1 - c->parity].p};
+      e = launch_step<float>(a, pa, c->grid, c->stream);
+    } else {
+      PoolArgs<double> pa{c->pd[c->parity].p, c->pd[1 - c->parity].p};
+      e = launch_step<double>(a, pa, c->grid, c->stream);
+    }
+    if (e != hipSuccess) return c->fail(GXT_EHIP, std::string("step: ") + hipGetErrorString(e));
+    c->parity = 1 - c->parity;
+    ++c->iterations;
+  }
+  return GXT_OK;
+}
+
+// Read counters and job live counts; complete finished jobs.
+int collect(gxt_ctx *c, bool &idle) {
+  HIP_TRY(hipMemcpyAsync(c->hcnt, c->cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->hjob_live, c->job_live, c->cfg.max_jobs * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const Counters &h = *c->hcnt;
+  if (h.overflow || (int64_t)h.cur[0] + h.cur[1] > c->cap)
+    return c->fail(GXT_ECAPACITY, "track pool or queue overflow (increase pool_capacity)");
+  c->max_live = std::max<int64_t>(c->max_live, (int64_t)h.cur[0] + h.cur[1]);
+  std::vector<Job *> fin;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    for (auto &jp : c->active)
+      if (jp && c->hjob_live[jp->slot] == 0) fin.push_back(jp.get());
+  }
+  for (Job *j : fin) {
+    std::vector<DevEvent> evs(j->n_events);
+    HIP_TRY(hipMemcpy(evs.data(), c->ev + j->ev_base, evs.size() * sizeof(DevEvent), hipMemcpyDeviceToHost));
+    j->tallies.resize(j->n_events);
+    j->steps = 0;
+    for (int i = 0; i < j->n_events; ++i) {
+      const DevEvent &d = evs[i];
+      gxt_event_tally &t = j->tallies[i];
+      t.edep = d.edep;
+      t.visible = d.visible;
+      t.leak = d.leak;
+      t.charged_length = d.clen;
+      for (int k = 0; k < 3; ++k) t.n_secondaries[k] = (int64_t)d.nsec[k];
+      t.n_steps = (int64_t)d.nsteps;
+      t.n_pairs = (int64_t)d.npairs;
+      t.n_annihil = (int64_t)d.nannihil;
+      j->steps += t.n_steps;
+    }
+    j->done_ms = now_ms(c->t0);
+    std::lock_guard<std::mutex> lk(c->mu);
+    for (int k = 0; k < j->n_events; ++k) c->ev_used[j->ev_base + k] = 0;
+    c->footprint_used -= j->footprint;
+    if (c->footprint_used < 1e-6) c->footprint_used = 0;
+    std::unique_ptr<Job> owned = std::move(c->active[j->slot]);
+    owned->prims.clear();
+    owned->prims.shrink_to_fit();
+    c->done_unpolled.emplace_back(new Job(*owned));  // completion order for gxt_poll
+    c->finished[owned->id] = std::move(owned);
+  }
+  bool any_active = false;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    for (auto &jp : c->active)
+      if (jp) any_active = true;
+    idle = !any_active && c->pending.empty();
+  }
+  if (!any_active && (h.cur[0] + h.cur[1]) != 0)
+    return c->fail(GXT_ESTATE, "tracks alive with no active job (live-count bookkeeping error)");
+  return GXT_OK;
+}
+
+// One scheduling round: admit, run k iterations, collect.
+int round_trip(gxt_ctx *c, int k, bool &idle) {
+  int rc = admit(c);
+  if (rc) return rc;
+  bool any = false;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    for (auto &jp : c->active)
+      if (jp) any = true;
+  }
+  if (any) {
+    rc = iterate(c, k);
+    if (rc) return rc;
+  }
+  return collect(c, idle);
+}
+
+int iters_per_sync(gxt_ctx *c) {
+  const int64_t live = (int64_t)c->hcnt->cur[0] + c->hcnt->cur[1];
+  return live > 100000 ? 8 : 4;
+}
+
+void serve_loop(gxt_ctx *c) {
+  (void)hipSetDevice(c->device);
+  while (!c->stop.load()) {
+    bool idle = true;
+    const int rc = round_trip(c, iters_per_sync(c), idle);
+    if (rc) {
+      c->serve_rc = rc;
+      break;
+    }
+    if (idle) std::this_thread::sleep_for(std::chrono::microseconds(200));
+  }
+}
+
+void destroy_ctx(gxt_ctx *c) {
+  if (c->serving.load()) {
+    c->stop = true;
+    if (c->server.joinable()) c->server.join();
+  }
+  for (int i = 0; i < 2; ++i) {
+    c->pd[i].release();
+    c->pf[i].release();
+  }
+  c->dd.release();
+  (void)hipFree(c->queues);
+  (void)hipFree(c->cnt);
+  (void)hipHostFree(c->hcnt);
+  (void)hipFree(c->ev);
+  (void)hipFree(c->job_live);
+  (void)hipHostFree(c->hjob_live);
+  (void)hipFree(c->rows);
+  (void)hipFree(c->spec);
+  (void)hipFree(c->trace_buf);
+  (void)hipFree(c->stage);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int check_config(const gxt_config *c) {
+  if (!c) return set_err(GXT_EINVAL, "null config");
+  if (c->geometry != GXT_GEOM_BOX && c->geometry != GXT_GEOM_CALO) return set_err(GXT_EINVAL, "bad geometry");
+  if (c->geometry == GXT_GEOM_CALO && (c->n_layers <= 0 || 2 * c->n_layers > kMaxVol))
+    return set_err(GXT_EINVAL, "n_layers out of range");
+  if (c->n_zbins <= 0 || c->n_zbins > kMaxZBins || c->n_rbins <= 0 || c->n_rbins > kMaxRBins)
+    return set_err(GXT_EINVAL, "tally bins out of range");
+  if (c->zbin_mm <= 0 || c->rbin_mm <= 0) return set_err(GXT_EINVAL, "bad tally bin width");
+  if (c->pool_capacity < 1024 || c->pool_capacity > ((int64_t)1 << 27)) return set_err(GXT_EINVAL, "pool_capacity out of range");
+  if (c->max_events <= 0 || c->max_jobs <= 0 || c->max_steps <= 0) return set_err(GXT_EINVAL, "bad limits");
+  if (c->precision != GXT_FP64 && c->precision != GXT_FP32) return set_err(GXT_EINVAL, "bad precision");
+  if (!(c->tracking_cut > 0)) return set_err(GXT_EINVAL, "tracking_cut must be > 0");
+  if (!c->data_dir) return set_err(GXT_EINVAL, "data_dir not set");
+  return GXT_OK;
+}
+
+// Library-wide parameters for the RNG component entry points.
+DeviceData g_rng_dd;
+std::mutex g_rng_mu;
+
+int rng_params(const DevParams **out) {
+  std::lock_guard<std::mutex> lk(g_rng_mu);
+  if (!g_rng_dd.P) {
+    DevParams p;
+    std::memset(&p, 0, sizeof(p));
+    build_jump_matrices(p.jump);
+    HIP_TRY(dalloc(&g_rng_dd.P, 1));
+    HIP_TRY(hipMemcpy(g_rng_dd.P, &p, sizeof(p), hipMemcpyHostToDevice));
+  }
+  *out = g_rng_dd.P;
+  return GXT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gxt_last_error(void) { return g_last_error.c_str(); }
+const char *gxt_ctx_error(gxt_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int gxt_default_config(gxt_config *c) {
+  if (!c) return set_err(GXT_EINVAL, "null config");
+  std::memset(c, 0, sizeof(*c));
+  c->geometry = GXT_GEOM_BOX;
+  c->material = GXT_MAT_PB;
+  c->box_xy = 200.0;
+  c->box_depth = 200.0;
+  c->n_layers = 50;
+  c->absorber = GXT_MAT_PB;
+  c->gap = GXT_MAT_LAR;
+  c->absorber_mm = 2.3;
+  c->gap_mm = 5.7;
+  c->lar_density = 1.396;
+  c->tracking_cut = 1.0e-3;
+  c->seed = 12345;
+  c->precision = GXT_FP64;
+  c->zbin_mm = 10.0;
+  c->rbin_mm = 10.0;
+  c->n_zbins = 20;
+  c->n_rbins = 20;
+  c->pool_capacity = 2000000;
+  c->max_steps = 100000;
+  c->max_events = 1000;
+  c->max_jobs = 64;
+  return GXT_OK;
+}
+
+int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
+  if (!out) return set_err(GXT_EINVAL, "null output");
+  int rc = check_config(cfg);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(GXT_EHIP, "no HIP device");
+  std::unique_ptr<gxt_ctx, void (*)(gxt_ctx *)> c(new gxt_ctx(), destroy_ctx);
+  c->cfg = *cfg;
+  c->data_dir = cfg->data_dir;
+  c->cfg.data_dir = c->data_dir.c_str();
+  c->precision = cfg->precision;
+  c->cap = cfg->pool_capacity;
+  c->trace = cfg->trace_tracks > 0;
+  c->t0 = std::chrono::steady_clock::now();
+  HIP_TRY(hipGetDevice(&c->device));
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+  c->grid = std::max(cus, 1) * 4;
+  // physics data
+  DevParams &P = c->hd.params;
+  std::memset(&P, 0, sizeof(P));
+  std::vector<int> mats = cfg->geometry == GXT_GEOM_BOX ? std::vector<int>{cfg->material}
+                                                        : std::vector<int>{cfg->absorber, cfg->gap};
+  for (size_t s = 0; s < mats.size(); ++s) {
+    std::string err;
+    rc = load_material(c->data_dir, mats[s], cfg->lar_density, c->hd, (int)s, err);
+    if (rc) return set_err(rc, err);
+  }
+  build_geometry(*cfg, P.geo);
+  P.B[0] = cfg->field[0];
+  P.B[1] = cfg->field[1];
+  P.B[2] = cfg->field[2];
+  P.bmag = std::sqrt(P.B[0] * P.B[0] + P.B[1] * P.B[1] + P.B[2] * P.B[2]);
+  P.tracking_cut = cfg->tracking_cut;
+  P.zbin = cfg->zbin_mm;
+  P.rbin = cfg->rbin_mm;
+  P.nz = cfg->n_zbins;
+  P.nr = cfg->n_rbins;
+  P.max_steps = cfg->max_steps;
+  P.trace_events = cfg->trace_tracks > 0 ? cfg->trace_tracks : 0;
+  P.trace_steps = cfg->trace_steps;
+  for (int i = 0; i < 6; ++i) P.seed[i] = cfg->seed;
+  build_jump_matrices(P.jump);
+  rc = upload(c->hd, c->dd);
+  if (rc) return rc;
+  // pools, queues, counters, tallies
+  for (int i = 0; i < 2; ++i) {
+    rc = c->precision == GXT_FP32 ? c->pf[i].alloc(c->cap, c->trace) : c->pd[i].alloc(c->cap, c->trace);
+    if (rc) return rc;
+  }
+  HIP_TRY(dalloc(&c->queues, (size_t)kNumQueues * c->cap));
+  HIP_TRY(dalloc(&c->cnt, 1));
+  HIP_TRY(hipMemset(c->cnt, 0, sizeof(Counters)));
+  HIP_TRY(hipHostMalloc((void **)&c->hcnt, sizeof(Counters), hipHostMallocDefault));
+  std::memset(c->hcnt, 0, sizeof(Counters));
+  HIP_TRY(dalloc(&c->ev, cfg->max_events));
+  HIP_TRY(hipMemset(c->ev, 0, sizeof(DevEvent) * cfg->max_events));
+  HIP_TRY(dalloc(&c->job_live, cfg->max_jobs));
+  HIP_TRY(hipMemset(c->job_live, 0, sizeof(int) * cfg->max_jobs));
+  HIP_TRY(hipHostMalloc((void **)&c->hjob_live, sizeof(int) * cfg->max_jobs, hipHostMallocDefault));
+  std::memset(c->hjob_live, 0, sizeof(int) * cfg->max_jobs);
+  c->row_stride = cfg->n_zbins + cfg->n_rbins + P.geo.nvol;
+  HIP_TRY(dalloc(&c->rows, (size_t)c->grid * c->row_stride));
+  HIP_TRY(hipMemset(c->rows, 0, sizeof(double) * c->grid * c->row_stride));
+  HIP_TRY(dalloc(&c->spec, (size_t)c->grid * 3 * kSpecBins));
+  HIP_TRY(hipMemset(c->spec, 0, sizeof(unsigned long long) * c->grid * 3 * kSpecBins));
+  if (c->trace) {
+    c->trace_cap = 4000000;
+    HIP_TRY(dalloc(&c->trace_buf, (size_t)12 * c->trace_cap));
+  }
+  c->active.resize(cfg->max_jobs);
+  c->ev_used.assign(cfg->max_events, 0);
+  HIP_TRY(hipDeviceSynchronize());
+  *out = c.release();
+  return GXT_OK;
+}
+
+int gxt_destroy(gxt_ctx *ctx) {
+  if (!ctx) return set_err(GXT_EINVAL, "null context");
+  destroy_ctx(ctx);
+  return GXT_OK;
+}
+
+double gxt_now_ms(gxt_ctx *ctx) { return ctx ? now_ms(ctx->t0) : 0.0; }
+
+int gxt_submit(gxt_ctx *c, const gxt_primary *prim, int n, uint64_t job_id) {
+  if (!c || !prim || n <= 0) return set_err(GXT_EINVAL, "bad submit arguments");
+  auto j = std::make_unique<Job>();
+  j->id = job_id;
+  j->prims.assign(prim, prim + n);
+  int maxev = -1;
+  for (int i = 0; i < n; ++i) {
+    const gxt_primary &p = prim[i];
+    if (p.species < 0 || p.species > 2 || !(p.ekin > 0) || p.event < 0)
+      return c->fail(GXT_EINVAL, "bad primary " + std::to_string(i));
+    maxev = std::max(maxev, p.event);
+  }
+  j->n_events = maxev + 1;
+  if (j->n_events > c->cfg.max_events) return c->fail(GXT_ECAPACITY, "job has more events than max_events");
+  j->footprint = job_footprint(c, j->prims);
+  j->submit_ms = now_ms(c->t0);
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (c->finished.count(job_id)) return c->fail(GXT_EINVAL, "job id already used");
+  c->pending.push_back(std::move(j));
+  return GXT_OK;
+}
+
+int gxt_run(gxt_ctx *c, int64_t max_iterations) {
+  if (!c) return set_err(GXT_EINVAL, "null context");
+  if (c->serving.load()) return c->fail(GXT_ESTATE, "gxt_run while serving");
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, c->stream));
+  const int64_t start = c->iterations;
+  int rc = GXT_OK;
+  for (;;) {
+    bool idle = false;
+    rc = round_trip(c, iters_per_sync(c), idle);
+    if (rc || idle) break;
+    if (max_iterations > 0 && c->iterations - start >= max_iterations) break;
+  }
+  (void)hipEventRecord(e1, c->stream);
+  (void)hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  c->wall_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int gxt_poll(gxt_ctx *c, gxt_job_result *out, int max, int *n_out) {
+  if (!c || !n_out || (max > 0 && !out)) return set_err(GXT_EINVAL, "bad poll arguments");
+  std::lock_guard<std::mutex> lk(c->mu);
+  int n = 0;
+  while (n < max && !c->done_unpolled.empty()) {
+    std::unique_ptr<Job> j = std::move(c->done_unpolled.front());
+    c->done_unpolled.erase(c->done_unpolled.begin());
+    gxt_job_result &r = out[n++];
+    std::memset(&r, 0, sizeof(r));
+    r.job_id = j->id;
+    r.n_events = j->n_events;
+    for (const gxt_event_tally &t : j->tallies) {
+      r.edep += t.edep;
+      r.visible += t.visible;
+    }
+    r.submit_ms = j->submit_ms;
+    r.admit_ms = j->admit_ms;
+    r.done_ms = j->done_ms;
+    r.track_steps = j->steps;
+  }
+  *n_out = n;
+  return c->serve_rc;
+}
+
+int gxt_get_event_tallies(gxt_ctx *c, uint64_t job_id, gxt_event_tally *out, int n) {
+  if (!c || !out) return set_err(GXT_EINVAL, "bad arguments");
+  std::lock_guard<std::mutex> lk(c->mu);
+  auto it = c->finished.find(job_id);
+  if (it == c->finished.end()) return c->fail(GXT_ENOTFOUND, "job not finished or unknown");
+  const Job &j = *it->second;
+  for (int i = 0; i < n && i < j.n_events; ++i) out[i] = j.tallies[i];
+  return GXT_OK;
+}
+
+int gxt_get_profiles(gxt_ctx *c, double *ez, double *er, double *ev) {
+  if (!c) return set_err(GXT_EINVAL, "null context");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::vector<double> rows((size_t)c->grid * c->row_stride);
+  HIP_TRY(hipMemcpy(rows.data(), c->rows, rows.size() * sizeof(double), hipMemcpyDeviceToHost));
+  const int nz = c->cfg.n_zbins, nr = c->cfg.n_rbins, nv = c->hd.params.geo.nvol;
+  if (ez) std::fill(ez, ez + nz, 0.0);
+  if (er) std::fill(er, er + nr, 0.0);
+  if (ev) std::fill(ev, ev + nv, 0.0);
+  for (int g = 0; g < c->grid; ++g) {
+    const double *r = rows.data() + (size_t)g * c->row_stride;
+    for (int i = 0; i < nz; ++i)
+      if (ez) ez[i] += r[i];
+    for (int i = 0; i < nr; ++i)
+      if (er) er[i] += r[nz + i];
+    for (int i = 0; i < nv; ++i)
+      if (ev) ev[i] += r[nz + nr + i];
+  }
+  return GXT_OK;
+}
+
+int gxt_get_spectra(gxt_ctx *c, int64_t *counts) {
+  if (!c || !counts) return set_err(GXT_EINVAL, "bad arguments");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::vector<unsigned long long> s((size_t)c->grid * 3 * kSpecBins);
+  HIP_TRY(hipMemcpy(s.data(), c->spec, s.size() * sizeof(s[0]), hipMemcpyDeviceToHost));
+  std::fill(counts, counts + 3 * kSpecBins, 0);
+  for (int g = 0; g < c->grid; ++g)
+    for (int i = 0; i < 3 * kSpecBins; ++i) counts[i] += (int64_t)s[(size_t)g * 3 * kSpecBins + i];
+  return GXT_OK;
+}
+
+int gxt_get_stats(gxt_ctx *c, gxt_stats *o) {
+  if (!c || !o) return set_err(GXT_EINVAL, "bad arguments");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  Counters h;
+  HIP_TRY(hipMemcpy(&h, c->cnt, sizeof(h), hipMemcpyDeviceToHost));
+  o->track_steps = (int64_t)h.steps;
+  o->iterations = c->iterations;
+  o->max_live = c->max_live;
+  o->killed_by_limit = (int64_t)h.killed;
+  o->wall_ms = c->wall_ms;
+  return GXT_OK;
+}
+
+int gxt_reset_tallies(gxt_ctx *c) {
+  if (!c) return set_err(GXT_EINVAL, "null context");
+  if (c->serving.load()) return c->fail(GXT_ESTATE, "reset while serving");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemset(c->rows, 0, sizeof(double) * c->grid * c->row_stride));
+  HIP_TRY(hipMemset(c->spec, 0, sizeof(unsigned long long) * c->grid * 3 * kSpecBins));
+  Counters h;
+  HIP_TRY(hipMemcpy(&h, c->cnt, sizeof(h), hipMemcpyDeviceToHost));
+  h.steps = 0;
+  h.killed = 0;
+  h.trace_n = 0;
+  HIP_TRY(hipMemcpy(c->cnt, &h, sizeof(h), hipMemcpyHostToDevice));
+  c->iterations = 0;
+  c->max_live = 0;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->finished.clear();
+  c->done_unpolled.clear();
+  return GXT_OK;
+}
+
+int gxt_get_trace(gxt_ctx *c, double *out, int64_t max_records, int64_t *n_out) {
+  if (!c || !n_out) return set_err(GXT_EINVAL, "bad arguments");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  Counters h;
+  HIP_TRY(hipMemcpy(&h, c->cnt, sizeof(h), hipMemcpyDeviceToHost));
+  const int64_t n = std::min<int64_t>((int64_t)h.trace_n, c->trace_cap);
+  *n_out = n;
+  if (out && n > 0 && c->trace_buf)
+    HIP_TRY(hipMemcpy(out, c->trace_buf, sizeof(double) * 12 * std::min(n, max_records), hipMemcpyDeviceToHost));
+  if ((int64_t)h.trace_n > c->trace_cap) return c->fail(GXT_ECAPACITY, "trace buffer overflowed");
+  return GXT_OK;
+}
+
+int gxt_serve_start(gxt_ctx *c) {
+  if (!c) return set_err(GXT_EINVAL, "null context");
+  if (c->serving.load()) return c->fail(GXT_ESTATE, "already serving");
+  c->stop = false;
+  c->serve_rc = GXT_OK;
+  c->serving = true;
+  c->server = std::thread(serve_loop, c);
+  return GXT_OK;
+}
+
+int gxt_serve_stop(gxt_ctx *c) {
+  if (!c) return set_err(GXT_EINVAL, "null context");
+  if (!c->serving.load()) return c->fail(GXT_ESTATE, "not serving");
+  c->stop = true;
+  if (c->server.joinable()) c->server.join();
+  c->serving = false;
+  return c->serve_rc;
+}
+
+int gxt_make_primaries(uint32_t seed, int species, double emin, double emax, double spot_mm, int n, uint64_t first,
+                       uint64_t n_total, gxt_primary *out) {
+  if (n < 0 || !out || species < 0 || species > 2) return set_err(GXT_EINVAL, "bad primaries request");
+  // host MRG32k3a (same recurrence and jumps as the device)
+  static uint32_t J[2][64][9];
+  static std::once_flag once;
+  std::call_once(once, [] { build_jump_matrices(J); });
+  auto state = [&](uint64_t idx, uint64_t s[6]) {
+    for (int i = 0; i < 6; ++i) s[i] = seed;
+    const uint64_t m[2] = {4294967087ULL, 4294944443ULL};
+    for (int j = 0; j < 64; ++j)
+      if ((idx >> j) & 1ULL)
+        for (int c = 0; c < 2; ++c) {
+          uint64_t r[3];
+          for (int a = 0; a < 3; ++a) {
+            uint64_t acc = 0;
+            for (int b = 0; b < 3; ++b) acc += ((uint64_t)J[c][j][a * 3 + b] * s[3 * c + b]) % m[c];
+            r[a] = acc % m[c];
+          }
+          for (int a = 0; a < 3; ++a) s[3 * c + a] = r[a];
+        }
+  };
+  auto uniform = [](uint64_t s[6]) {
+    int64_t p1 = (1403580LL * (int64_t)s[1] - 810728LL * (int64_t)s[0]) % 4294967087LL;
+    if (p1 < 0) p1 += 4294967087LL;
+    s[0] = s[1];
+    s[1] = s[2];
+    s[2] = (uint64_t)p1;
+    int64_t p2 = (527612LL * (int64_t)s[5] - 1370589LL * (int64_t)s[3]) % 4294944443LL;
+    if (p2 < 0) p2 += 4294944443LL;
+    s[3] = s[4];
+    s[4] = s[5];
+    s[5] = (uint64_t)p2;
+    const double norm = 1.0 / (4294967087.0 + 1.0);
+    return (p1 > p2) ? (double)(p1 - p2) * norm : (double)(p1 - p2 + 4294967087LL) * norm;
+  };
+  for (int i = 0; i < n; ++i) {
+    gxt_primary &p = out[i];
+    std::memset(&p, 0, sizeof(p));
+    p.species = species;
+    p.event = i;
+    p.substream = first + (uint64_t)i;
+    p.ekin = emin;
+    p.dir[2] = 1.0;
+    if (emax > emin || spot_mm > 0.0) {
+      uint64_t s[6];
+      state(n_total + first + (uint64_t)i, s);
+      const double u0 = uniform(s), u1 = uniform(s), u2 = uniform(s);
+      p.ekin = emin + (emax - emin) * u0;
+      p.pos[0] = (u1 - 0.5) * spot_mm;
+      p.pos[1] = (u2 - 0.5) * spot_mm;
+    }
+  }
+  return GXT_OK;
+}
+
+// ---- component entry points
+int gxt_rng_init(const uint32_t seed[6], const uint64_t *idx, int n, uint32_t *states) {
+  if (!seed || !idx || !states || n <= 0) return set_err(GXT_EINVAL, "bad rng_init arguments");
+  const DevParams *P = nullptr;
+  int rc = rng_params(&P);
+  if (rc) return rc;
+  uint32_t *dseed = nullptr, *dout = nullptr;
+  uint64_t *didx = nullptr;
+  HIP_TRY(dalloc(&dseed, 6));
+  HIP_TRY(dalloc(&didx, n));
+  HIP_TRY(dalloc(&dout, 6 * (size_t)n));
+  HIP_TRY(hipMemcpy(dseed, seed, 6 * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(didx, idx, n * sizeof(uint64_t), hipMemcpyHostToDevice));
+  HIP_TRY(launch_rng_init(P, dseed, didx, n, dout, 0));
+  HIP_TRY(hipMemcpy(states, dout, 6 * (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  (void)hipFree(dseed);
+  (void)hipFree(didx);
+  (void)hipFree(dout);
+  return GXT_OK;
+}
+
+int gxt_rng_draw(uint32_t *states, int n, int k, double *out) {
+  if (!states || !out || n <= 0 || k < 0) return set_err(GXT_EINVAL, "bad rng_draw arguments");
+  uint32_t *ds = nullptr;
+  double *dout = nullptr;
+  HIP_TRY(dalloc(&ds, 6 * (size_t)n));
+  HIP_TRY(dalloc(&dout, (size_t)n * k));
+  HIP_TRY(hipMemcpy(ds, states, 6 * (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIP_TRY(launch_rng_draw(ds, n, k, dout, 0));
+  HIP_TRY(hipMemcpy(out, dout, (size_t)n * k * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(states, ds, 6 * (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  (void)hipFree(ds);
+  (void)hipFree(dout);
+  return GXT_OK;
+}
+
+static int nslots(gxt_ctx *c) { return c->cfg.geometry == GXT_GEOM_BOX ? 1 : 2; }
+
+int gxt_table_value(gxt_ctx *c, int slot, int table, int precision, const double *x, double *out, int n) {
+  if (!c || !x || !out || n <= 0 || slot < 0 || slot >= nslots(c) || table < 0 || table >= GXT_NTAB)
+    return set_err(GXT_EINVAL, "bad table_value arguments");
+  double *dx = nullptr, *dy = nullptr;
+  HIP_TRY(dalloc(&dx, n));
+  HIP_TRY(dalloc(&dy, n));
+  HIP_TRY(hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(launch_table_value(c->dd.P, precision, slot, table, dx, dy, n, c->stream));
+  HIP_TRY(hipMemcpyAsync(out, dy, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  return GXT_OK;
+}
+
+int gxt_macro_xs(gxt_ctx *c, int slot, int species, int process, int precision, const double *e, double *out, int n) {
+  if (!c || !e || !out || n <= 0 || slot < 0 || slot >= nslots(c) || species < 0 || species > 2 || process < 0 ||
+      process >= (species == 1 ? 2 : 3))
+    return set_err(GXT_EINVAL, "bad macro_xs arguments");
+  double *de = nullptr, *dy = nullptr;
+  HIP_TRY(dalloc(&de, n));
+  HIP_TRY(dalloc(&dy, n));
+  HIP_TRY(hipMemcpy(de, e, n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(launch_macro_xs(c->dd.P, precision, slot, species, process, de, dy, n, c->stream));
+  HIP_TRY(hipMemcpyAsync(out, dy, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  (void)hipFree(de);
+  (void)hipFree(dy);
+  return GXT_OK;
+}
+
+int gxt_sample(gxt_ctx *c, int model, int slot, int precision, gxt_sample_io *io) {
+  if (!c || !io || io->n <= 0 || model < 0 || model > GXT_MODEL_ANNIHIL || slot < 0 || slot >= nslots(c))
+    return set_err(GXT_EINVAL, "bad sample arguments");
+  const int n = io->n;
+  SampleArgs s;
+  s.n = n;
+  double *ek = nullptr, *dir = nullptr, *oek = nullptr, *odir = nullptr, *oed = nullptr, *sek = nullptr, *sdir = nullptr;
+  uint32_t *rng = nullptr;
+  int32_t *nsec = nullptr, *ssp = nullptr;
+  HIP_TRY(dalloc(&ek, n));
+  HIP_TRY(dalloc(&dir, 3 * (size_t)n));
+  HIP_TRY(dalloc(&rng, 6 * (size_t)n));
+  HIP_TRY(dalloc(&oek, n));
+  HIP_TRY(dalloc(&odir, 3 * (size_t)n));
+  HIP_TRY(dalloc(&oed, n));
+  HIP_TRY(dalloc(&nsec, n));
+  HIP_TRY(dalloc(&ssp, 2 * (size_t)n));
+  HIP_TRY(dalloc(&sek, 2 * (size_t)n));
+  HIP_TRY(dalloc(&sdir, 6 * (size_t)n));
+  HIP_TRY(hipMemcpy(ek, io->ekin, n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dir, io->dir, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(rng, io->rng, 6 * (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  s.ekin = ek;
+  s.dir = dir;
+  s.rng = rng;
+  s.out_ekin = oek;
+  s.out_dir = odir;
+  s.out_edep = oed;
+  s.nsec = nsec;
+  s.sec_species = ssp;
+  s.sec_ekin = sek;
+  s.sec_dir = sdir;
+  HIP_TRY(launch_sample(c->dd.P, precision, slot, model, s, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(io->rng, rng, 6 * (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(io->out_ekin, oek, n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(io->out_dir, odir, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(io->out_edep, oed, n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(io->nsec, nsec, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(io->sec_species, ssp, 2 * (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(io->sec_ekin, sek, 2 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(io->sec_dir, sdir, 6 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  void *ptrs[] = {ek, dir, rng, oek, odir, oed, nsec, ssp, sek, sdir};
+  for (void *p : ptrs) (void)hipFree(p);
+  return GXT_OK;
+}
+
+int gxt_helix_step(const double field[3], int precision, const double *charge, const double *mom, const double *step,
+                   double *pos, double *dir, int n) {
+  if (!field || !charge || !mom || !step || !pos || !dir || n <= 0) return set_err(GXT_EINVAL, "bad helix arguments");
+  const double bm = std::sqrt(field[0] * field[0] + field[1] * field[1] + field[2] * field[2]);
+  if (!(bm > 0)) return set_err(GXT_EINVAL, "zero field");
+  double *dq = nullptr, *dm = nullptr, *ds = nullptr, *dp = nullptr, *dd = nullptr;
+  HIP_TRY(dalloc(&dq, n));
+  HIP_TRY(dalloc(&dm, n));
+  HIP_TRY(dalloc(&ds, n));
+  HIP_TRY(dalloc(&dp, 3 * (size_t)n));
+  HIP_TRY(dalloc(&dd, 3 * (size_t)n));
+  HIP_TRY(hipMemcpy(dq, charge, n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dm, mom, n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ds, step, n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dp, pos, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dd, dir, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(launch_helix(field, precision, dq, dm, ds, dp, dd, n, 0));
+  HIP_TRY(hipMemcpy(pos, dp, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(dir, dd, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  void *ptrs[] = {dq, dm, ds, dp, dd};
+  for (void *p : ptrs) (void)hipFree(p);
+  return GXT_OK;
+}
+
+}  // extern "C"

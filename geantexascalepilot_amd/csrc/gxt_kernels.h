@@ -1,0 +1,65 @@
+// gxt_kernels.h -- kernel argument blocks and host-side launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gxt_types.h"
+
+namespace gxt {
+
+// Per-workgroup tally rows: bins[row * stride + (z | nz + r | nz + nr + vol)]
+// and spec[row * 3 * kSpecBins + species * kSpecBins + bin].
+struct TallyRows {
+  double *bins;
+  unsigned long long *spec;
+  int64_t stride;
+};
+
+struct StepArgs {
+  const DevParams *P;
+  Counters *cnt;
+  DevEvent *ev;
+  int *job_live;
+  int32_t *queues;  // kNumQueues x cap
+  TallyRows rows;
+  double *trace;
+  int64_t trace_cap;
+};
+
+template <class R>
+struct PoolArgs {
+  Pool<R> cur;
+  Pool<R> nxt;
+};
+
+struct SampleArgs {
+  int n;
+  const double *ekin;
+  const double *dir;
+  uint32_t *rng;
+  double *out_ekin;
+  double *out_dir;
+  double *out_edep;
+  int32_t *nsec;
+  int32_t *sec_species;
+  double *sec_ekin;
+  double *sec_dir;
+};
+
+template <class R>
+hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, hipStream_t st);
+template <class R>
+hipError_t launch_inject(const StepArgs &a, const PoolArgs<R> &pa, const gxt_primary *prim, int n, hipStream_t st);
+
+hipError_t launch_rng_init(const DevParams *P, const uint32_t *seed, const uint64_t *idx, int n, uint32_t *out,
+                           hipStream_t st);
+hipError_t launch_rng_draw(uint32_t *states, int n, int k, double *out, hipStream_t st);
+hipError_t launch_table_value(const DevParams *P, int precision, int slot, int table, const double *x, double *out,
+                              int n, hipStream_t st);
+hipError_t launch_macro_xs(const DevParams *P, int precision, int slot, int species, int proc, const double *e,
+                           double *out, int n, hipStream_t st);
+hipError_t launch_sample(const DevParams *P, int precision, int slot, int model, const SampleArgs &s, hipStream_t st);
+hipError_t launch_helix(const double B[3], int precision, const double *charge, const double *mom, const double *step,
+                        double *pos, double *dir, int n, hipStream_t st);
+
+}  // namespace gxt

@@ -1,0 +1,125 @@
+// gxt_types.h -- host/device layout of the physics data, geometry, track
+// pool and counters of the MI355X step loop.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/gxt.h"
+
+namespace gxt {
+
+constexpr int kMaxVol = 128;      // volumes (calorimeter: 2 per layer)
+constexpr int kMaxElem = 3;       // elements per material
+constexpr int kMaxSlots = 2;      // materials per setup
+constexpr int kMaxZBins = 64;
+constexpr int kMaxRBins = 64;
+constexpr int kNumQueues = 8;     // (species, process) interaction queues
+constexpr int kSpecBins = 64;     // secondary spectrum bins per species
+
+// One physics vector inside the node arrays (x, y, d2 at [off, off+n)).
+struct DevTable {
+  int off;
+  int n;
+  double dl, base;  // log-binning: bin = log(E)/dl - base
+  float dlf, basef;
+};
+
+struct DevElement {
+  int Z;
+  int sb;          // offset of the element's Seltzer-Berger grid (x[32], y[57], v[57][32])
+  int sandia;      // offset of its Sandia rows (5 doubles each)
+  int nint;        // number of Sandia intervals
+  int shell;       // offset of its shell binding energies (MeV)
+  int nshell;
+  double natoms;   // atoms per mm3
+  double zoa;      // Sandia Z/A ratio
+  double wbrem;    // n Z (Z+1): element weight for pair and brems
+};
+
+struct DevMaterial {
+  double ne;          // electrons per mm3
+  double migdal;      // k_p^2 = migdal * E_tot^2
+  double cut_gamma;   // gamma production threshold (MeV)
+  double cut_e;       // electron production threshold (MeV)
+  double cut_length;  // production cut (mm)
+  double compt_split; // Lambda.compt upper edge: above it LambdaPrim.compt / E
+  double phot_emin;   // LambdaPrim.phot lower edge: below it Sandia
+  int nel;
+  int pad;
+  DevElement el[kMaxElem];
+  DevTable tab[GXT_NTAB];
+};
+
+struct DevGeometry {
+  int nvol;
+  int pad;
+  double hx, hy;
+  double z[kMaxVol + 1];
+  int slot[kMaxVol];
+  int visible[kMaxVol];
+};
+
+// Everything a kernel needs besides the pool: lives in one device buffer.
+struct DevParams {
+  DevMaterial mat[kMaxSlots];
+  DevGeometry geo;
+  double B[3];
+  double bmag;
+  double tracking_cut;
+  double zbin, rbin;
+  int nz, nr;
+  int max_steps;
+  int trace_events;   // events [0, trace_events) are traced
+  int trace_steps;
+  int pad;
+  uint32_t seed[6];
+  uint32_t jump[2][64][9];  // A_c^(2^(76+j)) mod m_c
+  const double *xd, *yd, *d2d;  // node arrays, FP64
+  const float *xf, *yf, *d2f;   // FP32 mirrors
+  const double *sbd;            // Seltzer-Berger grids, FP64
+  const float *sbf;             // FP32 mirror
+  const double *sandia;         // Sandia rows (edge keV, a1..a4)
+  const double *shells;         // binding energies (MeV)
+};
+
+// Structure-of-arrays track pool.  R is the precision of the physics state
+// (double, or float in the precision form); positions stay FP64.
+template <class R>
+struct Pool {
+  int64_t cap;
+  double *x, *y, *z;
+  R *dx, *dy, *dz;
+  R *e;
+  R *nil0, *nil1, *nil2;
+  uint32_t *rng;      // 6 planes of cap words
+  int32_t *event;     // event slot
+  int32_t *meta;      // volume | species << 16
+  int32_t *nsteps;
+  uint64_t *uid;      // substream index (written only while tracing)
+  double *tedep;      // step deposit carried to the interaction kernel (tracing only)
+};
+
+// Device counters of the step loop.
+struct Counters {
+  int32_t cur[2];          // live neutral / charged tracks in the current pool
+  int32_t nxt[2];          // written to the next pool
+  int32_t q[kNumQueues];   // interaction queue lengths
+  int32_t overflow;        // a pool or queue write would have gone out of range
+  int32_t prim_head;       // primaries consumed from the primary queue
+  int32_t prim_tail;       // primaries available in the primary queue
+  int32_t pad;
+  unsigned long long steps;   // track-steps
+  unsigned long long killed;  // tracks killed by the step limit
+  unsigned long long trace_n; // trace records written
+};
+
+// Per-event tallies on the device (FP64, atomics).
+struct DevEvent {
+  double edep, visible, leak, clen;
+  unsigned long long nsec[3];
+  unsigned long long nsteps, npairs, nannihil;
+  int job;       // job slot owning the event
+  int pad;
+};
+
+}  // namespace gxt

@@ -1,0 +1,203 @@
+"""Python host interface of libgxt.so (the MI355X step loop).
+
+Mirrors the reference's device-manager flow (ProxyDeviceManager:
+UploadTrackState -> DoStep -> DownloadTrackData, proxy/ProxyDeviceManager.hpp:46-65)
+as Transport.submit -> Transport.run -> Transport.event_tallies / profiles,
+and exposes the component entry points (RNG, tables, samplers, helix) that
+the parity tests drive.  Every call goes through the C ABI of include/gxt.h;
+there is no Python or CPU fallback: if libgxt.so is missing this module
+raises at import.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgxt.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("libgxt.so not built (run __graft_entry__.build() or geantexascalepilot_amd/build.py)")
+
+lib = C.CDLL(LIB_PATH)
+lib.gxt_last_error.restype = C.c_char_p
+lib.gxt_ctx_error.restype = C.c_char_p
+lib.gxt_now_ms.restype = C.c_double
+lib.gxt_now_ms.argtypes = [C.c_void_p]
+
+
+class GxtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (abi.ERRORS.get(code, "error"), code, msg))
+        self.code = code
+
+
+def _check(rc, ctx=None):
+    if rc != 0:
+        msg = lib.gxt_ctx_error(ctx).decode() if ctx else lib.gxt_last_error().decode()
+        raise GxtError(rc, msg)
+
+
+def make_primaries(seed, species, emin, emax, spot_mm, n, first=0, n_total=None):
+    """Primary gun (ProxyEgammaGun::GenerateOneEvent, proxy/src/ProxyEgammaGun.cpp:43-85)."""
+    p = abi.primaries_array(n)
+    _check(lib.gxt_make_primaries(C.c_uint32(seed), species, C.c_double(emin), C.c_double(emax),
+                                  C.c_double(spot_mm), n, C.c_uint64(first),
+                                  C.c_uint64(n if n_total is None else n_total), p))
+    return p
+
+
+def rng_init(seed6, substreams):
+    seed6 = np.ascontiguousarray(seed6, dtype=np.uint32)
+    idx = np.ascontiguousarray(substreams, dtype=np.uint64)
+    out = np.zeros((len(idx), 6), dtype=np.uint32)
+    _check(lib.gxt_rng_init(abi.ptr(seed6, C.c_uint32), abi.ptr(idx, C.c_uint64), len(idx),
+                            abi.ptr(out, C.c_uint32)))
+    return out
+
+
+def rng_draw(states, k):
+    st = np.ascontiguousarray(states, dtype=np.uint32).copy()
+    n = st.shape[0]
+    out = np.zeros((n, k))
+    _check(lib.gxt_rng_draw(abi.ptr(st, C.c_uint32), n, k, abi.ptr(out)))
+    return out, st
+
+
+def helix_step(field, charge, mom, step, pos, dirs, precision=abi.FP64):
+    field = np.ascontiguousarray(field, dtype=np.float64)
+    charge = np.ascontiguousarray(charge, dtype=np.float64)
+    mom = np.ascontiguousarray(mom, dtype=np.float64)
+    step = np.ascontiguousarray(step, dtype=np.float64)
+    pos = np.ascontiguousarray(pos, dtype=np.float64).copy()
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64).copy()
+    _check(lib.gxt_helix_step(abi.ptr(field), precision, abi.ptr(charge), abi.ptr(mom), abi.ptr(step),
+                              abi.ptr(pos), abi.ptr(dirs), len(charge)))
+    return pos, dirs
+
+
+class Transport:
+    """One device context: physics tables, track pool, tallies, one HIP stream."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.ctx = C.c_void_p()
+        _check(lib.gxt_create(C.byref(cfg), C.byref(self.ctx)))
+        self.nvol = 1 if cfg.geometry == abi.GEOM_BOX else 2 * cfg.n_layers
+
+    def close(self):
+        if self.ctx:
+            lib.gxt_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        _check(rc, self.ctx)
+
+    def submit(self, prims, job_id):
+        self._c(lib.gxt_submit(self.ctx, prims, len(prims), C.c_uint64(job_id)))
+
+    def run(self, max_iterations=0):
+        self._c(lib.gxt_run(self.ctx, C.c_int64(max_iterations)))
+
+    def poll(self, max_jobs=256):
+        out = (abi.JobResult * max_jobs)()
+        n = C.c_int(0)
+        self._c(lib.gxt_poll(self.ctx, out, max_jobs, C.byref(n)))
+        return [out[i] for i in range(n.value)]
+
+    def event_tallies(self, job_id, n):
+        t = (abi.EventTally * n)()
+        self._c(lib.gxt_get_event_tallies(self.ctx, C.c_uint64(job_id), t, n))
+        return abi.tallies_to_numpy(t)
+
+    def profiles(self):
+        ez = np.zeros(self.cfg.n_zbins)
+        er = np.zeros(self.cfg.n_rbins)
+        ev = np.zeros(self.nvol)
+        self._c(lib.gxt_get_profiles(self.ctx, abi.ptr(ez), abi.ptr(er), abi.ptr(ev)))
+        return ez, er, ev
+
+    def spectra(self):
+        s = np.zeros(3 * 64, dtype=np.int64)
+        self._c(lib.gxt_get_spectra(self.ctx, abi.ptr(s, C.c_int64)))
+        return s.reshape(3, 64)
+
+    def stats(self):
+        s = abi.Stats()
+        self._c(lib.gxt_get_stats(self.ctx, C.byref(s)))
+        return s
+
+    def reset_tallies(self):
+        self._c(lib.gxt_reset_tallies(self.ctx))
+
+    def trace(self):
+        n = C.c_int64()
+        self._c(lib.gxt_get_trace(self.ctx, None, 0, C.byref(n)))
+        out = np.zeros((max(n.value, 1), 12))
+        self._c(lib.gxt_get_trace(self.ctx, abi.ptr(out), n.value, C.byref(n)))
+        return out[: n.value]
+
+    def now_ms(self):
+        return lib.gxt_now_ms(self.ctx)
+
+    def serve_start(self):
+        self._c(lib.gxt_serve_start(self.ctx))
+
+    def serve_stop(self):
+        self._c(lib.gxt_serve_stop(self.ctx))
+
+    def table_value(self, slot, table, x, precision=abi.FP64):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros_like(x)
+        self._c(lib.gxt_table_value(self.ctx, slot, table, precision, abi.ptr(x), abi.ptr(out), len(x)))
+        return out
+
+    def macro_xs(self, slot, species, process, e, precision=abi.FP64):
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        out = np.zeros_like(e)
+        self._c(lib.gxt_macro_xs(self.ctx, slot, species, process, precision, abi.ptr(e), abi.ptr(out), len(e)))
+        return out
+
+    def sample(self, model, slot, ekin, dirs, states, precision=abi.FP64):
+        from_io = _sample_with(lambda io: lib.gxt_sample(self.ctx, model, slot, precision, C.byref(io)),
+                               ekin, dirs, states)
+        return from_io
+
+
+def _sample_with(call, ekin, dirs, states):
+    n = len(ekin)
+    ekin = np.ascontiguousarray(ekin, dtype=np.float64)
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(3, n)
+    st = np.ascontiguousarray(states, dtype=np.uint32).reshape(n, 6).copy()
+    out = dict(ekin=np.zeros(n), dir=np.zeros((3, n)), edep=np.zeros(n), nsec=np.zeros(n, dtype=np.int32),
+               sec_species=np.zeros((n, 2), dtype=np.int32), sec_ekin=np.zeros((n, 2)),
+               sec_dir=np.zeros((n, 2, 3)))
+    io = abi.SampleIO()
+    io.n = n
+    io.ekin = abi.ptr(ekin)
+    io.dir = abi.ptr(dirs)
+    io.rng = abi.ptr(st, C.c_uint32)
+    io.out_ekin = abi.ptr(out["ekin"])
+    io.out_dir = abi.ptr(out["dir"])
+    io.out_edep = abi.ptr(out["edep"])
+    io.nsec = abi.ptr(out["nsec"], C.c_int32)
+    io.sec_species = abi.ptr(out["sec_species"], C.c_int32)
+    io.sec_ekin = abi.ptr(out["sec_ekin"])
+    io.sec_dir = abi.ptr(out["sec_dir"])
+    _check(call(io))
+    out["rng"] = st
+    return out
