@@ -1,0 +1,104 @@
+"""The C ABI without a GPU: libgxt.so loads and exports every function that
+include/gxt.h declares; the ctypes mirror has the header's struct layout
+(checked against a C compiler's sizeof/offsetof); argument validation fails
+with the documented codes before any device work."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from geantexascalepilot_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gxt.h")
+LIB = os.path.join(ROOT, "geantexascalepilot_amd", "libgxt.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gxt_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    assert os.path.exists(LIB), "libgxt.so not built"
+    lib = C.CDLL(LIB)
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+STRUCTS = {
+    "gxt_config": abi.Config,
+    "gxt_primary": abi.Primary,
+    "gxt_event_tally": abi.EventTally,
+    "gxt_stats": abi.Stats,
+    "gxt_job_result": abi.JobResult,
+    "gxt_sample_io": abi.SampleIO,
+}
+
+
+def test_struct_layout_matches_header(tmp_path):
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HEADER, "int main(void){"]
+    for cname, py in STRUCTS.items():
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-o", str(exe), str(src)])
+    out = subprocess.check_output([str(exe)]).decode().split("\n")
+    got = {}
+    for line in out:
+        if line:
+            a, b, v = line.split()
+            got[(a, b)] = int(v)
+    for cname, py in STRUCTS.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_constants_match_header():
+    text = open(HEADER).read()
+    consts = dict((k, int(v)) for k, v in re.findall(r"#define (GXT_[A-Z0-9_]+) \(?(-?\d+)\)?", text))
+    for name, v in abi.TAB.items():
+        assert consts["GXT_TAB_" + name] == v
+    for name, v in abi.MODEL.items():
+        assert consts["GXT_MODEL_" + name] == v
+    assert consts["GXT_NTAB"] == abi.NTAB
+    for code, name in abi.ERRORS.items():
+        assert consts[name] == code
+
+
+def test_validation_without_device():
+    lib = C.CDLL(LIB)
+    lib.gxt_last_error.restype = C.c_char_p
+    cfg = abi.config_lead_block()
+    cfg.n_zbins = 0
+    ctx = C.c_void_p()
+    assert lib.gxt_create(C.byref(cfg), C.byref(ctx)) == -1
+    assert b"bins" in lib.gxt_last_error()
+    assert lib.gxt_create(None, C.byref(ctx)) == -1
+    d = abi.Config()
+    assert lib.gxt_default_config(C.byref(d)) == 0
+    assert d.tracking_cut == 1e-3 and d.pool_capacity == 2_000_000
+
+
+def test_host_gun_matches_oracle():
+    import oracle_harness as O
+    lib = C.CDLL(LIB)
+    n = 50
+    a = abi.primaries_array(n)
+    assert lib.gxt_make_primaries(C.c_uint32(2468), 1, C.c_double(1000.0), C.c_double(50000.0), C.c_double(10.0),
+                                  n, C.c_uint64(0), C.c_uint64(n), a) == 0
+    b = O.make_primaries(2468, 1, 1000.0, 50000.0, 10.0, n)
+    for x, y in zip(a, b):
+        assert x.ekin == y.ekin and list(x.pos) == list(y.pos) and x.substream == y.substream
+    e = [x.ekin for x in a]
+    assert 1000.0 <= min(e) and max(e) <= 50000.0
