@@ -1,0 +1,213 @@
+"""Benchmark of the MI355X EM step loop on one GPU.
+
+Workload (the headline configuration): 10,000 primary electrons of 10 GeV
+along +z into a 30x30x30 cm PbWO4 block (8.28 g/cm3), no field, Compton,
+photoelectric, conversion, e-/e+ bremsstrahlung, ionisation and e+
+annihilation, 0.7 mm production cut, 1 keV tracking cut, live-track pool of
+2e6 tracks, MRG32k3a seed 777 with one substream per primary.  A "step" of
+this benchmark is one transport of the whole batch (every shower run to
+completion); value = events/s of the FP64 form.  The same line reports
+track-steps/s, the FP32 precision form on the same batch, a short serving
+run (Poisson job mix, FP32, p50/p99 job latency), the roofline of the
+dominant kernel from HIP events around each launch, and the CPU oracle
+timed on the host cores on a bounded sample.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+ME = 0.510998910
+
+
+def log(msg):
+    sys.stderr.write("[bench] %s\n" % msg)
+    sys.stderr.flush()
+
+
+def run_batch(g, abi, precision, n_events, steps, warmup, seed=777):
+    cfg = abi.config_pbwo4_block(max_events=n_events, precision=precision, profile_kernels=1,
+                                 pool_capacity=2_000_000, seed=seed)
+    prims = g.make_primaries(seed, abi.ELECTRON, 10000.0, 10000.0, 0.0, n_events)
+    import torch
+    with g.Transport(cfg) as t:
+        job = 0
+        for w in range(warmup):
+            t.submit(prims, job)
+            t.run()
+            job += 1
+            log("warmup %d/%d done (precision %d)" % (w + 1, warmup, precision))
+        t.reset_tallies()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            t.submit(prims, job)
+            t.run()
+            job += 1
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ks = t.kernel_stats()
+        st = t.stats()
+        edep = t.event_tallies(job - 1, n_events)
+    log("timed %d batches in %.3f s (precision %d)" % (steps, t1 - t0, precision))
+    return dict(seconds=t1 - t0, ks=ks, stats=st, edep=edep)
+
+
+def serving(g, abi, seconds, rate_jobs_s, seed=4242):
+    """Poisson stream of jobs (40 % one 1 GeV gamma, 40 % 10 x 10 GeV e-,
+    20 % 100 x 50 GeV e-) into the PbWO4 block, FP32 form, one shared pool."""
+    import numpy as np
+    rs = np.random.default_rng(seed)
+    cfg = abi.config_pbwo4_block(max_events=20000, precision=abi.FP32, max_jobs=256, pool_capacity=8_000_000)
+    kinds = []
+    with g.Transport(cfg) as t:
+        t.serve_start()
+        t_start = time.perf_counter()
+        next_sub = 0
+        first = 0
+        submitted = 0
+        done = []
+        while True:
+            now = time.perf_counter() - t_start
+            if now >= seconds:
+                break
+            if now >= next_sub:
+                u = rs.random()
+                if u < 0.4:
+                    kind, n, sp, e = "gamma1", 1, abi.GAMMA, 1000.0
+                elif u < 0.8:
+                    kind, n, sp, e = "e10x10", 10, abi.ELECTRON, 10000.0
+                else:
+                    kind, n, sp, e = "e100x50", 100, abi.ELECTRON, 50000.0
+                p = g.make_primaries(777, sp, e, e, 0.0, n, first=first, n_total=1 << 40)
+                first += n
+                t.submit(p, submitted)
+                kinds.append(kind)
+                submitted += 1
+                next_sub = now + rs.exponential(1.0 / rate_jobs_s)
+            done += t.poll()
+            time.sleep(0.0005)
+        # drain
+        t_end_submit = time.perf_counter() - t_start
+        while len(done) < submitted:
+            done += t.poll()
+            time.sleep(0.002)
+            if time.perf_counter() - t_start > seconds + 120:
+                break
+        wall = time.perf_counter() - t_start
+        st = t.stats()
+        t.serve_stop()
+    lat = np.array([r.done_ms - r.submit_ms for r in done])
+    kind_of = {i: k for i, k in enumerate(kinds)}
+    single = np.array([r.done_ms - r.submit_ms for r in done if kind_of[r.job_id] == "gamma1"])
+    events = sum(r.n_events for r in done)
+    steps = sum(r.track_steps for r in done)
+    return dict(jobs=len(done), submitted=submitted, events=events, wall_s=wall, submit_s=t_end_submit,
+                events_per_s=events / wall, track_steps_per_s=steps / wall,
+                p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                single_event_p50_ms=float(np.percentile(single, 50)) if len(single) else None,
+                single_event_p99_ms=float(np.percentile(single, 99)) if len(single) else None,
+                rate_jobs_s=rate_jobs_s, max_live=int(st.max_live))
+
+
+def cpu_baseline(abi, target_s=15.0):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_harness as O
+    threads = max(1, min(16, os.cpu_count() or 1))
+    cfg = abi.config_pbwo4_block(max_events=100000)
+
+    def timed(n):
+        o = O.Oracle(cfg)
+        p = O.make_primaries(777, abi.ELECTRON, 10000.0, 10000.0, 0.0, n)
+        t0 = time.perf_counter()
+        o.run(p, threads)
+        dt = time.perf_counter() - t0
+        return dt, o.stats().track_steps
+
+    dt, _ = timed(threads)
+    n = int(max(threads, min(4000, threads * target_s / max(dt, 1e-3))))
+    dt, steps = timed(n)
+    log("cpu oracle: %d events on %d threads in %.2f s" % (n, threads, dt))
+    return dict(value=n / dt, unit="events/s", cores=threads, kind="port",
+                sample="first %d primaries of the same 10 GeV e- / PbWO4 batch (FP64 oracle, %d track-steps, %.1f s)"
+                % (n, steps, dt), track_steps_per_s=steps / dt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--events", type=int, default=10000)
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-fp32", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        sys.stderr.write("bench.py: this build runs the step loop on one GPU; multi-GPU runs are not supported "
+                         "(events shard independently; see the design notes)\n")
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "needs a GPU"
+    torch.cuda.set_device(0)
+    from geantexascalepilot_amd import abi
+    import geantexascalepilot_amd.gxt as g
+
+    n = a.events
+    r64 = run_batch(g, abi, abi.FP64, n, a.steps, a.warmup)
+    value = a.steps * n / r64["seconds"]
+    ks = r64["ks"]
+    steps_per_s = ks.track_steps / r64["seconds"]
+    rec = ks.record_bytes
+    t_ms, i_ms = ks.transport_ms, ks.interact_ms
+    if t_ms >= i_ms:
+        name, ms = "k_transport", t_ms
+        nbytes = ks.track_steps * 2 * rec
+        per_unit = "2 x %d B per track-step (one read and one write of the track record)" % rec
+    else:
+        name, ms = "k_interact", i_ms
+        nbytes = (2 * ks.interactions + ks.created) * rec
+        per_unit = "%d B per sampled track read + written, %d B per created track" % (2 * rec, rec)
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s", frac=achieved / PEAK_HBM_GBS,
+                    traffic=None, kernel=name, launches=int(ks.launches), avg_launch_us=1e3 * ms / max(ks.launches, 1),
+                    algorithmic_bytes=int(nbytes), per_unit=per_unit,
+                    kernel_ms={"transport": t_ms, "interact": i_ms, "advance": ks.advance_ms})
+    e64 = r64["edep"]["edep"]
+    out = dict(metric="events/s and track-steps/s, FP32 vs FP64 on 1 MI355X; p99 job latency", value=value,
+               unit="events/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
+               ms_per_step=1e3 * r64["seconds"] / a.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
+               dtype="f64", data="synthetic",
+               config=dict(workload="10 GeV e- x %d into 30x30x30 cm PbWO4 (8.28 g/cm3), no field, FP64, "
+                                    "pool 2e6 tracks, seed 777" % n, primaries=n, energy_gev=10.0,
+                           material="PbWO4", pool_capacity=2_000_000),
+               track_steps_per_s=steps_per_s, track_steps_per_event=ks.track_steps / (a.steps * n),
+               mean_edep_mev=float(e64.mean()), roofline=roofline)
+    if not a.skip_fp32:
+        r32 = run_batch(g, abi, abi.FP32, n, a.steps, a.warmup)
+        e32 = r32["edep"]["edep"]
+        out["fp32"] = dict(value=a.steps * n / r32["seconds"], unit="events/s",
+                           track_steps_per_s=r32["ks"].track_steps / r32["seconds"],
+                           speedup_vs_fp64=(a.steps * n / r32["seconds"]) / value,
+                           edep_rel_diff_vs_fp64=float(e32.sum() / e64.sum() - 1.0))
+    if not a.skip_serving:
+        gev_per_s = (out.get("fp32", {}).get("value", value)) * 10.0
+        rate = 0.6 * gev_per_s / (0.4 * 1.0 + 0.4 * 100.0 + 0.2 * 5000.0)
+        log("serving for %.0f s at %.2f jobs/s" % (a.serve_seconds, rate))
+        out["serving"] = serving(g, abi, a.serve_seconds, rate)
+    if not a.skip_cpu:
+        out["cpu_baseline"] = cpu_baseline(abi)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

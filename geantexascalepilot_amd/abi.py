@@ -37,7 +37,7 @@ class Config(C.Structure):
         ("precision", C.c_int32), ("zbin_mm", C.c_double), ("rbin_mm", C.c_double), ("n_zbins", C.c_int32),
         ("n_rbins", C.c_int32), ("pool_capacity", C.c_int64), ("max_steps", C.c_int32),
         ("max_events", C.c_int32), ("max_jobs", C.c_int32), ("trace_tracks", C.c_int32),
-        ("trace_steps", C.c_int32), ("pad1", C.c_int32), ("data_dir", C.c_char_p),
+        ("trace_steps", C.c_int32), ("profile_kernels", C.c_int32), ("data_dir", C.c_char_p),
     ]
 
 
@@ -55,6 +55,12 @@ class EventTally(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("track_steps", C.c_int64), ("iterations", C.c_int64), ("max_live", C.c_int64),
                 ("killed_by_limit", C.c_int64), ("wall_ms", C.c_double)]
+
+
+class KernelStats(C.Structure):
+    _fields_ = [("transport_ms", C.c_double), ("interact_ms", C.c_double), ("advance_ms", C.c_double),
+                ("launches", C.c_int64), ("track_steps", C.c_int64), ("interactions", C.c_int64),
+                ("created", C.c_int64), ("record_bytes", C.c_int32), ("pad", C.c_int32)]
 
 
 class JobResult(C.Structure):
@@ -101,6 +107,7 @@ def base_config(**kw):
     c.max_jobs = 64
     c.trace_tracks = 0
     c.trace_steps = 0
+    c.profile_kernels = 0
     c._data = DATA_DIR.encode()
     c.data_dir = c._data
     for k, v in kw.items():

@@ -889,11 +889,21 @@ __device__ inline int propagate(const DevParams *P, int species, R e, double pos
     if (pt2 < R(0)) pt2 = R(0);
     const R pt = m_sqrt(pt2);
     const double sl = pt > R(0) ? (double)(R(2) * m_sqrt(R(0.1) / (R(0.299792458) * (R)P->bmag / pt))) : kHuge;
-    double s;
-    if (slin <= sl && slin <= remaining)
-      s = fmax(slin, 1.0e-6);
-    else
-      s = fmin(remaining, fmax(sl, saf));
+    if (slin <= sl && slin <= remaining) {
+      // aimed step: cross the aimed face by construction, on its plane
+      const double s = fmax(slin, 1.0e-6);
+      helix_step<R>(P->B, P->bmag, q, mom, (R)s, pos, dir);
+      step += s;
+      if (face == 0 || face == 1) {
+        const int nv = face == 1 ? vol + 1 : vol - 1;
+        pos[2] = g.z[face == 1 ? vol + 1 : vol];
+        if (nv < 0 || nv >= g.nvol || fabs(pos[0]) > g.hx || fabs(pos[1]) > g.hy) return kEscaped;
+        vol = nv;
+        return kBoundary;
+      }
+      return kEscaped;
+    }
+    const double s = fmin(remaining, fmax(sl, saf));
     helix_step<R>(P->B, P->bmag, q, mom, (R)s, pos, dir);
     step += s;
     remaining -= s;

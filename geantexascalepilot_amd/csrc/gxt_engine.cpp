@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -196,7 +197,11 @@ struct gxt_ctx {
   std::map<uint64_t, std::unique_ptr<Job>> finished;
   std::vector<char> ev_used;
   double footprint_used = 0;
-  double tracks_per_gev = 2500.0;
+  double tracks_per_gev = 1000.0;  // admission estimate of peak live tracks per GeV of primary energy
+  // kernel timing (profile_kernels)
+  bool prof = false;
+  std::vector<std::array<hipEvent_t, 4>> ev_free, ev_pending;
+  gxt_kernel_stats kst{};
   // stats
   int64_t iterations = 0;
   int64_t max_live = 0;
@@ -324,12 +329,23 @@ int iterate(gxt_ctx *c, int k) {
   const StepArgs a = c->args();
   for (int i = 0; i < k; ++i) {
     hipError_t e;
+    hipEvent_t *evs = nullptr;
+    if (c->prof) {
+      if (c->ev_free.empty()) {
+        std::array<hipEvent_t, 4> q;
+        for (auto &x : q) HIP_TRY(hipEventCreate(&x));
+        c->ev_free.push_back(q);
+      }
+      c->ev_pending.push_back(c->ev_free.back());
+      c->ev_free.pop_back();
+      evs = c->ev_pending.back().data();
+    }
     if (c->precision == GXT_FP32) {
       PoolArgs<float> pa{c->pf[c->parity].p, c->pf[1 - c->parity].p};
-      e = launch_step<float>(a, pa, c->grid, c->stream);
+      e = launch_step<float>(a, pa, c->grid, c->stream, evs);
     } else {
       PoolArgs<double> pa{c->pd[c->parity].p, c->pd[1 - c->parity].p};
-      e = launch_step<double>(a, pa, c->grid, c->stream);
+      e = launch_step<double>(a, pa, c->grid, c->stream, evs);
     }
     if (e != hipSuccess) return c->fail(GXT_EHIP, std::string("step: ") + hipGetErrorString(e));
     c->parity = 1 - c->parity;
@@ -343,6 +359,18 @@ int collect(gxt_ctx *c, bool &idle) {
   HIP_TRY(hipMemcpyAsync(c->hcnt, c->cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(c->hjob_live, c->job_live, c->cfg.max_jobs * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  for (auto &q : c->ev_pending) {
+    float t0 = 0, t1 = 0, t2 = 0;
+    HIP_TRY(hipEventElapsedTime(&t0, q[0], q[1]));
+    HIP_TRY(hipEventElapsedTime(&t1, q[1], q[2]));
+    HIP_TRY(hipEventElapsedTime(&t2, q[2], q[3]));
+    c->kst.transport_ms += t0;
+    c->kst.interact_ms += t1;
+    c->kst.advance_ms += t2;
+    c->kst.launches++;
+    c->ev_free.push_back(q);
+  }
+  c->ev_pending.clear();
   const Counters &h = *c->hcnt;
   if (h.overflow || (int64_t)h.cur[0] + h.cur[1] > c->cap)
     return c->fail(GXT_ECAPACITY, "track pool or queue overflow (increase pool_capacity)");
@@ -438,6 +466,10 @@ void destroy_ctx(gxt_ctx *c) {
     c->pd[i].release();
     c->pf[i].release();
   }
+  for (auto &q : c->ev_free)
+    for (auto &x : q) (void)hipEventDestroy(x);
+  for (auto &q : c->ev_pending)
+    for (auto &x : q) (void)hipEventDestroy(x);
   c->dd.release();
   (void)hipFree(c->queues);
   (void)hipFree(c->cnt);
@@ -533,6 +565,7 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
   c->precision = cfg->precision;
   c->cap = cfg->pool_capacity;
   c->trace = cfg->trace_tracks > 0;
+  c->prof = cfg->profile_kernels > 0;
   c->t0 = std::chrono::steady_clock::now();
   HIP_TRY(hipGetDevice(&c->device));
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -732,6 +765,21 @@ int gxt_get_stats(gxt_ctx *c, gxt_stats *o) {
   return GXT_OK;
 }
 
+int gxt_get_kernel_stats(gxt_ctx *c, gxt_kernel_stats *o) {
+  if (!c || !o) return set_err(GXT_EINVAL, "bad arguments");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  Counters h;
+  HIP_TRY(hipMemcpy(&h, c->cnt, sizeof(h), hipMemcpyDeviceToHost));
+  *o = c->kst;
+  o->track_steps = (int64_t)h.steps;
+  o->interactions = (int64_t)h.interactions;
+  o->created = (int64_t)h.created;
+  // x, y, z (FP64) + dir, E, 3 interaction lengths (R) + 6 RNG words + event, meta, nsteps
+  const int r = c->precision == GXT_FP32 ? 4 : 8;
+  o->record_bytes = 3 * 8 + 7 * r + 6 * 4 + 3 * 4;
+  return GXT_OK;
+}
+
 int gxt_reset_tallies(gxt_ctx *c) {
   if (!c) return set_err(GXT_EINVAL, "null context");
   if (c->serving.load()) return c->fail(GXT_ESTATE, "reset while serving");
@@ -743,6 +791,9 @@ int gxt_reset_tallies(gxt_ctx *c) {
   h.steps = 0;
   h.killed = 0;
   h.trace_n = 0;
+  h.interactions = 0;
+  h.created = 0;
+  c->kst = gxt_kernel_stats{};
   HIP_TRY(hipMemcpy(c->cnt, &h, sizeof(h), hipMemcpyHostToDevice));
   c->iterations = 0;
   c->max_live = 0;

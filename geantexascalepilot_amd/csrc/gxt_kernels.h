@@ -47,7 +47,7 @@ struct SampleArgs {
 };
 
 template <class R>
-hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, hipStream_t st);
+hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, hipStream_t st, hipEvent_t *ev = nullptr);
 template <class R>
 hipError_t launch_inject(const StepArgs &a, const PoolArgs<R> &pa, const gxt_primary *prim, int n, hipStream_t st);
 

@@ -523,6 +523,7 @@ __global__ __launch_bounds__(kBlock) void k_interact(StepArgs a, PoolArgs<R> pa)
       }
       live_delta = nnew + (survive ? 1 : 0) - 1;
     }
+    wave_add_u64(&a.cnt->interactions, active);
     int outN = 0, outC = 0;
     if (survive) (t.species == 0 ? outN : outC)++;
     for (int k = 0; k < nnew; ++k) (nt[k].species == 0 ? outN : outC)++;
@@ -545,6 +546,12 @@ __global__ __launch_bounds__(kBlock) void k_interact(StepArgs a, PoolArgs<R> pa)
         atomicOr(&a.cnt->overflow, 1);
       atomicAdd(&a.ev[t.event].nsec[sp], 1ULL);
       spectrum(lt, sp, (double)nt[k].e);
+    }
+    {
+      const int made = nnew;
+      const int incl = wave_incl_scan(made);
+      const int tot = __shfl(incl, 63, 64);
+      if (tot > 0 && lane() == 63) atomicAdd(&a.cnt->created, (unsigned long long)tot);
     }
     if (live_delta != 0) atomicAdd(&a.job_live[a.ev[t.event].job], live_delta);
   }
@@ -661,10 +668,15 @@ __global__ void k_helix(double b0, double b1, double b2, const double *charge, c
 static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
 
 template <class R>
-hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, hipStream_t st) {
+hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, hipStream_t st, hipEvent_t *ev) {
+  // ev (optional): 4 events recorded before/after the transport and interaction kernels
+  if (ev) (void)hipEventRecord(ev[0], st);
   hipLaunchKernelGGL(k_transport<R>, dim3(grid), dim3(kBlock), 0, st, a, pa);
+  if (ev) (void)hipEventRecord(ev[1], st);
   hipLaunchKernelGGL(k_interact<R>, dim3(grid), dim3(kBlock), 0, st, a, pa);
+  if (ev) (void)hipEventRecord(ev[2], st);
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, a.cnt, pa.cur.cap);
+  if (ev) (void)hipEventRecord(ev[3], st);
   return hipGetLastError();
 }
 
@@ -677,8 +689,8 @@ hipError_t launch_inject(const StepArgs &a, const PoolArgs<R> &pa, const gxt_pri
   return hipGetLastError();
 }
 
-template hipError_t launch_step<double>(const StepArgs &, const PoolArgs<double> &, int, hipStream_t);
-template hipError_t launch_step<float>(const StepArgs &, const PoolArgs<float> &, int, hipStream_t);
+template hipError_t launch_step<double>(const StepArgs &, const PoolArgs<double> &, int, hipStream_t, hipEvent_t *);
+template hipError_t launch_step<float>(const StepArgs &, const PoolArgs<float> &, int, hipStream_t, hipEvent_t *);
 template hipError_t launch_inject<double>(const StepArgs &, const PoolArgs<double> &, const gxt_primary *, int,
                                           hipStream_t);
 template hipError_t launch_inject<float>(const StepArgs &, const PoolArgs<float> &, const gxt_primary *, int,

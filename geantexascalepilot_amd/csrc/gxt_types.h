@@ -111,6 +111,8 @@ struct Counters {
   unsigned long long steps;   // track-steps
   unsigned long long killed;  // tracks killed by the step limit
   unsigned long long trace_n; // trace records written
+  unsigned long long interactions;  // tracks sampled by the interaction kernel
+  unsigned long long created;       // tracks created by the interaction kernel
 };
 
 // Per-event tallies on the device (FP64, atomics).

@@ -141,6 +141,11 @@ class Transport:
         self._c(lib.gxt_get_stats(self.ctx, C.byref(s)))
         return s
 
+    def kernel_stats(self):
+        s = abi.KernelStats()
+        self._c(lib.gxt_get_kernel_stats(self.ctx, C.byref(s)))
+        return s
+
     def reset_tallies(self):
         self._c(lib.gxt_reset_tallies(self.ctx))
 

@@ -6,8 +6,8 @@
  * No exception crosses this boundary.  Host buffers are owned by the caller;
  * all device memory (track pool, RNG states, tables, tallies) is owned by the
  * library.  A gxt_ctx is driven by one host thread and owns one HIP stream;
- * the component entry points (gxt_rng_*, gxt_table_*, gxt_sample_*,
- * gxt_helix_*) use a library-wide default context on device 0.
+ * gxt_table_value, gxt_macro_xs and gxt_sample use a context's physics data,
+ * gxt_rng_* and gxt_helix_step need no context (current HIP device).
  *
  * Units are CLHEP: mm, MeV, ns; magnetic field in tesla.
  * Species codes: 0 = gamma, 1 = electron, 2 = positron.
@@ -68,7 +68,7 @@ typedef struct gxt_config {
   int32_t max_jobs;         /* concurrent job slots                                 */
   int32_t trace_tracks;     /* >0: record the first trace_steps steps of tracks     */
   int32_t trace_steps;
-  int32_t pad1;
+  int32_t profile_kernels;  /* >0: time every transport / interaction launch (HIP events) */
   const char *data_dir;     /* directory holding the physics data files             */
 } gxt_config;
 
@@ -102,6 +102,19 @@ typedef struct gxt_stats {
   int64_t killed_by_limit;  /* tracks killed by max_steps              */
   double wall_ms;           /* device time of the last gxt_run          */
 } gxt_stats;
+
+/* Kernel timings of the step loop, accumulated while profile_kernels is set. */
+typedef struct gxt_kernel_stats {
+  double transport_ms;    /* summed duration of the transport-kernel launches      */
+  double interact_ms;     /* summed duration of the interaction-kernel launches    */
+  double advance_ms;      /* summed duration of the counter-advance launches        */
+  int64_t launches;       /* step-loop iterations timed                            */
+  int64_t track_steps;    /* tracks stepped (one per live track per iteration)     */
+  int64_t interactions;   /* tracks sampled by the interaction kernel              */
+  int64_t created;        /* tracks created by the interaction kernel              */
+  int32_t record_bytes;   /* bytes of one track record in the pool                 */
+  int32_t pad;
+} gxt_kernel_stats;
 
 /* Completed job, as returned by gxt_poll. */
 typedef struct gxt_job_result {
@@ -160,6 +173,7 @@ int gxt_get_event_tallies(gxt_ctx *ctx, uint64_t job_id, gxt_event_tally *out, i
  * over [-3, 5). */
 int gxt_get_spectra(gxt_ctx *ctx, int64_t *counts);
 int gxt_get_stats(gxt_ctx *ctx, gxt_stats *out);
+int gxt_get_kernel_stats(gxt_ctx *ctx, gxt_kernel_stats *out);
 int gxt_reset_tallies(gxt_ctx *ctx);
 /* Step trace of traced tracks: records of 12 doubles
  * {track uid (bits), step, species, process, ekin, x, y, z, dx, dy, dz, edep}.

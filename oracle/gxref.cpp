@@ -1306,11 +1306,24 @@ static int propagate(const gxref_ctx &c, Track &t, double pstep, double &step) {
     if (pt2 < 0.0) pt2 = 0.0;
     const double pt = std::sqrt(pt2);
     const double sl = pt > 0.0 ? 2.0 * std::sqrt(0.1 / (0.299792458 * c.bmag / pt)) : kHuge;
-    double s;
-    if (slin <= sl && slin <= remaining)
-      s = std::max(slin, 1.0e-6);
-    else
-      s = std::min(remaining, std::max(sl, saf));
+    if (slin <= sl && slin <= remaining) {
+      // The straight line reaches the boundary within the sagitta-limited
+      // length: step along the helix by that length and cross the aimed face
+      // by construction, the track placed on the plane (the curved path
+      // deviates from the chord by less than the 0.1 mm deflection tolerance).
+      const double s = std::max(slin, 1.0e-6);
+      helix_step(c.bfield, c.bmag, q, mom, s, t.pos, t.dir);
+      step += s;
+      if (face == 0 || face == 1) {
+        const int nv = face == 1 ? t.vol + 1 : t.vol - 1;
+        t.pos[2] = g.z[face == 1 ? t.vol + 1 : t.vol];
+        if (nv < 0 || nv >= g.nvol() || std::fabs(t.pos[0]) > g.hx || std::fabs(t.pos[1]) > g.hy) return kEscaped;
+        t.vol = nv;
+        return kBoundary;
+      }
+      return kEscaped;
+    }
+    const double s = std::min(remaining, std::max(sl, saf));
     helix_step(c.bfield, c.bmag, q, mom, s, t.pos, t.dir);
     step += s;
     remaining -= s;

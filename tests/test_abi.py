@@ -38,6 +38,7 @@ STRUCTS = {
     "gxt_stats": abi.Stats,
     "gxt_job_result": abi.JobResult,
     "gxt_sample_io": abi.SampleIO,
+    "gxt_kernel_stats": abi.KernelStats,
 }
 
 
