@@ -4,8 +4,9 @@
 // precision form (table interpolation, rejection sampling, helix in FP32;
 // positions, safety, boundary distances and tallies stay FP64).  The
 // algorithms restate the reference's leaf functions (file:line citations are
-// relative to GeantExascalePilot/source/Geant/) with the quirk fixes Q1-Q20
-// of the physics contract; the CPU oracle implements the same contract.
+// relative to GeantExascalePilot/source/Geant/) with the fixes of the
+// reference's hard-coded values listed beside each function; the CPU oracle
+// implements the same contract.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -103,7 +104,7 @@ __device__ __forceinline__ void matvec_mod(const uint32_t *M, uint64_t v[3], uin
 
 // Substream idx of the stream seeded with seed: (A^(2^76))^idx seed, by
 // binary powering over the jump matrices A^(2^(76+j)).
-__device__ inline void substream_state(const DevParams *P, uint64_t idx, double out[6]) {
+__device__ __forceinline__ void substream_state(const DevParams *P, uint64_t idx, double out[6]) {
   uint64_t v1[3] = {P->seed[0], P->seed[1], P->seed[2]};
   uint64_t v2[3] = {P->seed[3], P->seed[4], P->seed[5]};
   for (int j = 0; j < 64; ++j) {
@@ -169,7 +170,8 @@ __device__ __forceinline__ R pvec_value(const Nodes<R> &N, const DevTable &t, R 
   return spline(N, o + bin, e);
 }
 
-// InvRangeValue (ProxyPhysicsVector.cu:491-524): binary search (Q7).
+// InvRangeValue (ProxyPhysicsVector.cu:491-524): binary search (the reference calls the
+// log-bin Value() on this non-log vector).
 template <class R>
 __device__ __forceinline__ R pvec_inv_value(const Nodes<R> &N, const DevTable &t, R r) {
   const int o = t.off, n = t.n;
@@ -224,7 +226,7 @@ __device__ __forceinline__ R sb_value(const R *g, R x, R y) {
 }
 
 // Sandia photo-absorption per atom (mm2), interval = last edge <= E.
-__device__ inline double sandia_atom(const DevParams *P, const DevElement &el, double ekin) {
+__device__ __forceinline__ double sandia_atom(const DevParams *P, const DevElement &el, double ekin) {
   const double ek = ekin * 1.0e3;
   const double *rows = P->sandia + el.sandia;
   int row = el.nint - 1;
@@ -235,7 +237,7 @@ __device__ inline double sandia_atom(const DevParams *P, const DevElement &el, d
   return (el.Z / el.zoa) / kAvogadro * s * 100.0;
 }
 
-__device__ inline double sandia_macro(const DevParams *P, const DevMaterial &m, double ekin) {
+__device__ __forceinline__ double sandia_macro(const DevParams *P, const DevMaterial &m, double ekin) {
   double s = 0;
   for (int i = 0; i < m.nel; ++i) s += m.el[i].natoms * sandia_atom(P, m.el[i], ekin);
   return s;
@@ -254,9 +256,10 @@ __device__ __forceinline__ R heitler_xs(R ekin) {
 }
 
 // Macroscopic cross-section (1/mm) of discrete process p of a species
-// (ProxyEmProcess::GetLambda and the processes' GetLambda; Q1, Q4, Q5).
+// (ProxyEmProcess::GetLambda and the processes' GetLambda) with the track's
+// couple, Compton above 1 MeV and photoelectric from LambdaPrim/E.
 template <class R>
-__device__ inline R macro_xs(const DevParams *P, const Nodes<R> &N, const DevMaterial &m, int species, int p, R e) {
+__device__ __forceinline__ R macro_xs(const DevParams *P, const Nodes<R> &N, const DevMaterial &m, int species, int p, R e) {
   if (species == 0) {
     if (p == 0) {
       if ((double)e < m.compt_split) return pvec_value(N, m.tab[GXT_TAB_LAMBDA_COMPT], e);
@@ -330,9 +333,9 @@ __device__ __forceinline__ void ia_init(Interaction<R> &o, R e, const R d[3], R 
   (void)e;
 }
 
-// Element choice for compounds (Q2): kind 0 = n Z(Z+1), kind 1 = partial Sandia.
+// Element choice for compounds (the reference hard-codes Z): kind 0 = n Z(Z+1), kind 1 = partial Sandia.
 template <class R>
-__device__ inline int select_element(const DevParams *P, const DevMaterial &m, int kind, R e, Rng &rng) {
+__device__ __forceinline__ int select_element(const DevParams *P, const DevMaterial &m, int kind, R e, Rng &rng) {
   if (m.nel == 1) return 0;
   double w[kMaxElem], tot = 0;
   for (int i = 0; i < m.nel; ++i) {
@@ -350,7 +353,7 @@ __device__ inline int select_element(const DevParams *P, const DevMaterial &m, i
 
 // Klein-Nishina (proxy/ProxyKleinNishina.hpp:96-166).
 template <class R>
-__device__ inline void sample_compton(R e0, const R d0[3], Rng &rng, Interaction<R> &o) {
+__device__ __forceinline__ void sample_compton(R e0, const R d0[3], Rng &rng, Interaction<R> &o) {
   const R lowE = R(100.0e-6);
   ia_init(o, e0, d0, e0);
   if (e0 < lowE) return;
@@ -383,7 +386,8 @@ __device__ inline void sample_compton(R e0, const R d0[3], Rng &rng, Interaction
     o.d[0] = g0;
     o.d[1] = g1;
     o.d[2] = g2;
-    Sec<R> &el = o.sec[o.nsec++];
+    Sec<R> &el = o.sec[0];
+    o.nsec = 1;
     el.species = 1;
     el.e = te;
     el.d[0] = e0 * d0[0] - e1 * g0;
@@ -410,9 +414,10 @@ template <class R> __device__ __forceinline__ R coulomb_factor(R Z) {
   return (k1 * az4 + k2 + R(1) / (R(1) + az2)) * az2 - (k3 * az4 + k4) * az4;
 }
 
-// Bethe-Heitler (proxy/ProxyBetheHeitler.hpp:64-182; Q2, Q19).
+// Bethe-Heitler (proxy/ProxyBetheHeitler.hpp:64-182): element from the material, Tsai
+// angle used as an angle.
 template <class R>
-__device__ inline void sample_pair(const DevParams *P, const DevMaterial &m, R e0, const R d0[3], Rng &rng,
+__device__ __forceinline__ void sample_pair(const DevParams *P, const DevMaterial &m, R e0, const R d0[3], Rng &rng,
                                    Interaction<R> &o) {
   ia_init(o, e0, d0, e0);
   const R eps0 = R(kMe) / e0;
@@ -479,9 +484,10 @@ __device__ inline void sample_pair(const DevParams *P, const DevMaterial &m, R e
   o.ekin = R(0);
 }
 
-// Sauter-Gavrila (proxy/ProxySauterGavrila.hpp:66-130; Q2, Q12, Q17).
+// Sauter-Gavrila (proxy/ProxySauterGavrila.hpp:66-130): element by partial Sandia
+// cross-section, first shell bound below E, binding energy deposited.
 template <class R>
-__device__ inline void sample_photo(const DevParams *P, const DevMaterial &m, R e0, const R d0[3], Rng &rng,
+__device__ __forceinline__ void sample_photo(const DevParams *P, const DevMaterial &m, R e0, const R d0[3], Rng &rng,
                                     Interaction<R> &o) {
   ia_init(o, e0, d0, R(0));
   const int ie = select_element(P, m, 1, e0, rng);
@@ -527,9 +533,10 @@ __device__ inline void sample_photo(const DevParams *P, const DevMaterial &m, R 
   o.edep = bind;
 }
 
-// Seltzer-Berger (proxy/ProxySeltzerBerger.hpp:204-315; Q2, Q3, Q15, Q16, Q20).
+// Seltzer-Berger (proxy/ProxySeltzerBerger.hpp:204-315): element from the material, its
+// own grid, Migdal factor, gamma cut of the couple, electron kinetic energy.
 template <class R>
-__device__ inline void sample_brems(const DevParams *P, const DevMaterial &m, R t0, const R d0[3], Rng &rng,
+__device__ __forceinline__ void sample_brems(const DevParams *P, const DevMaterial &m, R t0, const R d0[3], Rng &rng,
                                     Interaction<R> &o) {
   ia_init(o, t0, d0, t0);
   const R cut = rmin((R)m.cut_gamma, t0);
@@ -585,9 +592,10 @@ __device__ inline void sample_brems(const DevParams *P, const DevMaterial &m, R 
   unit3(o.d[0], o.d[1], o.d[2]);
 }
 
-// Moller / Bhabha (proxy/ProxyMollerScattering.hpp:92-159; Q6; Bhabha branch A12).
+// Moller / Bhabha (proxy/ProxyMollerScattering.hpp:92-159, electron cut of the couple;
+// the Bhabha branch from Geant4's G4MollerBhabhaModel, absent in the reference).
 template <class R>
-__device__ inline void sample_ioni(const DevMaterial &m, bool positron, R t0, const R d0[3], Rng &rng,
+__device__ __forceinline__ void sample_ioni(const DevMaterial &m, bool positron, R t0, const R d0[3], Rng &rng,
                                    Interaction<R> &o) {
   ia_init(o, t0, d0, t0);
   const R tmin = (R)m.cut_e;
@@ -649,9 +657,10 @@ __device__ inline void sample_ioni(const DevMaterial &m, bool positron, R t0, co
   unit3(o.d[0], o.d[1], o.d[2]);
 }
 
-// e+ annihilation in flight (Heitler; A12).
+// e+ annihilation in flight (Heitler; Geant4's G4eeToTwoGammaModel, absent in
+// the reference).
 template <class R>
-__device__ inline void sample_annihil(R t0, const R d0[3], Rng &rng, Interaction<R> &o) {
+__device__ __forceinline__ void sample_annihil(R t0, const R d0[3], Rng &rng, Interaction<R> &o) {
   ia_init(o, t0, d0, R(0));
   const R tau = t0 / R(kMe);
   const R gam = tau + R(1);
@@ -691,7 +700,7 @@ __device__ inline void sample_annihil(R t0, const R d0[3], Rng &rng, Interaction
 }
 
 template <class R>
-__device__ inline void sample_model(const DevParams *P, const DevMaterial &m, int model, R e, const R d[3], Rng &rng,
+__device__ __forceinline__ void sample_model(const DevParams *P, const DevMaterial &m, int model, R e, const R d[3], Rng &rng,
                                     Interaction<R> &o) {
   switch (model) {
     case GXT_MODEL_KLEIN_NISHINA: sample_compton(e, d, rng, o); break;
@@ -703,6 +712,32 @@ __device__ inline void sample_model(const DevParams *P, const DevMaterial &m, in
     case GXT_MODEL_BHABHA: sample_ioni(m, true, e, d, rng, o); break;
     default: sample_annihil(e, d, rng, o); break;
   }
+}
+
+// Samplers of the photon queues (0 Compton, 1 conversion, 2 photoelectric).
+template <class R>
+__device__ __forceinline__ void sample_neutral(const DevParams *P, const DevMaterial &m, int q, R e, const R d[3],
+                                               Rng &rng, Interaction<R> &o) {
+  if (q == 0)
+    sample_compton(e, d, rng, o);
+  else if (q == 1)
+    sample_pair(P, m, e, d, rng, o);
+  else
+    sample_photo(P, m, e, d, rng, o);
+}
+
+// Samplers of the e-/e+ queues (3, 5 bremsstrahlung, 4 Moller, 6 Bhabha, 7 annihilation).
+template <class R>
+__device__ __forceinline__ void sample_charged(const DevParams *P, const DevMaterial &m, int q, R e, const R d[3],
+                                               Rng &rng, Interaction<R> &o) {
+  if (q == 3 || q == 5)
+    sample_brems(P, m, e, d, rng, o);
+  else if (q == 4)
+    sample_ioni(m, false, e, d, rng, o);
+  else if (q == 6)
+    sample_ioni(m, true, e, d, rng, o);
+  else
+    sample_annihil(e, d, rng, o);
 }
 
 // queue index -> species, process, model
@@ -721,7 +756,8 @@ __device__ __forceinline__ int queue_model(int q) {
 }
 
 // ------------------------------------------------------------------ continuous loss
-// ProxyIonization::AlongStepGPIL (ProxyIonization.hpp:108-128; Q6, Q18).
+// ProxyIonization::AlongStepGPIL (ProxyIonization.hpp:108-128): production-cut
+// length for finR and Geant4's (2 - finR/range) form.
 template <class R>
 __device__ __forceinline__ R along_limit(const Nodes<R> &N, const DevMaterial &m, bool pos, R e, R mine) {
   R range = pvec_value(N, m.tab[pos ? GXT_TAB_RANGE_EP : GXT_TAB_RANGE_EM], e);
@@ -730,7 +766,8 @@ __device__ __forceinline__ R along_limit(const Nodes<R> &N, const DevMaterial &m
   return range > finR ? range * R(0.2) + finR * R(0.8) * (R(2.0) - finR / range) : range;
 }
 
-// ProxyIonization::AlongStepDoIt (ProxyIonization.hpp:130-168; Q7).
+// ProxyIonization::AlongStepDoIt (ProxyIonization.hpp:130-168): binary-search
+// inverse range.
 template <class R>
 __device__ __forceinline__ R along_doit(const Nodes<R> &N, const DevMaterial &m, bool pos, R e, R step, R mine,
                                         R &eloss) {
@@ -755,7 +792,7 @@ __device__ __forceinline__ R along_doit(const Nodes<R> &N, const DevMaterial &m,
 // ------------------------------------------------------------------ geometry
 // Analytic slab/box navigator (replaces the VecGeom navigation of
 // geometry/NavigationInterface.hpp:42-98).  Positions and distances are FP64.
-__device__ inline int locate(const DevGeometry &g, const double p[3], double d0, double d1, double d2) {
+__device__ __forceinline__ int locate(const DevGeometry &g, const double p[3], double d0, double d1, double d2) {
   const double ax = fabs(p[0]), ay = fabs(p[1]);
   if (ax > g.hx || ay > g.hy) return -1;
   if (ax == g.hx && p[0] * d0 > 0) return -1;
@@ -816,7 +853,7 @@ __device__ __forceinline__ double safety(const DevGeometry &g, int v, const doub
 // in mm/MeV/tesla; the direction is renormalised (FieldPropagationHandler.cpp:323-330).
 // The arithmetic is in R; the position update is accumulated in FP64.
 template <class R>
-__device__ inline void helix_step(const double B[3], double bmag, R charge, R mom, R step, double pos[3], R dir[3]) {
+__device__ __forceinline__ void helix_step(const double B[3], double bmag, R charge, R mom, R step, double pos[3], R dir[3]) {
   const R ub[3] = {(R)(B[0] / bmag), (R)(B[1] / bmag), (R)(B[2] / bmag)};
   const R uvb = dir[0] * ub[0] + dir[1] * ub[1] + dir[2] * ub[2];
   R dt2 = dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2] - uvb * uvb;
@@ -843,7 +880,7 @@ enum { kPhysics = 0, kBoundary = 1, kEscaped = 2, kLimit = 3 };
 // (FieldPropagationHandler.cpp:87-360; LinearPropagationHandler.cpp:13-63;
 // driver loop test_BasicCpuTransport.cpp:319-336).
 template <class R>
-__device__ inline int propagate(const DevParams *P, int species, R e, double pos[3], R dir[3], int &vol, double pstep,
+__device__ __forceinline__ int propagate(const DevParams *P, int species, R e, double pos[3], R dir[3], int &vol, double pstep,
                                 double &step) {
   const DevGeometry &g = P->geo;
   if (species == 0 || P->bmag == 0.0) {

@@ -128,6 +128,9 @@ struct DevPool {
     HIP_TRY(dalloc(&p.event, cap));
     HIP_TRY(dalloc(&p.meta, cap));
     HIP_TRY(dalloc(&p.nsteps, cap));
+    HIP_TRY(dalloc(&p.aed, cap));
+    HIP_TRY(dalloc(&p.avis, cap));
+    HIP_TRY(dalloc(&p.alen, cap));
     if (trace) {
       HIP_TRY(dalloc(&p.uid, cap));
       HIP_TRY(dalloc(&p.tedep, cap));
@@ -136,7 +139,7 @@ struct DevPool {
   }
   void release() {
     void *ptrs[] = {p.x, p.y, p.z, p.dx, p.dy, p.dz, p.e, p.nil0, p.nil1, p.nil2, p.rng, p.event, p.meta,
-                    p.nsteps, p.uid, p.tedep};
+                    p.nsteps, p.aed, p.avis, p.alen, p.uid, p.tedep};
     for (void *q : ptrs)
       if (q) (void)hipFree(q);
     p = Pool<R>{};
@@ -149,11 +152,16 @@ double now_ms(std::chrono::steady_clock::time_point t0) {
 
 struct Job {
   uint64_t id = 0;
-  std::vector<gxt_primary> prims;
+  std::vector<gxt_primary> prims;   // sorted by event
+  std::vector<int> ev_first;        // prims[ev_first[e] .. ev_first[e+1]) belong to event e
+  std::vector<double> ev_gev;       // summed primary energy of each event (GeV)
+  std::vector<double> ev_fp;        // footprint committed for an event in flight (tracks)
+  std::vector<char> ev_state;       // 0 pending, 1 in flight, 2 finished
   int n_events = 0;
-  int slot = -1;
-  int ev_base = -1;
-  double footprint = 0;
+  int next_event = 0;               // first event not yet injected
+  int n_done = 0;
+  int ev_base = -1;                 // first event slot of the job
+  bool started = false;
   double submit_ms = 0, admit_ms = 0, done_ms = 0;
   std::vector<gxt_event_tally> tallies;
   int64_t steps = 0;
@@ -180,8 +188,8 @@ struct gxt_ctx {
   Counters *cnt = nullptr;
   Counters *hcnt = nullptr;  // pinned
   DevEvent *ev = nullptr;
-  int *job_live = nullptr;
-  int *hjob_live = nullptr;  // pinned
+  int *ev_live = nullptr;    // live tracks per event slot
+  int *hev_live = nullptr;   // pinned mirror
   double *rows = nullptr;
   unsigned long long *spec = nullptr;
   int64_t row_stride = 0;
@@ -191,13 +199,14 @@ struct gxt_ctx {
   int64_t stage_cap = 0;
   // jobs
   std::mutex mu;
-  std::deque<std::unique_ptr<Job>> pending;
-  std::vector<std::unique_ptr<Job>> active;   // index = job slot
+  std::deque<std::unique_ptr<Job>> jobs;   // submitted, not finished (FIFO)
   std::vector<std::unique_ptr<Job>> done_unpolled;
   std::map<uint64_t, std::unique_ptr<Job>> finished;
   std::vector<char> ev_used;
-  double footprint_used = 0;
-  double tracks_per_gev = 1000.0;  // admission estimate of peak live tracks per GeV of primary energy
+  double committed = 0;             // summed footprint of events in flight
+  double tracks_per_gev = 3000.0;   // admission estimate of peak live tracks per GeV (adapted)
+  double peak_ratio = 0;            // highest observed live tracks per GeV in flight
+  int64_t syncs_observed = 0;
   // kernel timing (profile_kernels)
   bool prof = false;
   std::vector<std::array<hipEvent_t, 4>> ev_free, ev_pending;
@@ -218,7 +227,7 @@ struct gxt_ctx {
     a.P = dd.P;
     a.cnt = cnt;
     a.ev = ev;
-    a.job_live = job_live;
+    a.ev_live = ev_live;
     a.queues = queues;
     a.rows.bins = rows;
     a.rows.spec = spec;
@@ -236,10 +245,9 @@ struct gxt_ctx {
 
 namespace {
 
-double job_footprint(const gxt_ctx *c, const std::vector<gxt_primary> &p) {
-  double f = 0;
-  for (const gxt_primary &q : p) f += 16.0 + c->tracks_per_gev * q.ekin * 1.0e-3;
-  return f;
+// Estimated peak live tracks of an event, from its primaries' energy.
+double event_footprint(const gxt_ctx *c, const Job *j, int e) {
+  return 16.0 * (j->ev_first[e + 1] - j->ev_first[e]) + c->tracks_per_gev * j->ev_gev[e];
 }
 
 int alloc_events(gxt_ctx *c, int n) {
@@ -256,59 +264,67 @@ int alloc_events(gxt_ctx *c, int n) {
   return -1;
 }
 
-// Admit pending jobs whose footprint fits; inject their primaries.
+// Admit events, oldest job first, while their estimated footprint fits the
+// pool budget; inject their primaries with one kernel launch.  Events are
+// admitted whole (all primaries of an event together).
 int admit(gxt_ctx *c) {
-  std::vector<Job *> admitted;
+  const double budget = 0.8 * (double)c->cap;
+  std::vector<gxt_primary> buf;
+  std::vector<std::vector<DevEvent>> evbufs;  // alive until the stream synchronises
+  std::vector<std::pair<int, int>> zero_ranges;
   {
     std::lock_guard<std::mutex> lk(c->mu);
-    while (!c->pending.empty()) {
-      Job *j = c->pending.front().get();
-      const double budget = 0.85 * (double)c->cap;
-      const bool nothing_active = c->footprint_used == 0.0;
-      if (!nothing_active && c->footprint_used + j->footprint > budget) break;
-      int slot = -1;
-      for (int s = 0; s < (int)c->active.size(); ++s)
-        if (!c->active[s]) {
-          slot = s;
+    int active = 0;
+    for (auto &jp : c->jobs) {
+      Job *j = jp.get();
+      if (!j->started) {
+        if (active >= c->cfg.max_jobs) break;
+        const int eb = alloc_events(c, j->n_events);
+        if (eb < 0) break;
+        j->ev_base = eb;
+        j->started = true;
+        j->admit_ms = now_ms(c->t0);
+        zero_ranges.emplace_back(eb, j->n_events);
+      }
+      ++active;
+      bool full = false;
+      while (j->next_event < j->n_events) {
+        const int e = j->next_event;
+        const double fp = event_footprint(c, j, e);
+        if (c->committed > 0 && c->committed + fp > budget) {
+          full = true;
           break;
         }
-      if (slot < 0) break;
-      const int eb = alloc_events(c, j->n_events);
-      if (eb < 0) break;
-      j->slot = slot;
-      j->ev_base = eb;
-      j->admit_ms = now_ms(c->t0);
-      c->footprint_used += j->footprint;
-      c->active[slot] = std::move(c->pending.front());
-      c->pending.pop_front();
-      admitted.push_back(j);
+        for (int k = j->ev_first[e]; k < j->ev_first[e + 1]; ++k) {
+          gxt_primary p = j->prims[k];
+          p.event += j->ev_base;
+          buf.push_back(p);
+        }
+        j->ev_state[e] = 1;
+        j->ev_fp[e] = fp;
+        c->committed += fp;
+        j->next_event++;
+      }
+      if (full) break;
     }
   }
-  if (admitted.empty()) return GXT_OK;
-  int64_t total = 0;
-  for (Job *j : admitted) total += (int64_t)j->prims.size();
+  for (auto &r : zero_ranges) {
+    evbufs.emplace_back(r.second);
+    std::memset(evbufs.back().data(), 0, r.second * sizeof(DevEvent));
+    HIP_TRY(hipMemcpyAsync(c->ev + r.first, evbufs.back().data(), r.second * sizeof(DevEvent),
+                           hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemsetAsync(c->ev_live + r.first, 0, r.second * sizeof(int), c->stream));
+  }
+  if (buf.empty()) {
+    if (!zero_ranges.empty()) HIP_TRY(hipStreamSynchronize(c->stream));
+    return GXT_OK;
+  }
+  const int64_t total = (int64_t)buf.size();
   if (total > c->stage_cap) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
     (void)hipFree(c->stage);
     c->stage_cap = std::max<int64_t>(total, 4096);
     HIP_TRY(dalloc(&c->stage, c->stage_cap));
-  }
-  std::vector<gxt_primary> buf;
-  buf.reserve(total);
-  std::vector<std::vector<DevEvent>> evbufs;  // alive until the stream synchronises
-  for (Job *j : admitted) {
-    evbufs.emplace_back(j->n_events);
-    std::vector<DevEvent> &evs = evbufs.back();
-    std::memset(evs.data(), 0, evs.size() * sizeof(DevEvent));
-    for (DevEvent &e : evs) e.job = j->slot;
-    HIP_TRY(hipMemcpyAsync(c->ev + j->ev_base, evs.data(), evs.size() * sizeof(DevEvent), hipMemcpyHostToDevice,
-                           c->stream));
-    c->hjob_live[j->slot] = (int)j->prims.size();
-    HIP_TRY(hipMemcpyAsync(c->job_live + j->slot, &c->hjob_live[j->slot], sizeof(int), hipMemcpyHostToDevice,
-                           c->stream));
-    for (gxt_primary p : j->prims) {
-      p.event += j->ev_base;
-      buf.push_back(p);
-    }
   }
   HIP_TRY(hipMemcpyAsync(c->stage, buf.data(), buf.size() * sizeof(gxt_primary), hipMemcpyHostToDevice, c->stream));
   const StepArgs a = c->args();
@@ -354,10 +370,11 @@ int iterate(gxt_ctx *c, int k) {
   return GXT_OK;
 }
 
-// Read counters and job live counts; complete finished jobs.
+// Read counters and the per-event live counts; finish events and jobs.
 int collect(gxt_ctx *c, bool &idle) {
   HIP_TRY(hipMemcpyAsync(c->hcnt, c->cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->hjob_live, c->job_live, c->cfg.max_jobs * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->hev_live, c->ev_live, c->cfg.max_events * sizeof(int), hipMemcpyDeviceToHost,
+                         c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (auto &q : c->ev_pending) {
     float t0 = 0, t1 = 0, t2 = 0;
@@ -374,12 +391,33 @@ int collect(gxt_ctx *c, bool &idle) {
   const Counters &h = *c->hcnt;
   if (h.overflow || (int64_t)h.cur[0] + h.cur[1] > c->cap)
     return c->fail(GXT_ECAPACITY, "track pool or queue overflow (increase pool_capacity)");
-  c->max_live = std::max<int64_t>(c->max_live, (int64_t)h.cur[0] + h.cur[1]);
+  const int64_t live = (int64_t)h.cur[0] + h.cur[1];
+  c->max_live = std::max<int64_t>(c->max_live, live);
   std::vector<Job *> fin;
+  double gev_in_flight = 0;
   {
     std::lock_guard<std::mutex> lk(c->mu);
-    for (auto &jp : c->active)
-      if (jp && c->hjob_live[jp->slot] == 0) fin.push_back(jp.get());
+    for (auto &jp : c->jobs) {
+      Job *j = jp.get();
+      if (!j->started) break;
+      for (int e = 0; e < j->next_event; ++e) {
+        if (j->ev_state[e] != 1) continue;
+        if (c->hev_live[j->ev_base + e] == 0) {
+          j->ev_state[e] = 2;
+          j->n_done++;
+          c->committed -= j->ev_fp[e];
+        } else {
+          for (int k = j->ev_first[e]; k < j->ev_first[e + 1]; ++k) gev_in_flight += j->prims[k].ekin * 1e-3;
+        }
+      }
+      if (j->n_done == j->n_events) fin.push_back(j);
+    }
+    if (c->committed < 1e-6) c->committed = 0;
+  }
+  // adapt the admission estimate to the observed peak live tracks per GeV
+  if (gev_in_flight > 0) {
+    c->peak_ratio = std::max(c->peak_ratio, (double)live / gev_in_flight);
+    if (++c->syncs_observed >= 16) c->tracks_per_gev = std::min(3000.0, std::max(100.0, 2.0 * c->peak_ratio));
   }
   for (Job *j : fin) {
     std::vector<DevEvent> evs(j->n_events);
@@ -402,22 +440,25 @@ int collect(gxt_ctx *c, bool &idle) {
     j->done_ms = now_ms(c->t0);
     std::lock_guard<std::mutex> lk(c->mu);
     for (int k = 0; k < j->n_events; ++k) c->ev_used[j->ev_base + k] = 0;
-    c->footprint_used -= j->footprint;
-    if (c->footprint_used < 1e-6) c->footprint_used = 0;
-    std::unique_ptr<Job> owned = std::move(c->active[j->slot]);
-    owned->prims.clear();
-    owned->prims.shrink_to_fit();
-    c->done_unpolled.emplace_back(new Job(*owned));  // completion order for gxt_poll
-    c->finished[owned->id] = std::move(owned);
+    for (auto it = c->jobs.begin(); it != c->jobs.end(); ++it)
+      if (it->get() == j) {
+        std::unique_ptr<Job> owned = std::move(*it);
+        c->jobs.erase(it);
+        owned->prims.clear();
+        owned->prims.shrink_to_fit();
+        c->done_unpolled.emplace_back(new Job(*owned));  // completion order for gxt_poll
+        c->finished[owned->id] = std::move(owned);
+        break;
+      }
   }
-  bool any_active = false;
+  bool any_started = false;
   {
     std::lock_guard<std::mutex> lk(c->mu);
-    for (auto &jp : c->active)
-      if (jp) any_active = true;
-    idle = !any_active && c->pending.empty();
+    for (auto &jp : c->jobs)
+      if (jp->started) any_started = true;
+    idle = c->jobs.empty();
   }
-  if (!any_active && (h.cur[0] + h.cur[1]) != 0)
+  if (!any_started && live != 0)
     return c->fail(GXT_ESTATE, "tracks alive with no active job (live-count bookkeeping error)");
   return GXT_OK;
 }
@@ -429,8 +470,8 @@ int round_trip(gxt_ctx *c, int k, bool &idle) {
   bool any = false;
   {
     std::lock_guard<std::mutex> lk(c->mu);
-    for (auto &jp : c->active)
-      if (jp) any = true;
+    for (auto &jp : c->jobs)
+      if (jp->started) any = true;
   }
   if (any) {
     rc = iterate(c, k);
@@ -475,8 +516,8 @@ void destroy_ctx(gxt_ctx *c) {
   (void)hipFree(c->cnt);
   (void)hipHostFree(c->hcnt);
   (void)hipFree(c->ev);
-  (void)hipFree(c->job_live);
-  (void)hipHostFree(c->hjob_live);
+  (void)hipFree(c->ev_live);
+  (void)hipHostFree(c->hev_live);
   (void)hipFree(c->rows);
   (void)hipFree(c->spec);
   (void)hipFree(c->trace_buf);
@@ -611,10 +652,10 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
   std::memset(c->hcnt, 0, sizeof(Counters));
   HIP_TRY(dalloc(&c->ev, cfg->max_events));
   HIP_TRY(hipMemset(c->ev, 0, sizeof(DevEvent) * cfg->max_events));
-  HIP_TRY(dalloc(&c->job_live, cfg->max_jobs));
-  HIP_TRY(hipMemset(c->job_live, 0, sizeof(int) * cfg->max_jobs));
-  HIP_TRY(hipHostMalloc((void **)&c->hjob_live, sizeof(int) * cfg->max_jobs, hipHostMallocDefault));
-  std::memset(c->hjob_live, 0, sizeof(int) * cfg->max_jobs);
+  HIP_TRY(dalloc(&c->ev_live, cfg->max_events));
+  HIP_TRY(hipMemset(c->ev_live, 0, sizeof(int) * cfg->max_events));
+  HIP_TRY(hipHostMalloc((void **)&c->hev_live, sizeof(int) * cfg->max_events, hipHostMallocDefault));
+  std::memset(c->hev_live, 0, sizeof(int) * cfg->max_events);
   c->row_stride = cfg->n_zbins + cfg->n_rbins + P.geo.nvol;
   HIP_TRY(dalloc(&c->rows, (size_t)c->grid * c->row_stride));
   HIP_TRY(hipMemset(c->rows, 0, sizeof(double) * c->grid * c->row_stride));
@@ -624,7 +665,6 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
     c->trace_cap = 4000000;
     HIP_TRY(dalloc(&c->trace_buf, (size_t)12 * c->trace_cap));
   }
-  c->active.resize(cfg->max_jobs);
   c->ev_used.assign(cfg->max_events, 0);
   HIP_TRY(hipDeviceSynchronize());
   *out = c.release();
@@ -653,11 +693,21 @@ int gxt_submit(gxt_ctx *c, const gxt_primary *prim, int n, uint64_t job_id) {
   }
   j->n_events = maxev + 1;
   if (j->n_events > c->cfg.max_events) return c->fail(GXT_ECAPACITY, "job has more events than max_events");
-  j->footprint = job_footprint(c, j->prims);
+  std::stable_sort(j->prims.begin(), j->prims.end(),
+                   [](const gxt_primary &a, const gxt_primary &b) { return a.event < b.event; });
+  j->ev_first.assign(j->n_events + 1, 0);
+  for (const gxt_primary &p : j->prims) j->ev_first[p.event + 1]++;
+  for (int e = 0; e < j->n_events; ++e) j->ev_first[e + 1] += j->ev_first[e];
+  j->ev_gev.assign(j->n_events, 0.0);
+  j->ev_fp.assign(j->n_events, 0.0);
+  for (const gxt_primary &p : j->prims) j->ev_gev[p.event] += p.ekin * 1.0e-3;
+  j->ev_state.assign(j->n_events, 0);
   j->submit_ms = now_ms(c->t0);
   std::lock_guard<std::mutex> lk(c->mu);
   if (c->finished.count(job_id)) return c->fail(GXT_EINVAL, "job id already used");
-  c->pending.push_back(std::move(j));
+  for (const auto &q : c->jobs)
+    if (q->id == job_id) return c->fail(GXT_EINVAL, "job id already used");
+  c->jobs.push_back(std::move(j));
   return GXT_OK;
 }
 
@@ -775,8 +825,9 @@ int gxt_get_kernel_stats(gxt_ctx *c, gxt_kernel_stats *o) {
   o->interactions = (int64_t)h.interactions;
   o->created = (int64_t)h.created;
   // x, y, z (FP64) + dir, E, 3 interaction lengths (R) + 6 RNG words + event, meta, nsteps
+  // + 3 FP64 per-track tallies
   const int r = c->precision == GXT_FP32 ? 4 : 8;
-  o->record_bytes = 3 * 8 + 7 * r + 6 * 4 + 3 * 4;
+  o->record_bytes = 3 * 8 + 7 * r + 6 * 4 + 3 * 4 + 3 * 8;
   return GXT_OK;
 }
 

@@ -19,7 +19,7 @@ struct StepArgs {
   const DevParams *P;
   Counters *cnt;
   DevEvent *ev;
-  int *job_live;
+  int *ev_live;      // live tracks per event slot
   int32_t *queues;  // kNumQueues x cap
   TallyRows rows;
   double *trace;

@@ -22,6 +22,10 @@
 namespace gxt {
 
 constexpr int kBlock = 256;
+// Minimum waves per SIMD the step kernels are compiled for (register budget).
+#ifndef GXT_STEP_WAVES
+#define GXT_STEP_WAVES 2
+#endif
 
 // ------------------------------------------------------------------ wave helpers
 __device__ __forceinline__ int lane() { return (int)__lane_id(); }
@@ -112,37 +116,37 @@ __device__ __forceinline__ void lds_flush(const DevParams *P, LdsTally &t, const
     if (t.spec[i]) srow[i] += t.spec[i];
 }
 
-__device__ __forceinline__ void deposit(const DevParams *P, LdsTally &t, DevEvent *ev, int event, double e,
-                                        const double p[3], int vol) {
-  if (e <= 0.0) return;
-  unsafeAtomicAdd(&ev[event].edep, e);
-  if (P->geo.visible[vol]) unsafeAtomicAdd(&ev[event].visible, e);
-  unsafeAtomicAdd(&t.v[vol], e);
-  int iz = (int)floor(p[2] / P->zbin);
-  iz = iz < 0 ? 0 : (iz >= P->nz ? P->nz - 1 : iz);
-  unsafeAtomicAdd(&t.z[iz], e);
-  const double r = sqrt(p[0] * p[0] + p[1] * p[1]);
-  int ir = (int)(r / P->rbin);
-  ir = ir < 0 ? 0 : (ir >= P->nr ? P->nr - 1 : ir);
-  unsafeAtomicAdd(&t.r[ir], e);
-}
-
 __device__ __forceinline__ void spectrum(LdsTally &t, int species, double e) {
   const int b = (int)floor((m_log10(e) + 3.0) * 8.0);
   if (b >= 0 && b < kSpecBins) atomicAdd(&t.spec[species * kSpecBins + b], 1u);
 }
 
 // ------------------------------------------------------------------ pool I/O
+// The three interaction lengths are named members so that, with the process
+// loops unrolled, every access is static and stays in registers.
 template <class R>
 struct TrackRegs {
   double pos[3];
   R dir[3];
   R e;
-  R nil[3];
+  R nil0, nil1, nil2;
   Rng rng;
   int event, vol, species, nsteps;
+  double aed, avis, alen;
   uint64_t uid;
+  // value selects (a store through a selected member address would force the
+  // whole struct out of registers)
+  __device__ __forceinline__ void set_nil(int p, R v) {
+    nil0 = p == 0 ? v : nil0;
+    nil1 = p == 1 ? v : nil1;
+    nil2 = p == 2 ? v : nil2;
+  }
 };
+
+// First RNG word of a track whose state is still to be derived from its
+// substream index (words 1, 2 hold the index); never a valid MRG32k3a word,
+// which is below m1 = 4294967087.
+constexpr uint32_t kRngPending = 0xffffffffu;
 
 template <class R>
 __device__ __forceinline__ void load_track(const Pool<R> &p, int64_t i, TrackRegs<R> &t, bool trace) {
@@ -153,15 +157,18 @@ __device__ __forceinline__ void load_track(const Pool<R> &p, int64_t i, TrackReg
   t.dir[1] = p.dy[i];
   t.dir[2] = p.dz[i];
   t.e = p.e[i];
-  t.nil[0] = p.nil0[i];
-  t.nil[1] = p.nil1[i];
-  t.nil[2] = p.nil2[i];
+  t.nil0 = p.nil0[i];
+  t.nil1 = p.nil1[i];
+  t.nil2 = p.nil2[i];
   t.rng.load(p.rng, p.cap, i);
   t.event = p.event[i];
   const int meta = p.meta[i];
   t.vol = meta & 0xffff;
   t.species = meta >> 16;
   t.nsteps = p.nsteps[i];
+  t.aed = p.aed[i];
+  t.avis = p.avis[i];
+  t.alen = p.alen[i];
   t.uid = trace ? p.uid[i] : 0;
 }
 
@@ -174,53 +181,79 @@ __device__ __forceinline__ void store_track(const Pool<R> &p, int64_t i, const T
   p.dy[i] = t.dir[1];
   p.dz[i] = t.dir[2];
   p.e[i] = t.e;
-  p.nil0[i] = t.nil[0];
-  p.nil1[i] = t.nil[1];
-  p.nil2[i] = t.nil[2];
+  p.nil0[i] = t.nil0;
+  p.nil1[i] = t.nil1;
+  p.nil2[i] = t.nil2;
   t.rng.store(p.rng, p.cap, i);
   p.event[i] = t.event;
   p.meta[i] = t.vol | (t.species << 16);
   p.nsteps[i] = t.nsteps;
+  p.aed[i] = t.aed;
+  p.avis[i] = t.avis;
+  p.alen[i] = t.alen;
   if (trace) p.uid[i] = t.uid;
 }
 
-// A track created by an interaction or an annihilation at rest.
+// A track created by an interaction or an annihilation at rest, at the
+// position, volume and event of its parent; its RNG state is substream uid.
 template <class R>
-struct NewTrack {
-  int species;
-  R e;
-  R d[3];
-  uint64_t uid;
-};
-
-template <class R>
-__device__ inline void store_new(const DevParams *P, const Pool<R> &p, int64_t i, const double pos[3], int vol,
-                                 int event, const NewTrack<R> &n, bool trace) {
-  p.x[i] = pos[0];
-  p.y[i] = pos[1];
-  p.z[i] = pos[2];
-  p.dx[i] = n.d[0];
-  p.dy[i] = n.d[1];
-  p.dz[i] = n.d[2];
-  p.e[i] = n.e;
+__device__ inline void store_new(const DevParams *P, const Pool<R> &p, int64_t i, const TrackRegs<R> &par,
+                                 int species, R e, R d0, R d1, R d2, uint64_t uid, bool trace) {
+  p.x[i] = par.pos[0];
+  p.y[i] = par.pos[1];
+  p.z[i] = par.pos[2];
+  p.dx[i] = d0;
+  p.dy[i] = d1;
+  p.dz[i] = d2;
+  p.e[i] = e;
   p.nil0[i] = R(-1);
   p.nil1[i] = R(-1);
   p.nil2[i] = R(-1);
-  double s[6];
-  substream_state(P, n.uid, s);
-#pragma unroll
-  for (int k = 0; k < 6; ++k) p.rng[k * p.cap + i] = (uint32_t)s[k];
-  p.event[i] = event;
-  p.meta[i] = vol | (n.species << 16);
+  // the RNG state is derived from uid by the transport kernel at the first step
+  p.rng[i] = kRngPending;
+  p.rng[p.cap + i] = (uint32_t)uid;
+  p.rng[2 * p.cap + i] = (uint32_t)(uid >> 32);
+  p.event[i] = par.event;
+  p.meta[i] = par.vol | (species << 16);
   p.nsteps[i] = 0;
-  if (trace) p.uid[i] = n.uid;
+  p.aed[i] = 0.0;
+  p.avis[i] = 0.0;
+  p.alen[i] = 0.0;
+  if (trace) p.uid[i] = uid;
 }
 
-// Slot of output k of a lane in the next pool (neutral from the bottom,
+// Slot of output idx of a region in the next pool (neutral from the bottom,
 // charged from the top); -1 when it would leave the pool.
 __device__ __forceinline__ int64_t out_slot(int region, int idx, int64_t cap) {
   if (idx < 0 || (int64_t)idx >= cap) return -1;
   return region == 0 ? (int64_t)idx : cap - 1 - (int64_t)idx;
+}
+
+// Local deposit: into the track's own tallies and the workgroup histograms.
+template <class R>
+__device__ __forceinline__ void deposit(const DevParams *P, LdsTally &lt, TrackRegs<R> &t, double e, const double p[3],
+                                        int vol) {
+  if (e <= 0.0) return;
+  t.aed += e;
+  if (P->geo.visible[vol]) t.avis += e;
+  unsafeAtomicAdd(&lt.v[vol], e);
+  int iz = (int)floor(p[2] / P->zbin);
+  iz = iz < 0 ? 0 : (iz >= P->nz ? P->nz - 1 : iz);
+  unsafeAtomicAdd(&lt.z[iz], e);
+  const double r = sqrt(p[0] * p[0] + p[1] * p[1]);
+  int ir = (int)(r / P->rbin);
+  ir = ir < 0 ? 0 : (ir >= P->nr ? P->nr - 1 : ir);
+  unsafeAtomicAdd(&lt.r[ir], e);
+}
+
+// A track dies: its tallies go to its event (one set of atomics per track).
+template <class R>
+__device__ __forceinline__ void retire(const StepArgs &a, const TrackRegs<R> &t) {
+  DevEvent &E = a.ev[t.event];
+  if (t.aed != 0.0) unsafeAtomicAdd(&E.edep, t.aed);
+  if (t.avis != 0.0) unsafeAtomicAdd(&E.visible, t.avis);
+  if (t.alen != 0.0) unsafeAtomicAdd(&E.clen, t.alen);
+  atomicAdd(&E.nsteps, (unsigned long long)t.nsteps);
 }
 
 template <class R>
@@ -246,28 +279,46 @@ __device__ __forceinline__ void trace_rec(const StepArgs &a, const TrackRegs<R> 
   r[11] = edep;
 }
 
-// Positron at rest: two back-to-back 511 keV photons (2 direction draws,
-// then 2 substream draws per photon).
+// Write one new track into the next pool (region 0 neutral, 1 charged).
 template <class R>
-__device__ __forceinline__ void at_rest(Rng &rng, NewTrack<R> *out, int &n) {
-  const double cost = 2.0 * rng.uniform() - 1.0;
+__device__ __forceinline__ void emit_new(const StepArgs &a, const Pool<R> &nxt, int idx, const TrackRegs<R> &par,
+                                         int species, R e, R d0, R d1, R d2, uint64_t uid, LdsTally &lt, bool trace) {
+  const int64_t s = out_slot(species == 0 ? 0 : 1, idx, nxt.cap);
+  if (s >= 0)
+    store_new(a.P, nxt, s, par, species, e, d0, d1, d2, uid, trace);
+  else
+    atomicOr(&a.cnt->overflow, 1);
+  atomicAdd(&a.ev[par.event].nsec[species], 1ULL);
+  spectrum(lt, species, (double)e);
+}
+
+// Positron at rest: two back-to-back 511 keV photons, isotropic (2 direction
+// draws, then 2 substream draws per photon), at neutral slots idx, idx+1.
+template <class R>
+__device__ __forceinline__ void emit_at_rest(const StepArgs &a, const Pool<R> &nxt, int idx, TrackRegs<R> &t,
+                                             LdsTally &lt, bool trace) {
+  const double cost = 2.0 * t.rng.uniform() - 1.0;
   const double sint = sqrt((1.0 - cost) * (1.0 + cost));
-  const double phi = kTwoPi * rng.uniform();
+  const double phi = kTwoPi * t.rng.uniform();
   const R d0 = (R)(sint * cos(phi)), d1 = (R)(sint * sin(phi)), d2 = (R)cost;
-  out[n].species = 0;
-  out[n].e = (R)kMe;
-  out[n].d[0] = d0;
-  out[n].d[1] = d1;
-  out[n].d[2] = d2;
-  out[n].uid = child_substream(rng);
-  ++n;
-  out[n].species = 0;
-  out[n].e = (R)kMe;
-  out[n].d[0] = -d0;
-  out[n].d[1] = -d1;
-  out[n].d[2] = -d2;
-  out[n].uid = child_substream(rng);
-  ++n;
+  const uint64_t u1 = child_substream(t.rng);
+  emit_new(a, nxt, idx, t, 0, (R)kMe, d0, d1, d2, u1, lt, trace);
+  const uint64_t u2 = child_substream(t.rng);
+  emit_new(a, nxt, idx + 1, t, 0, (R)kMe, -d0, -d1, -d2, u2, lt, trace);
+}
+
+// PostStep GPIL of process p (ProxyEmProcess.hpp:122-151): resample the
+// number of interaction lengths when exhausted, propose nil / lambda.
+template <int p, class R>
+__device__ __forceinline__ void gpil(const DevParams *P, const Nodes<R> &N, const DevMaterial &m, int species, R e,
+                                     R &nil, R &xs, Rng &rng, double &bestd, int &sel) {
+  xs = macro_xs(P, N, m, species, p, e);
+  if (nil <= R(0)) nil = -m_log(rng.template u<R>());
+  const double sp = xs > R(0) ? (double)(nil / xs) : kHuge;
+  if (sp < bestd) {
+    bestd = sp;
+    sel = p;
+  }
 }
 
 // ------------------------------------------------------------------ step loop kernels
@@ -289,17 +340,21 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
       for (int k = 0; k < 3; ++k) {
         t.pos[k] = p.pos[k];
         t.dir[k] = (R)p.dir[k];
-        t.nil[k] = R(-1);
       }
+      t.nil0 = t.nil1 = t.nil2 = R(-1);
       t.nsteps = 0;
+      t.aed = t.avis = t.alen = 0.0;
       t.uid = p.substream;
       t.vol = locate(P->geo, t.pos, p.dir[0], p.dir[1], p.dir[2]);
       if (t.vol < 0) {
         unsafeAtomicAdd(&a.ev[t.event].leak, p.ekin);
-        atomicAdd(&a.job_live[a.ev[t.event].job], -1);
       } else {
-        substream_state(P, t.uid, t.rng.s);
+        t.rng.s[0] = (double)kRngPending;
+        t.rng.s[1] = (double)(uint32_t)t.uid;
+        t.rng.s[2] = (double)(uint32_t)(t.uid >> 32);
+        t.rng.s[3] = t.rng.s[4] = t.rng.s[5] = 0.0;
         region = t.species == 0 ? 0 : 1;
+        atomicAdd(&a.ev_live[t.event], 1);
       }
     }
     const int b0 = wave_reserve(region == 0 ? 1 : 0, &a.cnt->cur[0]);
@@ -314,8 +369,11 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
   }
 }
 
-template <class R>
-__global__ __launch_bounds__(kBlock) void k_transport(StepArgs a, PoolArgs<R> pa) {
+// One kernel per region: kCharged = false steps the photons of [0, nN),
+// kCharged = true the electrons and positrons of [cap - nC, cap); the species
+// class is a compile-time constant so each variant carries only its own code.
+template <class R, bool kCharged>
+__global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a, PoolArgs<R> pa) {
   __shared__ LdsTally lt;
   lds_clear(lt);
   const DevParams *P = a.P;
@@ -327,39 +385,37 @@ __global__ __launch_bounds__(kBlock) void k_transport(StepArgs a, PoolArgs<R> pa
   const int64_t capc = cur.cap;
   const int nN = (int)min((int64_t)max(a.cnt->cur[0], 0), capc);
   const int nC = (int)min((int64_t)max(a.cnt->cur[1], 0), capc - nN);
-  const int total = nN + nC;
+  const int total = kCharged ? nC : nN;
   const R mine = (R)P->tracking_cut;
   for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < total; base += gridDim.x * blockDim.x) {
     const int i = base + lane();
     const bool active = i < total;
     TrackRegs<R> t;
-    NewTrack<R> nt[2];
-    int nnew = 0;
     int q = -1;
     int slot = 0;
-    bool survive = false;
+    bool survive = false, annihilate = false;
     int live_delta = 0;
     if (active) {
-      slot = i < nN ? i : (int)(cur.cap - 1 - (i - nN));
+      slot = kCharged ? (int)(cur.cap - 1 - i) : i;
       load_track(cur, slot, t, trace);
+      if (!kCharged) t.species = 0;
+      if (t.rng.s[0] == (double)kRngPending) {
+        // first step of a new track: its state is substream uid of the seed
+        const uint64_t uid = (uint64_t)(uint32_t)t.rng.s[1] | ((uint64_t)(uint32_t)t.rng.s[2] << 32);
+        substream_state(P, uid, t.rng.s);
+      }
       const DevMaterial &m = P->mat[P->geo.slot[t.vol]];
-      const int np = nproc(t.species);
-      R xs[3] = {R(0), R(0), R(0)};
+      // species as a value the compiler can bound: 0 here, or 1 / 2 there
+      const int sp = kCharged ? (t.species == 2 ? 2 : 1) : 0;
+      const bool charged = kCharged;
+      R xs0 = R(0), xs1 = R(0), xs2 = R(0);
       double bestd = kHuge;
       int sel = -1;
-      // PostStep GPIL (ProxyEmProcess.hpp:122-151)
-      for (int p = 0; p < np; ++p) {
-        xs[p] = macro_xs(P, N, m, t.species, p, t.e);
-        if (t.nil[p] <= R(0)) t.nil[p] = -m_log(t.rng.template u<R>());
-        const double sp = xs[p] > R(0) ? (double)(t.nil[p] / xs[p]) : kHuge;
-        if (sp < bestd) {
-          bestd = sp;
-          sel = p;
-        }
-      }
-      const bool charged = t.species != 0;
+      gpil<0>(P, N, m, sp, t.e, t.nil0, xs0, t.rng, bestd, sel);
+      gpil<1>(P, N, m, sp, t.e, t.nil1, xs1, t.rng, bestd, sel);
+      if (sp != 1) gpil<2>(P, N, m, sp, t.e, t.nil2, xs2, t.rng, bestd, sel);
       if (charged) {
-        const double al = (double)along_limit(N, m, t.species == 2, t.e, mine);
+        const double al = (double)along_limit(N, m, sp == 2, t.e, mine);
         if (al < bestd) {
           bestd = al;
           sel = -1;
@@ -368,18 +424,18 @@ __global__ __launch_bounds__(kBlock) void k_transport(StepArgs a, PoolArgs<R> pa
       const double pre[3] = {t.pos[0], t.pos[1], t.pos[2]};
       const int prevol = t.vol;
       double step = 0.0;
-      const int status = propagate<R>(P, t.species, t.e, t.pos, t.dir, t.vol, bestd, step);
-      for (int p = 0; p < np; ++p)
-        if (xs[p] > R(0)) t.nil[p] -= (R)step * xs[p];
+      const int status = propagate<R>(P, sp, t.e, t.pos, t.dir, t.vol, bestd, step);
+      if (xs0 > R(0)) t.nil0 -= (R)step * xs0;
+      if (xs1 > R(0)) t.nil1 -= (R)step * xs1;
+      if (xs2 > R(0)) t.nil2 -= (R)step * xs2;
       t.nsteps++;
-      atomicAdd(&a.ev[t.event].nsteps, 1ULL);
       double edep = 0.0;
       if (charged) {
-        unsafeAtomicAdd(&a.ev[t.event].clen, step);
+        t.alen += step;
         R loss = R(0);
         t.e = along_doit(N, m, t.species == 2, t.e, (R)step, mine, loss);
         const double mid[3] = {0.5 * (pre[0] + t.pos[0]), 0.5 * (pre[1] + t.pos[1]), 0.5 * (pre[2] + t.pos[2])};
-        deposit(P, lt, a.ev, t.event, (double)loss, mid, prevol);
+        deposit(P, lt, t, (double)loss, mid, prevol);
         edep += (double)loss;
       }
       int tproc = status == kPhysics ? sel : (status == kBoundary ? -2 : (status == kEscaped ? -3 : -1));
@@ -387,40 +443,47 @@ __global__ __launch_bounds__(kBlock) void k_transport(StepArgs a, PoolArgs<R> pa
       if (status == kEscaped) {
         unsafeAtomicAdd(&a.ev[t.event].leak, (double)t.e);
         if (trace) trace_rec(a, t, tproc, edep);
+        retire(a, t);
         live_delta = -1;
       } else if (t.e <= R(0)) {
         if (trace) trace_rec(a, t, tproc, edep);
-        if (t.species == 2) {
-          atomicAdd(&a.ev[t.event].nannihil, 1ULL);
-          at_rest(t.rng, nt, nnew);
+        annihilate = t.species == 2;
+        if (!annihilate) {
+          retire(a, t);
+          live_delta = -1;
         }
-        live_delta = nnew - 1;
       } else if (status == kPhysics && sel >= 0) {
-        t.nil[sel] = R(-1);
+        t.set_nil(sel, R(-1));
         q = queue_of(t.species, sel);
         store_track(cur, slot, t, trace);
         if (trace) cur.tedep[slot] = edep;
       } else {
         if (trace) trace_rec(a, t, tproc, edep);
         if (t.e < mine) {
-          deposit(P, lt, a.ev, t.event, (double)t.e, t.pos, t.vol);
-          if (t.species == 2) {
-            atomicAdd(&a.ev[t.event].nannihil, 1ULL);
-            at_rest(t.rng, nt, nnew);
+          deposit(P, lt, t, (double)t.e, t.pos, t.vol);
+          annihilate = t.species == 2;
+          if (!annihilate) {
+            retire(a, t);
+            live_delta = -1;
           }
-          live_delta = nnew - 1;
         } else if (t.nsteps >= P->max_steps) {
-          deposit(P, lt, a.ev, t.event, (double)t.e, t.pos, t.vol);
+          deposit(P, lt, t, (double)t.e, t.pos, t.vol);
           atomicAdd(&a.cnt->killed, 1ULL);
+          retire(a, t);
           live_delta = -1;
         } else {
           survive = true;
         }
       }
     }
+    if (annihilate) {
+      // positron at rest: its two photons are made by the interaction kernel
+      q = kQueueAtRest;
+      store_track(cur, slot, t, trace);
+    }
     wave_add_u64(&a.cnt->steps, active);
-    // emission: survivors and annihilation photons into the next pool
-    const int outN = nnew + ((survive && t.species == 0) ? 1 : 0);
+    // emission: survivors into the next pool
+    const int outN = (survive && t.species == 0) ? 1 : 0;
     const int outC = (survive && t.species != 0) ? 1 : 0;
     const int bN = wave_reserve(outN, &a.cnt->nxt[0]);
     const int bC = wave_reserve(outC, &a.cnt->nxt[1]);
@@ -431,34 +494,49 @@ __global__ __launch_bounds__(kBlock) void k_transport(StepArgs a, PoolArgs<R> pa
       else
         atomicOr(&a.cnt->overflow, 1);
     }
-    for (int k = 0; k < nnew; ++k) {
-      const int64_t s = out_slot(0, bN + (survive && t.species == 0 ? 1 : 0) + k, nxt.cap);
-      if (s >= 0)
-        store_new(P, nxt, s, t.pos, t.vol, t.event, nt[k], trace);
-      else
-        atomicOr(&a.cnt->overflow, 1);
-      atomicAdd(&a.ev[t.event].nsec[0], 1ULL);
-      spectrum(lt, 0, (double)nt[k].e);
-    }
     queue_push(q, slot, a.cnt->q, a.queues, cur.cap, &a.cnt->overflow);
-    if (live_delta != 0) atomicAdd(&a.job_live[a.ev[t.event].job], live_delta);
+    if (live_delta != 0) atomicAdd(&a.ev_live[t.event], live_delta);
   }
   lds_flush(P, lt, a.rows);
 }
 
+// Fate of a secondary: pushed (>= tracking cut), or deposited (and, for a
+// positron, annihilated at rest).
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_interact(StepArgs a, PoolArgs<R> pa) {
+__device__ __forceinline__ int sec_fate(const DevParams *P, LdsTally &lt, TrackRegs<R> &t, const Sec<R> &s, R mine,
+                                        int &outN, int &outC, const StepArgs &a) {
+  if (s.e < mine) {
+    deposit(P, lt, t, (double)s.e, t.pos, t.vol);
+    if (s.species == 2) {
+      atomicAdd(&a.ev[t.event].nannihil, 1ULL);
+      outN += 2;
+      return 2;
+    }
+    return 0;
+  }
+  if (s.species == 0)
+    outN++;
+  else
+    outC++;
+  return 1;
+}
+
+template <class R, bool kCharged>
+__global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a, PoolArgs<R> pa) {
   __shared__ LdsTally lt;
   lds_clear(lt);
   const DevParams *P = a.P;
   const Pool<R> &cur = pa.cur;
   const Pool<R> &nxt = pa.nxt;
   const bool trace = P->trace_events > 0;
+  // photon queues 0-2 in one variant, e-/e+ queues 3-8 in the other
+  constexpr int kQ0 = kCharged ? 3 : 0;
   int qc[kNumQueues];
   int total = 0;
 #pragma unroll
   for (int k = 0; k < kNumQueues; ++k) {
-    qc[k] = (int)min((int64_t)max(a.cnt->q[k], 0), cur.cap);
+    const bool mine_q = kCharged ? k >= 3 : k < 3;
+    qc[k] = mine_q ? (int)min((int64_t)max(a.cnt->q[k], 0), cur.cap) : 0;
     total += qc[k];
   }
   const R mine = (R)P->tracking_cut;
@@ -466,94 +544,114 @@ __global__ __launch_bounds__(kBlock) void k_interact(StepArgs a, PoolArgs<R> pa)
     const int i = base + lane();
     const bool active = i < total;
     TrackRegs<R> t;
-    NewTrack<R> nt[3];
-    int nnew = 0;
-    bool survive = false;
+    Interaction<R> o;
+    o.nsec = 0;
+    int fate0 = 0, fate1 = 0;  // 0 deposited, 1 pushed, 2 annihilated at rest
+    bool survive = false, prim_rest = false;
+    int outN = 0, outC = 0;
     int live_delta = 0;
     if (active) {
-      int q = 0, j = i;
-      while (j >= qc[q]) {
-        j -= qc[q];
-        ++q;
-      }
+      int q = kQ0, j = i;
+#pragma unroll
+      for (int k = kQ0; k < kNumQueues - 1; ++k)
+        if (q == k && j >= qc[k]) {
+          j -= qc[k];
+          q = k + 1;
+        }
       const int slot = a.queues[(int64_t)q * cur.cap + j];
       load_track(cur, slot, t, trace);
-      const DevMaterial &m = P->mat[P->geo.slot[t.vol]];
-      Interaction<R> o;
-      sample_model<R>(P, m, queue_model(q), t.e, t.dir, t.rng, o);
-      if (q == 1 && o.nsec == 2) atomicAdd(&a.ev[t.event].npairs, 1ULL);
-      if (q == 7) atomicAdd(&a.ev[t.event].nannihil, 1ULL);
-      deposit(P, lt, a.ev, t.event, (double)o.edep, t.pos, t.vol);
-      t.e = o.ekin;
-      t.dir[0] = o.d[0];
-      t.dir[1] = o.d[1];
-      t.dir[2] = o.d[2];
-      if (trace) trace_rec(a, t, queue_proc(q), cur.tedep[slot] + (double)o.edep);
-      for (int k = 0; k < o.nsec; ++k) {
-        const Sec<R> &s = o.sec[k];
-        if (s.e < mine) {
-          deposit(P, lt, a.ev, t.event, (double)s.e, t.pos, t.vol);
-          if (s.species == 2) {
-            atomicAdd(&a.ev[t.event].nannihil, 1ULL);
-            at_rest(t.rng, nt, nnew);
-          }
-        } else {
-          nt[nnew].species = s.species;
-          nt[nnew].e = s.e;
-          nt[nnew].d[0] = s.d[0];
-          nt[nnew].d[1] = s.d[1];
-          nt[nnew].d[2] = s.d[2];
-          nt[nnew].uid = child_substream(t.rng);
-          ++nnew;
-        }
+      if (!kCharged) t.species = 0;
+      if (kCharged && q == kQueueAtRest) {
+        // positron stopped by the continuous loss or the tracking cut
+        atomicAdd(&a.ev[t.event].nannihil, 1ULL);
+        prim_rest = true;
+        outN = 2;
+        t.e = R(0);
+      } else {
+        const DevMaterial &m = P->mat[P->geo.slot[t.vol]];
+        if (kCharged)
+          sample_charged<R>(P, m, q, t.e, t.dir, t.rng, o);
+        else
+          sample_neutral<R>(P, m, q, t.e, t.dir, t.rng, o);
+        if (q == 1 && o.nsec == 2) atomicAdd(&a.ev[t.event].npairs, 1ULL);
+        if (q == 7) atomicAdd(&a.ev[t.event].nannihil, 1ULL);
+        deposit(P, lt, t, (double)o.edep, t.pos, t.vol);
+        t.e = o.ekin;
+        t.dir[0] = o.d[0];
+        t.dir[1] = o.d[1];
+        t.dir[2] = o.d[2];
+        if (trace) trace_rec(a, t, queue_proc(q), cur.tedep[slot] + (double)o.edep);
+        if (o.nsec > 0) fate0 = sec_fate(P, lt, t, o.sec[0], mine, outN, outC, a);
+        if (o.nsec > 1) fate1 = sec_fate(P, lt, t, o.sec[1], mine, outN, outC, a);
       }
       if (t.e > R(0)) {
         if (t.e < mine) {
-          deposit(P, lt, a.ev, t.event, (double)t.e, t.pos, t.vol);
+          deposit(P, lt, t, (double)t.e, t.pos, t.vol);
           if (t.species == 2) {
             atomicAdd(&a.ev[t.event].nannihil, 1ULL);
-            at_rest(t.rng, nt, nnew);
+            prim_rest = true;
+            outN += 2;
           }
         } else if (t.nsteps >= P->max_steps) {
-          deposit(P, lt, a.ev, t.event, (double)t.e, t.pos, t.vol);
+          deposit(P, lt, t, (double)t.e, t.pos, t.vol);
           atomicAdd(&a.cnt->killed, 1ULL);
         } else {
           survive = true;
+          if (t.species == 0)
+            outN++;
+          else
+            outC++;
         }
       }
-      live_delta = nnew + (survive ? 1 : 0) - 1;
+      if (!survive) retire(a, t);
+      live_delta = outN + outC - 1;
     }
     wave_add_u64(&a.cnt->interactions, active);
-    int outN = 0, outC = 0;
-    if (survive) (t.species == 0 ? outN : outC)++;
-    for (int k = 0; k < nnew; ++k) (nt[k].species == 0 ? outN : outC)++;
     const int bN = wave_reserve(outN, &a.cnt->nxt[0]);
     const int bC = wave_reserve(outC, &a.cnt->nxt[1]);
-    int kN = 0, kC = 0;
-    if (survive) {
-      const int64_t s = t.species == 0 ? out_slot(0, bN + kN++, nxt.cap) : out_slot(1, bC + kC++, nxt.cap);
-      if (s >= 0)
-        store_track(nxt, s, t, trace);
-      else
-        atomicOr(&a.cnt->overflow, 1);
-    }
-    for (int k = 0; k < nnew; ++k) {
-      const int sp = nt[k].species;
-      const int64_t s = sp == 0 ? out_slot(0, bN + kN++, nxt.cap) : out_slot(1, bC + kC++, nxt.cap);
-      if (s >= 0)
-        store_new(P, nxt, s, t.pos, t.vol, t.event, nt[k], trace);
-      else
-        atomicOr(&a.cnt->overflow, 1);
-      atomicAdd(&a.ev[t.event].nsec[sp], 1ULL);
-      spectrum(lt, sp, (double)nt[k].e);
+    if (active) {
+      // new tracks in the oracle's draw order: secondary 0, secondary 1, then
+      // the primary's annihilation at rest; the survivor last
+      int kN = bN, kC = bC;
+      if (o.nsec > 0) {
+        const Sec<R> &s = o.sec[0];
+        if (fate0 == 1) {
+          const uint64_t u = child_substream(t.rng);
+          emit_new(a, nxt, s.species == 0 ? kN++ : kC++, t, s.species, s.e, s.d[0], s.d[1], s.d[2], u, lt, trace);
+        } else if (fate0 == 2) {
+          emit_at_rest(a, nxt, kN, t, lt, trace);
+          kN += 2;
+        }
+      }
+      if (o.nsec > 1) {
+        const Sec<R> &s = o.sec[1];
+        if (fate1 == 1) {
+          const uint64_t u = child_substream(t.rng);
+          emit_new(a, nxt, s.species == 0 ? kN++ : kC++, t, s.species, s.e, s.d[0], s.d[1], s.d[2], u, lt, trace);
+        } else if (fate1 == 2) {
+          emit_at_rest(a, nxt, kN, t, lt, trace);
+          kN += 2;
+        }
+      }
+      if (prim_rest) {
+        emit_at_rest(a, nxt, kN, t, lt, trace);
+        kN += 2;
+      }
+      if (survive) {
+        const int64_t s = t.species == 0 ? out_slot(0, kN, nxt.cap) : out_slot(1, kC, nxt.cap);
+        if (s >= 0)
+          store_track(nxt, s, t, trace);
+        else
+          atomicOr(&a.cnt->overflow, 1);
+      }
     }
     {
-      const int made = nnew;
+      const int made = outN + outC - (survive ? 1 : 0);
       const int incl = wave_incl_scan(made);
       const int tot = __shfl(incl, 63, 64);
       if (tot > 0 && lane() == 63) atomicAdd(&a.cnt->created, (unsigned long long)tot);
     }
-    if (live_delta != 0) atomicAdd(&a.job_live[a.ev[t.event].job], live_delta);
+    if (live_delta != 0) atomicAdd(&a.ev_live[t.event], live_delta);
   }
   lds_flush(P, lt, a.rows);
 }
@@ -671,9 +769,11 @@ template <class R>
 hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, hipStream_t st, hipEvent_t *ev) {
   // ev (optional): 4 events recorded before/after the transport and interaction kernels
   if (ev) (void)hipEventRecord(ev[0], st);
-  hipLaunchKernelGGL(k_transport<R>, dim3(grid), dim3(kBlock), 0, st, a, pa);
+  hipLaunchKernelGGL((k_transport<R, false>), dim3(grid), dim3(kBlock), 0, st, a, pa);
+  hipLaunchKernelGGL((k_transport<R, true>), dim3(grid), dim3(kBlock), 0, st, a, pa);
   if (ev) (void)hipEventRecord(ev[1], st);
-  hipLaunchKernelGGL(k_interact<R>, dim3(grid), dim3(kBlock), 0, st, a, pa);
+  hipLaunchKernelGGL((k_interact<R, false>), dim3(grid), dim3(kBlock), 0, st, a, pa);
+  hipLaunchKernelGGL((k_interact<R, true>), dim3(grid), dim3(kBlock), 0, st, a, pa);
   if (ev) (void)hipEventRecord(ev[2], st);
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, a.cnt, pa.cur.cap);
   if (ev) (void)hipEventRecord(ev[3], st);

@@ -13,7 +13,8 @@ constexpr int kMaxElem = 3;       // elements per material
 constexpr int kMaxSlots = 2;      // materials per setup
 constexpr int kMaxZBins = 64;
 constexpr int kMaxRBins = 64;
-constexpr int kNumQueues = 8;     // (species, process) interaction queues
+constexpr int kNumQueues = 9;     // (species, process) interaction queues + positrons at rest
+constexpr int kQueueAtRest = 8;
 constexpr int kSpecBins = 64;     // secondary spectrum bins per species
 
 // One physics vector inside the node arrays (x, y, d2 at [off, off+n)).
@@ -95,6 +96,8 @@ struct Pool {
   int32_t *event;     // event slot
   int32_t *meta;      // volume | species << 16
   int32_t *nsteps;
+  double *aed, *avis, *alen;  // per-track tallies (deposit, visible deposit, charged length),
+                              // added to the event once, when the track dies
   uint64_t *uid;      // substream index (written only while tracing)
   double *tedep;      // step deposit carried to the interaction kernel (tracing only)
 };
@@ -120,8 +123,7 @@ struct DevEvent {
   double edep, visible, leak, clen;
   unsigned long long nsec[3];
   unsigned long long nsteps, npairs, nannihil;
-  int job;       // job slot owning the event
-  int pad;
+  int pad0, pad1;
 };
 
 }  // namespace gxt

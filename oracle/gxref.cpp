@@ -233,7 +233,7 @@ static double pvec_value(const PVec &p, double e) {
 }
 
 // InvRangeValue (ProxyPhysicsVector.cu:491-524): binary search bin location
-// for the non-log inverse-range vector (Q7: the reference calls the log-bin
+// for the non-log inverse-range vector (the reference calls the log-bin
 // Value() on it, ProxyIonization.hpp:100-104).
 static double pvec_inv_value(const PVec &p, double r) {
   const int N = (int)p.x.size();
@@ -384,7 +384,7 @@ static bool load_shells(const std::string &dir, Element &e) {
 
 static bool read_cuts(const std::string &path, int couple, double &g, double &em, double &ep) {
   // cut.dat is particle-major (gamma, e-, e+, proton) x couples, energies in
-  // keV (see the Q6 note at ProxyDataManager.cu:83-122 / .cuh:53-54).
+  // keV (ProxyDataManager.cu:83-122 / .cuh:53-54).
   std::ifstream f(path);
   if (!f) return false;
   std::string tag;
@@ -491,7 +491,7 @@ static double sandia_macro(const Material &m, double ekin) {
 
 // Heitler two-photon annihilation cross-section per electron (mm2)
 // (G4eeToTwoGammaModel::ComputeCrossSectionPerElectron; the reference has no
-// positron process, SURVEY row A12).
+// positron process).
 static double heitler_xs(double ekin) {
   const double tau = ekin / kMe;
   const double gam = tau + 1.0;
@@ -561,8 +561,8 @@ static int locate(const Geometry &g, const double p[3], const double d[3]) {
   return lo;
 }
 
-// Straight-line distance to the boundary of volume v (A16 replaced by an
-// analytic slab/box navigator).  face: 0 = -z, 1 = +z, 2 = x side, 3 = y side.
+// Straight-line distance to the boundary of volume v (an analytic
+// slab/box navigator in place of the reference's VecGeom navigation).  face: 0 = -z, 1 = +z, 2 = x side, 3 = y side.
 static double dist_out(const Geometry &g, int v, const double p[3], const double d[3], int &face) {
   double s = kHuge;
   face = -1;
@@ -623,7 +623,7 @@ static inline void unit3(double *a) {
 }
 
 // ConstFieldHelixStepper::DoStep (magneticfield/ConstFieldHelixStepper.hpp:164-239)
-// in mm/MeV/tesla (Q8), followed by the direction normalisation of
+// in mm/MeV/tesla, followed by the direction normalisation of
 // FieldPropagationHandler::PropagateInVolume (FieldPropagationHandler.cpp:323-330).
 static void helix_step(const double B[3], double bmag, double charge, double mom, double step,
                        double pos[3], double dir[3]) {
@@ -669,8 +669,8 @@ struct Physics {
 // Macroscopic cross-section (1/mm) of discrete process p
 // (ProxyEmProcess::GetLambda via ProxyCompton.hpp:72-76, ProxyConversion.hpp:73-77,
 // ProxyPhotoElectric.hpp:73-77, ProxyBremsstrahlung.hpp:72-76, ProxyIonization.hpp:74-78)
-// with the track's couple (Q1), Compton above 1 MeV from LambdaPrim/E (Q4),
-// photoelectric LambdaPrim/E (Q5) and Sandia below the table.
+// with the track's couple, Compton above 1 MeV from LambdaPrim/E,
+// photoelectric LambdaPrim/E and Sandia below the table.
 static double macro_xs(const Material &m, int species, int p, double e) {
   if (species == 0) {
     if (p == 0) {
@@ -691,7 +691,7 @@ static double macro_xs(const Material &m, int species, int p, double e) {
 
 static inline int nproc(int species) { return species == 1 ? 2 : 3; }
 
-// Element choice (Q2: the reference hard-codes Z = 10): one draw, only for
+// Element choice (the reference hard-codes Z = 10): one draw, only for
 // compounds.  weights: 0 = n Z(Z+1) (pair, brems), 1 = partial Sandia (photo).
 static int select_element(const Material &m, int kind, double e, uint32_t s[6]) {
   const int ne = (int)m.el.size();
@@ -780,8 +780,8 @@ static inline double coulomb_factor(double Z) {
 }
 
 // Bethe-Heitler pair production (proxy/ProxyBetheHeitler.hpp:64-182) with
-// the element chosen from the material (Q2) and the Tsai angle used as an
-// angle, cos/sin of u*m/E (Q19: the reference uses u*m/E as the cosine).
+// the element chosen from the material and the Tsai angle used as an
+// angle, cos/sin of u*m/E (the reference uses u*m/E as the cosine).
 static void sample_pair(const Material &m, double e0, const double d0[3], uint32_t s[6], Interaction &o) {
   o.nsec = 0;
   o.edep = 0;
@@ -854,9 +854,9 @@ static void sample_pair(const Material &m, double e0, const double d0[3], uint32
 }
 
 // Sauter-Gavrila photo-electron (proxy/ProxySauterGavrila.hpp:66-130), with the
-// element chosen by partial Sandia cross-section (Q2), the first shell whose
-// binding energy is below E (Q17: the reference's shell loop runs past the
-// ionisable shells) and the binding energy deposited locally (Q12).
+// element chosen by partial Sandia cross-section, the first shell whose
+// binding energy is below E (the reference's shell loop runs past the
+// ionisable shells) and the binding energy deposited locally.
 static void sample_photo(const Material &m, double e0, const double d0[3], uint32_t s[6], Interaction &o) {
   o.nsec = 0;
   o.edep = 0;
@@ -906,11 +906,11 @@ static void sample_photo(const Material &m, double e0, const double d0[3], uint3
 }
 
 // Seltzer-Berger bremsstrahlung (proxy/ProxySeltzerBerger.hpp:204-286) with
-// the couple's gamma production cut (Q16), the element chosen from the
-// material (Q2), the element's own grid (Q3: the reference reads [Z] for data
-// stored at [Z-1]), the Migdal dielectric factor of the material (Q15: the
+// the couple's gamma production cut, the element chosen from the
+// material, the element's own grid (the reference reads [Z] for data
+// stored at [Z-1]), the Migdal dielectric factor of the material (the
 // reference leaves densityFactor 0), and the DipBust angle driven by the
-// electron's kinetic energy (Q20, ProxySeltzerBerger.hpp:288-315).
+// electron's kinetic energy (ProxySeltzerBerger.hpp:288-315).
 static void sample_brems(const Material &m, double t0, const double d0[3], uint32_t s[6], Interaction &o) {
   o.nsec = 0;
   o.edep = 0;
@@ -974,8 +974,8 @@ static void sample_brems(const Material &m, double t0, const double d0[3], uint3
 
 // Moller (e-) / Bhabha (e+) delta rays above the couple's electron
 // production cut (proxy/ProxyMollerScattering.hpp:92-159 for Moller with the
-// cut from the couple, Q6; the Bhabha branch from the published
-// G4MollerBhabhaModel cross-section, absent in the reference, row A12).
+// cut from the couple; the Bhabha branch from the published
+// G4MollerBhabhaModel cross-section, absent in the reference).
 static void sample_ioni(const Material &m, bool positron, double t0, const double d0[3], uint32_t s[6],
                         Interaction &o) {
   o.nsec = 0;
@@ -1044,7 +1044,7 @@ static void sample_ioni(const Material &m, bool positron, double t0, const doubl
 }
 
 // e+ annihilation in flight into two photons (Heitler; published
-// G4eeToTwoGammaModel algorithm, absent in the reference, row A12).
+// G4eeToTwoGammaModel algorithm, absent in the reference).
 static void sample_annihil(const Material &, double t0, const double d0[3], uint32_t s[6], Interaction &o) {
   o.nsec = 0;
   o.edep = 0;
@@ -1111,8 +1111,8 @@ static inline int model_of(int species, int p) {
 }
 
 // Continuous-loss step limit (ProxyIonization.hpp:108-128) with the
-// production-cut length for finR (Q6) and Geant4's (2 - finR/range) form
-// (Q18: the reference writes 2*finR/range, which exceeds the range near finR).
+// production-cut length for finR and Geant4's (2 - finR/range) form
+// (the reference writes 2*finR/range, which exceeds the range near finR).
 static double along_limit(const Material &m, bool pos, double e, double mine) {
   double range = pvec_value(m.tab[pos ? GXT_TAB_RANGE_EP : GXT_TAB_RANGE_EM], e);
   if (e < mine) range *= e / mine;
@@ -1121,7 +1121,7 @@ static double along_limit(const Material &m, bool pos, double e, double mine) {
 }
 
 // Continuous energy loss (ProxyIonization.hpp:130-168) with the tracking cut
-// as minimum energy and the binary-search inverse range (Q7).
+// as minimum energy and the binary-search inverse range.
 static double along_doit(const Material &m, bool pos, double e, double step, double mine, double &eloss) {
   const double range = pvec_value(m.tab[pos ? GXT_TAB_RANGE_EP : GXT_TAB_RANGE_EM], e);
   if (step >= range || e <= mine) {
