@@ -21,5 +21,7 @@ int load_material(const std::string &data_dir, int which, double lar_density, Ho
                   std::string &err);
 void build_geometry(const gxt_config &c, DevGeometry &g);
 void build_jump_matrices(uint32_t out[2][64][9]);
+// Byte-wise jump tables: out[((c * 8 + k) * 256 + d) * 9 + i] = A_c^(d 2^(76+8k)) mod m_c.
+void build_jump_tables(const uint32_t jump[2][64][9], std::vector<uint32_t> &out);
 
 }  // namespace gxt

@@ -87,32 +87,57 @@ struct Rng {
   template <class R> __device__ __forceinline__ R u() { return (R)uniform(); }
 };
 
-// (M v) mod m for 32-bit residues: every product < 2^64.
-__device__ __forceinline__ void matvec_mod(const uint32_t *M, uint64_t v[3], uint64_t m) {
+// a * b mod m for a, b < m < 2^32, exactly: the 64-bit product wraps
+// correctly, the FP64 quotient estimate is within one of the true quotient,
+// and one correction each way restores the residue (no 64-bit division).
+__device__ __forceinline__ uint64_t mulmod32(uint64_t a, uint64_t b, uint64_t m, double invm) {
+  const uint64_t p = a * b;
+  const double q = floor((double)a * (double)b * invm);
+  int64_t r = (int64_t)(p - (uint64_t)q * m);
+  if (r < 0) r += (int64_t)m;
+  if (r >= (int64_t)m) r -= (int64_t)m;
+  return (uint64_t)r;
+}
+
+// (M v) mod m for 32-bit residues.
+__device__ __forceinline__ void matvec_mod(const uint32_t *M, uint64_t v[3], uint64_t m, double invm) {
   uint64_t r[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    uint64_t acc = 0;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) acc += ((uint64_t)M[i * 3 + k] * v[k]) % m;
-    r[i] = acc % m;
+    uint64_t acc = mulmod32(M[i * 3], v[0], m, invm) + mulmod32(M[i * 3 + 1], v[1], m, invm) +
+                   mulmod32(M[i * 3 + 2], v[2], m, invm);
+    if (acc >= m) acc -= m;
+    if (acc >= m) acc -= m;
+    r[i] = acc;
   }
   v[0] = r[0];
   v[1] = r[1];
   v[2] = r[2];
 }
 
-// Substream idx of the stream seeded with seed: (A^(2^76))^idx seed, by
-// binary powering over the jump matrices A^(2^(76+j)).
-__device__ __forceinline__ void substream_state(const DevParams *P, uint64_t idx, double out[6]) {
-  uint64_t v1[3] = {P->seed[0], P->seed[1], P->seed[2]};
-  uint64_t v2[3] = {P->seed[3], P->seed[4], P->seed[5]};
-  for (int j = 0; j < 64; ++j) {
-    if ((idx >> j) & 1ULL) {
-      matvec_mod(P->jump[0][j], v1, 4294967087ULL);
-      matvec_mod(P->jump[1][j], v2, 4294944443ULL);
-    }
+// Substream idx of the stream seeded with s: (A^(2^76))^idx s, as the
+// product over the 8 bytes d_k of idx of A^(d_k 2^(76+8k)) (powers of one
+// matrix commute).  T[((c * 8 + k) * 256 + d) * 9 + i] holds A_c^(d 2^(76+8k))
+// mod m_c, the identity for d = 0.
+__device__ __forceinline__ void jump_substream(const uint32_t *T, const uint32_t s[6], uint64_t idx,
+                                               uint64_t v1[3], uint64_t v2[3]) {
+  v1[0] = s[0];
+  v1[1] = s[1];
+  v1[2] = s[2];
+  v2[0] = s[3];
+  v2[1] = s[4];
+  v2[2] = s[5];
+#pragma unroll 1
+  for (int k = 0; k < 8; ++k) {
+    const int d = (int)((idx >> (8 * k)) & 255u);
+    matvec_mod(T + ((0 * 8 + k) * 256 + d) * 9, v1, 4294967087ULL, 1.0 / 4294967087.0);
+    matvec_mod(T + ((1 * 8 + k) * 256 + d) * 9, v2, 4294944443ULL, 1.0 / 4294944443.0);
   }
+}
+
+__device__ __forceinline__ void substream_state(const DevParams *P, uint64_t idx, double out[6]) {
+  uint64_t v1[3], v2[3];
+  jump_substream(P->jump8, P->seed, idx, v1, v2);
   out[0] = (double)v1[0];
   out[1] = (double)v1[1];
   out[2] = (double)v1[2];

@@ -55,8 +55,10 @@ struct DeviceData {
   DevParams *P = nullptr;
   double *xd = nullptr, *yd = nullptr, *d2d = nullptr, *sbd = nullptr, *sandia = nullptr, *shells = nullptr;
   float *xf = nullptr, *yf = nullptr, *d2f = nullptr, *sbf = nullptr;
+  uint32_t *jump8 = nullptr;
   void release() {
     (void)hipFree(P);
+    (void)hipFree(jump8);
     (void)hipFree(xd);
     (void)hipFree(yd);
     (void)hipFree(d2d);
@@ -94,6 +96,10 @@ int upload(HostData &hd, DeviceData &dd) {
   HIP_TRY(put(&dd.sandia, hd.sandia));
   HIP_TRY(put(&dd.shells, hd.shells));
   DevParams &p = hd.params;
+  std::vector<uint32_t> j8;
+  build_jump_tables(p.jump, j8);
+  HIP_TRY(put(&dd.jump8, j8));
+  p.jump8 = dd.jump8;
   p.xd = dd.xd;
   p.yd = dd.yd;
   p.d2d = dd.d2d;
@@ -552,6 +558,11 @@ int rng_params(const DevParams **out) {
     DevParams p;
     std::memset(&p, 0, sizeof(p));
     build_jump_matrices(p.jump);
+    std::vector<uint32_t> j8;
+    build_jump_tables(p.jump, j8);
+    HIP_TRY(dalloc(&g_rng_dd.jump8, j8.size()));
+    HIP_TRY(hipMemcpy(g_rng_dd.jump8, j8.data(), j8.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    p.jump8 = g_rng_dd.jump8;
     HIP_TRY(dalloc(&g_rng_dd.P, 1));
     HIP_TRY(hipMemcpy(g_rng_dd.P, &p, sizeof(p), hipMemcpyHostToDevice));
   }

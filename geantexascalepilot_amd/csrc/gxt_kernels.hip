@@ -678,14 +678,8 @@ __global__ void k_advance(Counters *c, int64_t cap) {
 __global__ void k_rng_init(const DevParams *P, const uint32_t *seed, const uint64_t *idx, int n, uint32_t *out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint64_t v1[3] = {seed[0], seed[1], seed[2]};
-  uint64_t v2[3] = {seed[3], seed[4], seed[5]};
-  const uint64_t k = idx[i];
-  for (int j = 0; j < 64; ++j)
-    if ((k >> j) & 1ULL) {
-      matvec_mod(P->jump[0][j], v1, 4294967087ULL);
-      matvec_mod(P->jump[1][j], v2, 4294944443ULL);
-    }
+  uint64_t v1[3], v2[3];
+  jump_substream(P->jump8, seed, idx[i], v1, v2);
   out[6 * i + 0] = (uint32_t)v1[0];
   out[6 * i + 1] = (uint32_t)v1[1];
   out[6 * i + 2] = (uint32_t)v1[2];

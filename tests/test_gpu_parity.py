@@ -5,7 +5,6 @@ sequences, secondary counts) must be bit-exact; FP64 kinematics within 1e-10
 relative (device libm vs glibc may differ by an ulp), FP32 precision-form
 results within the float tolerances stated per test.
 """
-import os
 
 import numpy as np
 import pytest
@@ -222,9 +221,6 @@ def test_trace_first_50_steps(gpu_lib):
     names = ["ekin", "x", "y", "z", "dx", "dy", "dz", "edep"]
     bad = ~np.isclose(g[:, 4:], c[:, 4:], rtol=1e-10, atol=1e-10)
     if bad.any():
-        os.makedirs("gpurun_out", exist_ok=True)
-        rows = np.nonzero(bad.any(axis=1))[0]
-        np.savez("gpurun_out/trace_mismatch.npz", g=g[rows], c=c[rows], rows=rows)
         for j, nm in enumerate(names):
             if bad[:, j].any():
                 d = np.abs(g[:, 4 + j] - c[:, 4 + j])[bad[:, j]]
