@@ -33,9 +33,9 @@ def log(msg):
     sys.stderr.flush()
 
 
-def run_batch(g, abi, precision, n_events, steps, warmup, seed=777):
+def run_batch(g, abi, precision, n_events, steps, warmup, seed=777, pool=2_000_000):
     cfg = abi.config_pbwo4_block(max_events=n_events, precision=precision, profile_kernels=1,
-                                 pool_capacity=2_000_000, seed=seed)
+                                 pool_capacity=pool, seed=seed)
     prims = g.make_primaries(seed, abi.ELECTRON, 10000.0, 10000.0, 0.0, n_events)
     import torch
     with g.Transport(cfg) as t:
@@ -57,7 +57,8 @@ def run_batch(g, abi, precision, n_events, steps, warmup, seed=777):
         ks = t.kernel_stats()
         st = t.stats()
         edep = t.event_tallies(job - 1, n_events)
-    log("timed %d batches in %.3f s (precision %d)" % (steps, t1 - t0, precision))
+    log("timed %d batches in %.3f s (precision %d): %d iterations, peak %d live tracks"
+        % (steps, t1 - t0, precision, st.iterations, st.max_live))
     return dict(seconds=t1 - t0, ks=ks, stats=st, edep=edep)
 
 
@@ -151,6 +152,7 @@ def main():
     ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp32", action="store_true")
+    ap.add_argument("--pool", type=int, default=2_000_000, help="live-track pool capacity")
     a = ap.parse_args()
     if a.gpus != 1:
         sys.stderr.write("bench.py: this build runs the step loop on one GPU; multi-GPU runs are not supported "
@@ -163,37 +165,53 @@ def main():
     import geantexascalepilot_amd.gxt as g
 
     n = a.events
-    r64 = run_batch(g, abi, abi.FP64, n, a.steps, a.warmup)
+    r64 = run_batch(g, abi, abi.FP64, n, a.steps, a.warmup, pool=a.pool)
     value = a.steps * n / r64["seconds"]
     ks = r64["ks"]
     steps_per_s = ks.track_steps / r64["seconds"]
-    rec = ks.record_bytes
-    t_ms, i_ms = ks.transport_ms, ks.interact_ms
-    if t_ms >= i_ms:
-        name, ms = "k_transport", t_ms
-        nbytes = ks.track_steps * 2 * rec
-        per_unit = "2 x %d B per track-step (one read and one write of the track record)" % rec
+    # Algorithmic bytes per track-step: one read and one write of a 148 B
+    # track record (FP64 form; 108 B in the FP32 form): position 24, direction
+    # 24, energy 8, 3 interaction lengths 24, 3 inverse mean free paths 24,
+    # safety 8, RNG 24, couple, species/status, job 4 each.
+    rec = 148
+    kt = {"k_transport<f64,photon>": (ks.transport_neutral_ms, ks.neutral_steps),
+          "k_transport<f64,e-/e+>": (ks.transport_charged_ms, ks.charged_steps),
+          "k_interact<f64,photon>": (ks.interact_neutral_ms, None),
+          "k_interact<f64,e-/e+>": (ks.interact_charged_ms, None)}
+    name = max(kt, key=lambda k: kt[k][0])
+    ms, units = kt[name]
+    if units is not None:
+        nbytes = units * 2 * rec
+        per_unit = "2 x %d B per track-step (one read and one write of the track record), %d %s steps" % (
+            rec, units, "photon" if "photon" in name else "e-/e+")
     else:
-        name, ms = "k_interact", i_ms
-        nbytes = (2 * ks.interactions + ks.created) * rec
-        per_unit = "%d B per sampled track read + written, %d B per created track" % (2 * rec, rec)
-    achieved = nbytes / (ms * 1e-3) / 1e9
+        units = ks.interactions
+        nbytes = units * 2 * rec
+        per_unit = "2 x %d B per sampled track, %d sampled tracks (both interaction kernels)" % (rec, units)
+    launches = max(ks.launches, 1)
+    avg_us = 1e3 * ms / launches
+    achieved = (nbytes / launches) / (avg_us * 1e-6) / 1e9
     roofline = dict(bound="hbm", achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s", frac=achieved / PEAK_HBM_GBS,
-                    traffic=None, kernel=name, launches=int(ks.launches), avg_launch_us=1e3 * ms / max(ks.launches, 1),
-                    algorithmic_bytes=int(nbytes), per_unit=per_unit,
-                    kernel_ms={"transport": t_ms, "interact": i_ms, "advance": ks.advance_ms})
+                    traffic=None, kernel=name, launches=int(ks.launches), avg_launch_us=avg_us,
+                    bytes_per_launch=nbytes / launches, per_unit=per_unit, record_bytes_in_pool=int(ks.record_bytes),
+                    kernel_ms={k: v[0] for k, v in kt.items()} | {"k_derive": ks.derive_ms,
+                                                                  "k_advance": ks.advance_ms})
     e64 = r64["edep"]["edep"]
     out = dict(metric="events/s and track-steps/s, FP32 vs FP64 on 1 MI355X; p99 job latency", value=value,
                unit="events/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
                ms_per_step=1e3 * r64["seconds"] / a.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
                dtype="f64", data="synthetic",
                config=dict(workload="10 GeV e- x %d into 30x30x30 cm PbWO4 (8.28 g/cm3), no field, FP64, "
-                                    "pool 2e6 tracks, seed 777" % n, primaries=n, energy_gev=10.0,
-                           material="PbWO4", pool_capacity=2_000_000),
+                                    "pool %.0e tracks, seed 777" % (n, a.pool), primaries=n, energy_gev=10.0,
+                           material="PbWO4", pool_capacity=a.pool),
                track_steps_per_s=steps_per_s, track_steps_per_event=ks.track_steps / (a.steps * n),
+               counts=dict(track_steps=int(ks.track_steps), photon_steps=int(ks.neutral_steps),
+                           charged_steps=int(ks.charged_steps), interactions=int(ks.interactions),
+                           created=int(ks.created), iterations=int(r64["stats"].iterations),
+                           peak_live=int(r64["stats"].max_live)),
                mean_edep_mev=float(e64.mean()), roofline=roofline)
     if not a.skip_fp32:
-        r32 = run_batch(g, abi, abi.FP32, n, a.steps, a.warmup)
+        r32 = run_batch(g, abi, abi.FP32, n, a.steps, a.warmup, pool=a.pool)
         e32 = r32["edep"]["edep"]
         out["fp32"] = dict(value=a.steps * n / r32["seconds"], unit="events/s",
                            track_steps_per_s=r32["ks"].track_steps / r32["seconds"],

@@ -58,8 +58,11 @@ class Stats(C.Structure):
 
 
 class KernelStats(C.Structure):
-    _fields_ = [("transport_ms", C.c_double), ("interact_ms", C.c_double), ("advance_ms", C.c_double),
-                ("launches", C.c_int64), ("track_steps", C.c_int64), ("interactions", C.c_int64),
+    _fields_ = [("transport_neutral_ms", C.c_double), ("transport_charged_ms", C.c_double),
+                ("interact_neutral_ms", C.c_double), ("interact_charged_ms", C.c_double),
+                ("derive_ms", C.c_double),
+                ("advance_ms", C.c_double), ("launches", C.c_int64), ("track_steps", C.c_int64),
+                ("neutral_steps", C.c_int64), ("charged_steps", C.c_int64), ("interactions", C.c_int64),
                 ("created", C.c_int64), ("record_bytes", C.c_int32), ("pad", C.c_int32)]
 
 

@@ -191,6 +191,7 @@ struct gxt_ctx {
   int parity = 0;
   int grid = 1024;
   int32_t *queues = nullptr;
+  int32_t *newlist = nullptr;
   Counters *cnt = nullptr;
   Counters *hcnt = nullptr;  // pinned
   DevEvent *ev = nullptr;
@@ -215,7 +216,7 @@ struct gxt_ctx {
   int64_t syncs_observed = 0;
   // kernel timing (profile_kernels)
   bool prof = false;
-  std::vector<std::array<hipEvent_t, 4>> ev_free, ev_pending;
+  std::vector<std::array<hipEvent_t, kStepEvents>> ev_free, ev_pending;
   gxt_kernel_stats kst{};
   // stats
   int64_t iterations = 0;
@@ -235,6 +236,7 @@ struct gxt_ctx {
     a.ev = ev;
     a.ev_live = ev_live;
     a.queues = queues;
+    a.newlist = newlist;
     a.rows.bins = rows;
     a.rows.spec = spec;
     a.rows.stride = row_stride;
@@ -354,7 +356,7 @@ int iterate(gxt_ctx *c, int k) {
     hipEvent_t *evs = nullptr;
     if (c->prof) {
       if (c->ev_free.empty()) {
-        std::array<hipEvent_t, 4> q;
+        std::array<hipEvent_t, kStepEvents> q;
         for (auto &x : q) HIP_TRY(hipEventCreate(&x));
         c->ev_free.push_back(q);
       }
@@ -383,13 +385,14 @@ int collect(gxt_ctx *c, bool &idle) {
                          c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (auto &q : c->ev_pending) {
-    float t0 = 0, t1 = 0, t2 = 0;
-    HIP_TRY(hipEventElapsedTime(&t0, q[0], q[1]));
-    HIP_TRY(hipEventElapsedTime(&t1, q[1], q[2]));
-    HIP_TRY(hipEventElapsedTime(&t2, q[2], q[3]));
-    c->kst.transport_ms += t0;
-    c->kst.interact_ms += t1;
-    c->kst.advance_ms += t2;
+    float t[kStepEvents - 1] = {};
+    for (int k = 0; k + 1 < kStepEvents; ++k) HIP_TRY(hipEventElapsedTime(&t[k], q[k], q[k + 1]));
+    c->kst.transport_neutral_ms += t[0];
+    c->kst.transport_charged_ms += t[1];
+    c->kst.interact_neutral_ms += t[2];
+    c->kst.interact_charged_ms += t[3];
+    c->kst.derive_ms += t[4];
+    c->kst.advance_ms += t[5];
     c->kst.launches++;
     c->ev_free.push_back(q);
   }
@@ -519,6 +522,7 @@ void destroy_ctx(gxt_ctx *c) {
     for (auto &x : q) (void)hipEventDestroy(x);
   c->dd.release();
   (void)hipFree(c->queues);
+  (void)hipFree(c->newlist);
   (void)hipFree(c->cnt);
   (void)hipHostFree(c->hcnt);
   (void)hipFree(c->ev);
@@ -657,6 +661,7 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
     if (rc) return rc;
   }
   HIP_TRY(dalloc(&c->queues, (size_t)kNumQueues * c->cap));
+  HIP_TRY(dalloc(&c->newlist, (size_t)c->cap));
   HIP_TRY(dalloc(&c->cnt, 1));
   HIP_TRY(hipMemset(c->cnt, 0, sizeof(Counters)));
   HIP_TRY(hipHostMalloc((void **)&c->hcnt, sizeof(Counters), hipHostMallocDefault));
@@ -833,6 +838,8 @@ int gxt_get_kernel_stats(gxt_ctx *c, gxt_kernel_stats *o) {
   HIP_TRY(hipMemcpy(&h, c->cnt, sizeof(h), hipMemcpyDeviceToHost));
   *o = c->kst;
   o->track_steps = (int64_t)h.steps;
+  o->charged_steps = (int64_t)h.steps_charged;
+  o->neutral_steps = (int64_t)(h.steps - h.steps_charged);
   o->interactions = (int64_t)h.interactions;
   o->created = (int64_t)h.created;
   // x, y, z (FP64) + dir, E, 3 interaction lengths (R) + 6 RNG words + event, meta, nsteps
@@ -851,6 +858,7 @@ int gxt_reset_tallies(gxt_ctx *c) {
   Counters h;
   HIP_TRY(hipMemcpy(&h, c->cnt, sizeof(h), hipMemcpyDeviceToHost));
   h.steps = 0;
+  h.steps_charged = 0;
   h.killed = 0;
   h.trace_n = 0;
   h.interactions = 0;

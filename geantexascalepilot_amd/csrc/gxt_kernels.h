@@ -21,6 +21,7 @@ struct StepArgs {
   DevEvent *ev;
   int *ev_live;      // live tracks per event slot
   int32_t *queues;  // kNumQueues x cap
+  int32_t *newlist; // next-pool slots of the tracks created this iteration (cap)
   TallyRows rows;
   double *trace;
   int64_t trace_cap;
@@ -46,6 +47,11 @@ struct SampleArgs {
   double *sec_dir;
 };
 
+// One step-loop iteration.  With ev (kStepEvents events) every kernel is
+// bracketed by events: photon transport, e-/e+ transport, photon
+// interactions, e-/e+ interactions, RNG derivation of the new tracks,
+// counter advance.
+constexpr int kStepEvents = 7;
 template <class R>
 hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, hipStream_t st, hipEvent_t *ev = nullptr);
 template <class R>

@@ -143,11 +143,6 @@ struct TrackRegs {
   }
 };
 
-// First RNG word of a track whose state is still to be derived from its
-// substream index (words 1, 2 hold the index); never a valid MRG32k3a word,
-// which is below m1 = 4294967087.
-constexpr uint32_t kRngPending = 0xffffffffu;
-
 template <class R>
 __device__ __forceinline__ void load_track(const Pool<R> &p, int64_t i, TrackRegs<R> &t, bool trace) {
   t.pos[0] = p.x[i];
@@ -195,10 +190,13 @@ __device__ __forceinline__ void store_track(const Pool<R> &p, int64_t i, const T
 }
 
 // A track created by an interaction or an annihilation at rest, at the
-// position, volume and event of its parent; its RNG state is substream uid.
+// position, volume and event of its parent.  Its RNG state is substream uid;
+// RNG words 1 and 2 hold uid until k_derive replaces them with that state.
 template <class R>
 __device__ inline void store_new(const DevParams *P, const Pool<R> &p, int64_t i, const TrackRegs<R> &par,
                                  int species, R e, R d0, R d1, R d2, uint64_t uid, bool trace) {
+  p.rng[p.cap + i] = (uint32_t)uid;
+  p.rng[2 * p.cap + i] = (uint32_t)(uid >> 32);
   p.x[i] = par.pos[0];
   p.y[i] = par.pos[1];
   p.z[i] = par.pos[2];
@@ -209,10 +207,6 @@ __device__ inline void store_new(const DevParams *P, const Pool<R> &p, int64_t i
   p.nil0[i] = R(-1);
   p.nil1[i] = R(-1);
   p.nil2[i] = R(-1);
-  // the RNG state is derived from uid by the transport kernel at the first step
-  p.rng[i] = kRngPending;
-  p.rng[p.cap + i] = (uint32_t)uid;
-  p.rng[2 * p.cap + i] = (uint32_t)(uid >> 32);
   p.event[i] = par.event;
   p.meta[i] = par.vol | (species << 16);
   p.nsteps[i] = 0;
@@ -279,15 +273,20 @@ __device__ __forceinline__ void trace_rec(const StepArgs &a, const TrackRegs<R> 
   r[11] = edep;
 }
 
-// Write one new track into the next pool (region 0 neutral, 1 charged).
+// Write one new track into the next pool (region 0 neutral, 1 charged) and
+// its slot into entry nk of the new-track list.
 template <class R>
-__device__ __forceinline__ void emit_new(const StepArgs &a, const Pool<R> &nxt, int idx, const TrackRegs<R> &par,
-                                         int species, R e, R d0, R d1, R d2, uint64_t uid, LdsTally &lt, bool trace) {
+__device__ __forceinline__ void emit_new(const StepArgs &a, const Pool<R> &nxt, int idx, int &nk,
+                                         const TrackRegs<R> &par, int species, R e, R d0, R d1, R d2, uint64_t uid,
+                                         LdsTally &lt, bool trace) {
   const int64_t s = out_slot(species == 0 ? 0 : 1, idx, nxt.cap);
-  if (s >= 0)
+  if (s >= 0 && nk >= 0 && (int64_t)nk < nxt.cap) {
     store_new(a.P, nxt, s, par, species, e, d0, d1, d2, uid, trace);
-  else
+    a.newlist[nk] = (int32_t)s;
+  } else {
     atomicOr(&a.cnt->overflow, 1);
+  }
+  ++nk;
   atomicAdd(&a.ev[par.event].nsec[species], 1ULL);
   spectrum(lt, species, (double)e);
 }
@@ -295,16 +294,16 @@ __device__ __forceinline__ void emit_new(const StepArgs &a, const Pool<R> &nxt, 
 // Positron at rest: two back-to-back 511 keV photons, isotropic (2 direction
 // draws, then 2 substream draws per photon), at neutral slots idx, idx+1.
 template <class R>
-__device__ __forceinline__ void emit_at_rest(const StepArgs &a, const Pool<R> &nxt, int idx, TrackRegs<R> &t,
-                                             LdsTally &lt, bool trace) {
+__device__ __forceinline__ void emit_at_rest(const StepArgs &a, const Pool<R> &nxt, int idx, int &nk,
+                                             TrackRegs<R> &t, LdsTally &lt, bool trace) {
   const double cost = 2.0 * t.rng.uniform() - 1.0;
   const double sint = sqrt((1.0 - cost) * (1.0 + cost));
   const double phi = kTwoPi * t.rng.uniform();
   const R d0 = (R)(sint * cos(phi)), d1 = (R)(sint * sin(phi)), d2 = (R)cost;
   const uint64_t u1 = child_substream(t.rng);
-  emit_new(a, nxt, idx, t, 0, (R)kMe, d0, d1, d2, u1, lt, trace);
+  emit_new(a, nxt, idx, nk, t, 0, (R)kMe, d0, d1, d2, u1, lt, trace);
   const uint64_t u2 = child_substream(t.rng);
-  emit_new(a, nxt, idx + 1, t, 0, (R)kMe, -d0, -d1, -d2, u2, lt, trace);
+  emit_new(a, nxt, idx + 1, nk, t, 0, (R)kMe, -d0, -d1, -d2, u2, lt, trace);
 }
 
 // PostStep GPIL of process p (ProxyEmProcess.hpp:122-151): resample the
@@ -349,10 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
       if (t.vol < 0) {
         unsafeAtomicAdd(&a.ev[t.event].leak, p.ekin);
       } else {
-        t.rng.s[0] = (double)kRngPending;
-        t.rng.s[1] = (double)(uint32_t)t.uid;
-        t.rng.s[2] = (double)(uint32_t)(t.uid >> 32);
-        t.rng.s[3] = t.rng.s[4] = t.rng.s[5] = 0.0;
+        substream_state(P, t.uid, t.rng.s);
         region = t.species == 0 ? 0 : 1;
         atomicAdd(&a.ev_live[t.event], 1);
       }
@@ -399,11 +395,6 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
       slot = kCharged ? (int)(cur.cap - 1 - i) : i;
       load_track(cur, slot, t, trace);
       if (!kCharged) t.species = 0;
-      if (t.rng.s[0] == (double)kRngPending) {
-        // first step of a new track: its state is substream uid of the seed
-        const uint64_t uid = (uint64_t)(uint32_t)t.rng.s[1] | ((uint64_t)(uint32_t)t.rng.s[2] << 32);
-        substream_state(P, uid, t.rng.s);
-      }
       const DevMaterial &m = P->mat[P->geo.slot[t.vol]];
       // species as a value the compiler can bound: 0 here, or 1 / 2 there
       const int sp = kCharged ? (t.species == 2 ? 2 : 1) : 0;
@@ -482,6 +473,7 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
       store_track(cur, slot, t, trace);
     }
     wave_add_u64(&a.cnt->steps, active);
+    if (kCharged) wave_add_u64(&a.cnt->steps_charged, active);
     // emission: survivors into the next pool
     const int outN = (survive && t.species == 0) ? 1 : 0;
     const int outC = (survive && t.species != 0) ? 1 : 0;
@@ -609,6 +601,8 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
     wave_add_u64(&a.cnt->interactions, active);
     const int bN = wave_reserve(outN, &a.cnt->nxt[0]);
     const int bC = wave_reserve(outC, &a.cnt->nxt[1]);
+    const int made = outN + outC - (survive ? 1 : 0);
+    int nk = wave_reserve(made, &a.cnt->nnew);
     if (active) {
       // new tracks in the oracle's draw order: secondary 0, secondary 1, then
       // the primary's annihilation at rest; the survivor last
@@ -617,9 +611,10 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
         const Sec<R> &s = o.sec[0];
         if (fate0 == 1) {
           const uint64_t u = child_substream(t.rng);
-          emit_new(a, nxt, s.species == 0 ? kN++ : kC++, t, s.species, s.e, s.d[0], s.d[1], s.d[2], u, lt, trace);
+          emit_new(a, nxt, s.species == 0 ? kN++ : kC++, nk, t, s.species, s.e, s.d[0], s.d[1], s.d[2], u, lt,
+                   trace);
         } else if (fate0 == 2) {
-          emit_at_rest(a, nxt, kN, t, lt, trace);
+          emit_at_rest(a, nxt, kN, nk, t, lt, trace);
           kN += 2;
         }
       }
@@ -627,14 +622,15 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
         const Sec<R> &s = o.sec[1];
         if (fate1 == 1) {
           const uint64_t u = child_substream(t.rng);
-          emit_new(a, nxt, s.species == 0 ? kN++ : kC++, t, s.species, s.e, s.d[0], s.d[1], s.d[2], u, lt, trace);
+          emit_new(a, nxt, s.species == 0 ? kN++ : kC++, nk, t, s.species, s.e, s.d[0], s.d[1], s.d[2], u, lt,
+                   trace);
         } else if (fate1 == 2) {
-          emit_at_rest(a, nxt, kN, t, lt, trace);
+          emit_at_rest(a, nxt, kN, nk, t, lt, trace);
           kN += 2;
         }
       }
       if (prim_rest) {
-        emit_at_rest(a, nxt, kN, t, lt, trace);
+        emit_at_rest(a, nxt, kN, nk, t, lt, trace);
         kN += 2;
       }
       if (survive) {
@@ -646,7 +642,6 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
       }
     }
     {
-      const int made = outN + outC - (survive ? 1 : 0);
       const int incl = wave_incl_scan(made);
       const int tot = __shfl(incl, 63, 64);
       if (tot > 0 && lane() == 63) atomicAdd(&a.cnt->created, (unsigned long long)tot);
@@ -654,6 +649,24 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
     if (live_delta != 0) atomicAdd(&a.ev_live[t.event], live_delta);
   }
   lds_flush(P, lt, a.rows);
+}
+
+// RNG state of the tracks created this iteration: substream uid of the seed,
+// one track per lane (the jump is 16 table-driven matrix-vector products, so
+// it runs here, without divergence and at full occupancy, rather than in the
+// interaction kernels).
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_derive(StepArgs a, PoolArgs<R> pa) {
+  const Pool<R> &nxt = pa.nxt;
+  const int64_t n = min((int64_t)max(a.cnt->nnew, 0), nxt.cap);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = a.newlist[i];
+    if (s < 0 || s >= nxt.cap) continue;
+    const uint64_t uid = (uint64_t)nxt.rng[nxt.cap + s] | ((uint64_t)nxt.rng[2 * nxt.cap + s] << 32);
+    Rng r;
+    substream_state(a.P, uid, r.s);
+    r.store(nxt.rng, nxt.cap, s);
+  }
 }
 
 __global__ void k_advance(Counters *c, int64_t cap) {
@@ -670,6 +683,7 @@ __global__ void k_advance(Counters *c, int64_t cap) {
     }
     c->nxt[0] = 0;
     c->nxt[1] = 0;
+    c->nnew = 0;
     for (int k = 0; k < kNumQueues; ++k) c->q[k] = 0;
   }
 }
@@ -764,13 +778,17 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, hipSt
   // ev (optional): 4 events recorded before/after the transport and interaction kernels
   if (ev) (void)hipEventRecord(ev[0], st);
   hipLaunchKernelGGL((k_transport<R, false>), dim3(grid), dim3(kBlock), 0, st, a, pa);
-  hipLaunchKernelGGL((k_transport<R, true>), dim3(grid), dim3(kBlock), 0, st, a, pa);
   if (ev) (void)hipEventRecord(ev[1], st);
-  hipLaunchKernelGGL((k_interact<R, false>), dim3(grid), dim3(kBlock), 0, st, a, pa);
-  hipLaunchKernelGGL((k_interact<R, true>), dim3(grid), dim3(kBlock), 0, st, a, pa);
+  hipLaunchKernelGGL((k_transport<R, true>), dim3(grid), dim3(kBlock), 0, st, a, pa);
   if (ev) (void)hipEventRecord(ev[2], st);
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, a.cnt, pa.cur.cap);
+  hipLaunchKernelGGL((k_interact<R, false>), dim3(grid), dim3(kBlock), 0, st, a, pa);
   if (ev) (void)hipEventRecord(ev[3], st);
+  hipLaunchKernelGGL((k_interact<R, true>), dim3(grid), dim3(kBlock), 0, st, a, pa);
+  if (ev) (void)hipEventRecord(ev[4], st);
+  hipLaunchKernelGGL(k_derive<R>, dim3(2 * grid), dim3(kBlock), 0, st, a, pa);
+  if (ev) (void)hipEventRecord(ev[5], st);
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, a.cnt, pa.cur.cap);
+  if (ev) (void)hipEventRecord(ev[6], st);
   return hipGetLastError();
 }
 

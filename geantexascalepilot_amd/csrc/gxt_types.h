@@ -111,8 +111,9 @@ struct Counters {
   int32_t overflow;        // a pool or queue write would have gone out of range
   int32_t prim_head;       // primaries consumed from the primary queue
   int32_t prim_tail;       // primaries available in the primary queue
-  int32_t pad;
+  int32_t nnew;            // tracks created this iteration (entries of the new-track list)
   unsigned long long steps;   // track-steps
+  unsigned long long steps_charged;  // of which e-/e+ steps
   unsigned long long killed;  // tracks killed by the step limit
   unsigned long long trace_n; // trace records written
   unsigned long long interactions;  // tracks sampled by the interaction kernel

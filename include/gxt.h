@@ -105,11 +105,16 @@ typedef struct gxt_stats {
 
 /* Kernel timings of the step loop, accumulated while profile_kernels is set. */
 typedef struct gxt_kernel_stats {
-  double transport_ms;    /* summed duration of the transport-kernel launches      */
-  double interact_ms;     /* summed duration of the interaction-kernel launches    */
+  double transport_neutral_ms; /* summed durations of the photon transport launches    */
+  double transport_charged_ms; /* summed durations of the e-/e+ transport launches     */
+  double interact_neutral_ms;  /* summed durations of the photon interaction launches  */
+  double interact_charged_ms;  /* summed durations of the e-/e+ interaction launches   */
+  double derive_ms;       /* summed durations of the new-track RNG derivation       */
   double advance_ms;      /* summed duration of the counter-advance launches        */
-  int64_t launches;       /* step-loop iterations timed                            */
+  int64_t launches;       /* step-loop iterations timed (one launch of each kernel) */
   int64_t track_steps;    /* tracks stepped (one per live track per iteration)     */
+  int64_t neutral_steps;  /* of which photon steps                                 */
+  int64_t charged_steps;  /* of which e-/e+ steps                                  */
   int64_t interactions;   /* tracks sampled by the interaction kernel              */
   int64_t created;        /* tracks created by the interaction kernel              */
   int32_t record_bytes;   /* bytes of one track record in the pool                 */
