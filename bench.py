@@ -28,6 +28,20 @@ PEAK_HBM_GBS = 8000.0
 ME = 0.510998910
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` as measured by the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a copy
+    kernel of known size); None when the summary does not list the kernel."""
+    import re
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_c1_fp64.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"^\| %s \|(?:[^|\n]*\|){3}\s*([0-9.eE+]+)\s*\|" % re.escape(kernel), text, re.M)
+    return float(m.group(1)) if m else None
+
+
 def log(msg):
     sys.stderr.write("[bench] %s\n" % msg)
     sys.stderr.flush()
@@ -153,6 +167,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp32", action="store_true")
     ap.add_argument("--pool", type=int, default=2_000_000, help="live-track pool capacity")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run one device-to-device copy of MIB MiB (a kernel of known traffic that "
+                         "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
     a = ap.parse_args()
     if a.gpus != 1:
         sys.stderr.write("bench.py: this build runs the step loop on one GPU; multi-GPU runs are not supported "
@@ -164,6 +181,14 @@ def main():
     from geantexascalepilot_amd import abi
     import geantexascalepilot_amd.gxt as g
 
+    if a.copy_probe > 0:
+        src = torch.ones(a.copy_probe << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: %d MiB read, %d MiB written" % (a.copy_probe, a.copy_probe))
+        del src, dst
     n = a.events
     r64 = run_batch(g, abi, abi.FP64, n, a.steps, a.warmup, pool=a.pool)
     value = a.steps * n / r64["seconds"]
@@ -192,7 +217,7 @@ def main():
     avg_us = 1e3 * ms / launches
     achieved = (nbytes / launches) / (avg_us * 1e-6) / 1e9
     roofline = dict(bound="hbm", achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s", frac=achieved / PEAK_HBM_GBS,
-                    traffic=None, kernel=name, launches=int(ks.launches), avg_launch_us=avg_us,
+                    traffic=profiled_traffic(name), kernel=name, launches=int(ks.launches), avg_launch_us=avg_us,
                     bytes_per_launch=nbytes / launches, per_unit=per_unit, record_bytes_in_pool=int(ks.record_bytes),
                     kernel_ms={k: v[0] for k, v in kt.items()} | {"k_derive": ks.derive_ms,
                                                                   "k_advance": ks.advance_ms})

@@ -252,6 +252,27 @@ int load_material(const std::string &data, int which, double lar_density, HostDa
     dt.base = std::log(v.x[0]) / dt.dl;
     dt.dlf = (float)dt.dl;
     dt.basef = (float)dt.base;
+    dt.hint = -1;
+    dt.nhint = 0;
+    dt.hlog0 = 0.0f;
+    dt.hscale = 0.0f;
+    if (t == GXT_TAB_INVRANGE_EM || t == GXT_TAB_INVRANGE_EP) {
+      // hint k: last node below u_k = x[0] 2^(k / hscale); the device starts
+      // its search there, one bin below r's, and steps to the last node below r
+      const int K = kInvRangeHints;
+      const int n = (int)v.x.size();
+      const double l0 = std::log2(v.x[0]), l1 = std::log2(v.x[n - 1]);
+      dt.hint = (int)hd.hints.size();
+      dt.nhint = K;
+      dt.hlog0 = (float)l0;
+      dt.hscale = (float)(K / (l1 - l0));
+      for (int k = 0; k < K; ++k) {
+        const double u = std::exp2(l0 + k / (double)dt.hscale);
+        int j = 0;
+        while (j + 1 < n - 1 && v.x[j + 1] < u) ++j;
+        hd.hints.push_back(j);
+      }
+    }
     hd.x.insert(hd.x.end(), v.x.begin(), v.x.end());
     hd.y.insert(hd.y.end(), v.y.begin(), v.y.end());
     hd.d2.insert(hd.d2.end(), v.d2.begin(), v.d2.end());

@@ -14,6 +14,7 @@ struct HostData {
   std::vector<double> sb;
   std::vector<double> sandia;
   std::vector<double> shells;
+  std::vector<int> hints;
 };
 
 // Load material `which` (GXT_MAT_*) into params.mat[slot].

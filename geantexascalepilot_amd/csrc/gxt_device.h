@@ -157,11 +157,13 @@ __device__ __forceinline__ uint64_t child_substream(Rng &r) {
 template <class R> struct Nodes;
 template <> struct Nodes<double> {
   const double *x, *y, *d2;
-  __device__ Nodes(const DevParams *P) : x(P->xd), y(P->yd), d2(P->d2d) {}
+  const int *hints;
+  __device__ Nodes(const DevParams *P) : x(P->xd), y(P->yd), d2(P->d2d), hints(P->hints) {}
 };
 template <> struct Nodes<float> {
   const float *x, *y, *d2;
-  __device__ Nodes(const DevParams *P) : x(P->xf), y(P->yf), d2(P->d2f) {}
+  const int *hints;
+  __device__ Nodes(const DevParams *P) : x(P->xf), y(P->yf), d2(P->d2f), hints(P->hints) {}
 };
 
 template <class R> __device__ __forceinline__ R tdl(const DevTable &t);
@@ -202,15 +204,16 @@ __device__ __forceinline__ R pvec_inv_value(const Nodes<R> &N, const DevTable &t
   const int o = t.off, n = t.n;
   if (r <= N.x[o]) return N.y[o];
   if (r >= N.x[o + n - 1]) return N.y[o + n - 1];
-  int n1 = 0, n2 = n / 2, n3 = n - 1;
-  while (n1 != n3 - 1) {
-    if (r > N.x[o + n2])
-      n1 = n2;
-    else
-      n3 = n2;
-    n2 = n1 + (n3 - n1 + 1) / 2;
-  }
-  return spline(N, o + n1, r);
+  // The binary search ends at the last node below r, j with x[j] < r <= x[j+1].
+  // The same node is reached from the hint of the log2 bin one below r's
+  // (the FP32 log2 is off by far less than a bin), by stepping up while the
+  // next node is below r: about one dependent load instead of log2(n).
+  int k = (int)floorf((__log2f((float)r) - t.hlog0) * t.hscale) - 1;
+  k = k < 0 ? 0 : (k >= t.nhint ? t.nhint - 1 : k);
+  int j = N.hints[t.hint + k];
+  while (j > 0 && !(N.x[o + j] < r)) --j;
+  while (j < n - 2 && N.x[o + j + 1] < r) ++j;
+  return spline(N, o + j, r);
 }
 
 // Seltzer-Berger grid: x[32], y[57], v[57][32] at offset sb.

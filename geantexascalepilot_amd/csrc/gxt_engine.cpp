@@ -56,7 +56,9 @@ struct DeviceData {
   double *xd = nullptr, *yd = nullptr, *d2d = nullptr, *sbd = nullptr, *sandia = nullptr, *shells = nullptr;
   float *xf = nullptr, *yf = nullptr, *d2f = nullptr, *sbf = nullptr;
   uint32_t *jump8 = nullptr;
+  int *hints = nullptr;
   void release() {
+    (void)hipFree(hints);
     (void)hipFree(P);
     (void)hipFree(jump8);
     (void)hipFree(xd);
@@ -95,7 +97,9 @@ int upload(HostData &hd, DeviceData &dd) {
   HIP_TRY(put(&dd.sbf, tof(hd.sb)));
   HIP_TRY(put(&dd.sandia, hd.sandia));
   HIP_TRY(put(&dd.shells, hd.shells));
+  HIP_TRY(put(&dd.hints, hd.hints));
   DevParams &p = hd.params;
+  p.hints = dd.hints;
   std::vector<uint32_t> j8;
   build_jump_tables(p.jump, j8);
   HIP_TRY(put(&dd.jump8, j8));
@@ -240,6 +244,7 @@ struct gxt_ctx {
     a.rows.bins = rows;
     a.rows.spec = spec;
     a.rows.stride = row_stride;
+    a.rows.nrows = grid;
     a.trace = trace_buf;
     a.trace_cap = trace_cap;
     return a;
@@ -252,6 +257,12 @@ struct gxt_ctx {
 };
 
 namespace {
+
+int64_t live_tracks(const Counters &h) {
+  int64_t n = 0;
+  for (int s = 0; s < kStripes; ++s) n += (int64_t)h.cur[0][s] + h.cur[1][s];
+  return n;
+}
 
 // Estimated peak live tracks of an event, from its primaries' energy.
 double event_footprint(const gxt_ctx *c, const Job *j, int e) {
@@ -398,9 +409,8 @@ int collect(gxt_ctx *c, bool &idle) {
   }
   c->ev_pending.clear();
   const Counters &h = *c->hcnt;
-  if (h.overflow || (int64_t)h.cur[0] + h.cur[1] > c->cap)
-    return c->fail(GXT_ECAPACITY, "track pool or queue overflow (increase pool_capacity)");
-  const int64_t live = (int64_t)h.cur[0] + h.cur[1];
+  if (h.overflow) return c->fail(GXT_ECAPACITY, "track pool or queue overflow (increase pool_capacity)");
+  const int64_t live = live_tracks(h);
   c->max_live = std::max<int64_t>(c->max_live, live);
   std::vector<Job *> fin;
   double gev_in_flight = 0;
@@ -490,8 +500,7 @@ int round_trip(gxt_ctx *c, int k, bool &idle) {
 }
 
 int iters_per_sync(gxt_ctx *c) {
-  const int64_t live = (int64_t)c->hcnt->cur[0] + c->hcnt->cur[1];
-  return live > 100000 ? 8 : 4;
+  return live_tracks(*c->hcnt) > 100000 ? 8 : 4;
 }
 
 void serve_loop(gxt_ctx *c) {
@@ -619,7 +628,8 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
   c->data_dir = cfg->data_dir;
   c->cfg.data_dir = c->data_dir.c_str();
   c->precision = cfg->precision;
-  c->cap = cfg->pool_capacity;
+  // whole stripes
+  c->cap = (cfg->pool_capacity + kStripes - 1) / kStripes * kStripes;
   c->trace = cfg->trace_tracks > 0;
   c->prof = cfg->profile_kernels > 0;
   c->t0 = std::chrono::steady_clock::now();

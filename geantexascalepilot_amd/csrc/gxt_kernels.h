@@ -9,10 +9,14 @@ namespace gxt {
 
 // Per-workgroup tally rows: bins[row * stride + (z | nz + r | nz + nr + vol)]
 // and spec[row * 3 * kSpecBins + species * kSpecBins + bin].
+// Per-workgroup histogram rows: workgroup b adds into row b % nrows
+// (z bins, r bins, volumes; spectra: 3 species x kSpecBins).
 struct TallyRows {
   double *bins;
   unsigned long long *spec;
   int64_t stride;
+  int nrows;
+  int pad;
 };
 
 struct StepArgs {

@@ -11,9 +11,13 @@
 //   k_interact  : the concatenated queues are sampled, so a wave runs one
 //                 sampler; survivors and secondaries are appended to the next
 //                 pool by wave prefix sums (one atomic per wave and region);
+//   k_derive    : the RNG states of the tracks created this iteration;
 //   k_advance   : the next pool's counts become the current ones.
-// Tallies go to per-workgroup LDS histograms flushed into per-workgroup rows
-// (no global atomics on hot bins); per-event sums use FP64 global atomics.
+// The pool, the queues and the new-track list are striped (kStripes stripes
+// with their own counters; workgroup b appends to stripe b % kStripes) so no
+// counter takes the atomics of the whole grid.  Histogram tallies are
+// no-return FP64 atomics into per-workgroup rows; per-event sums are FP64
+// atomics on the event's record.
 #include <hip/hip_runtime.h>
 
 #include "gxt_device.h"
@@ -61,64 +65,86 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long *ctr, int pred) 
   }
 }
 
-// Append slot to queue q (q < 0: none) with one atomic per wave and queue.
-__device__ __forceinline__ void queue_push(int q, int slot, int *qcnt, int32_t *queues, int64_t cap,
-                                           int *overflow) {
+// Stripe the calling workgroup appends to.
+__device__ __forceinline__ int my_stripe() { return (int)(blockIdx.x % kStripes); }
+
+// Item counts of one region over the stripes, as a prefix, with the mapping
+// from a global item index to (stripe, offset in the stripe).
+struct StripeSpan {
+  int pre[kStripes + 1];
+  __device__ __forceinline__ int total() const { return pre[kStripes]; }
+  // (selects over the unrolled prefix: a lane-dependent index into pre[]
+  // would move it out of registers)
+  __device__ __forceinline__ void locate(int i, int &s, int &off) const {
+    s = 0;
+    int base = 0;
+#pragma unroll
+    for (int k = 1; k < kStripes; ++k) {
+      const bool ge = i >= pre[k];
+      s = ge ? k : s;
+      base = ge ? pre[k] : base;
+    }
+    off = i - base;
+  }
+};
+
+// Two-phase wave reservation.  issue() sends the wave's one atomic and get()
+// uses its result; issuing several before the first get() keeps their
+// atomics in flight together.  (vmcnt counts loads, stores and atomics in
+// issue order, so a returning atomic issued after stores also waits for
+// them: the kernels issue their reservations before their stores.)
+struct WaveReserve {
+  int n, incl, base;
+  __device__ __forceinline__ void issue(int cnt, int *ctr) {
+    n = cnt;
+    incl = wave_incl_scan(cnt);
+    const int total = __shfl(incl, 63, 64);
+    base = 0;
+    if (total > 0 && lane() == 63) base = atomicAdd(ctr, total);
+  }
+  __device__ __forceinline__ int get() const { return __shfl(base, 63, 64) + incl - n; }
+};
+
+// Reserve an entry of queue q (q < 0: none; q in [Q0, Q0 + NQ)) of the
+// workgroup's stripe, one atomic per queue present in the wave, all issued
+// before any result is used.  Returns the entry's index in queues[], -1 for
+// none, -2 when the stripe of the queue is full.
+template <int Q0, int NQ>
+__device__ __forceinline__ int64_t queue_reserve(int q, Counters *cnt, int64_t cap, int64_t sc) {
   const unsigned long long lt = (lane() == 0) ? 0ULL : ((~0ULL) >> (64 - lane()));
-#pragma unroll 1
-  for (int k = 0; k < kNumQueues; ++k) {
-    const unsigned long long m = __ballot(q == k);
-    if (m) {
-      const int leader = __ffsll((long long)m) - 1;
-      int base = 0;
-      if (lane() == leader) base = atomicAdd(&qcnt[k], (int)__popcll(m));
-      base = __shfl(base, leader, 64);
-      if (q == k) {
-        const int64_t idx = (int64_t)base + __popcll(m & lt);
-        if (idx < cap)
-          queues[(int64_t)k * cap + idx] = slot;
-        else
-          atomicOr(overflow, 1);
+  const int st = my_stripe();
+  unsigned long long m[NQ];
+  int base[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    m[k] = __ballot(q == Q0 + k);
+    base[k] = 0;
+    if (m[k] && lane() == __ffsll((long long)m[k]) - 1) base[k] = atomicAdd(&cnt->q[Q0 + k][st], (int)__popcll(m[k]));
+  }
+  int64_t pos = -1;
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    if (m[k]) {
+      const int b = __shfl(base[k], __ffsll((long long)m[k]) - 1, 64);
+      if (q == Q0 + k) {
+        const int64_t idx = (int64_t)b + __popcll(m[k] & lt);
+        pos = idx < sc ? (int64_t)(Q0 + k) * cap + (int64_t)st * sc + idx : -2;
       }
     }
   }
+  return pos;
 }
 
 // ------------------------------------------------------------------ tallies
-struct LdsTally {
-  double z[kMaxZBins];
-  double r[kMaxRBins];
-  double v[kMaxVol];
-  unsigned int spec[3 * kSpecBins];
-};
-
-__device__ __forceinline__ void lds_clear(LdsTally &t) {
-  for (int i = threadIdx.x; i < kMaxZBins; i += blockDim.x) t.z[i] = 0.0;
-  for (int i = threadIdx.x; i < kMaxRBins; i += blockDim.x) t.r[i] = 0.0;
-  for (int i = threadIdx.x; i < kMaxVol; i += blockDim.x) t.v[i] = 0.0;
-  for (int i = threadIdx.x; i < 3 * kSpecBins; i += blockDim.x) t.spec[i] = 0u;
-  __syncthreads();
+// Row of the calling workgroup: z bins, r bins, volumes; then its spectrum row.
+__device__ __forceinline__ double *tally_row(const TallyRows &r) {
+  return r.bins + (int64_t)(blockIdx.x % r.nrows) * r.stride;
 }
 
-// Add the workgroup's histograms to its own row (no atomics: one owner).
-__device__ __forceinline__ void lds_flush(const DevParams *P, LdsTally &t, const TallyRows &rows) {
-  __syncthreads();
-  const int nz = P->nz, nr = P->nr, nv = P->geo.nvol;
-  double *row = rows.bins + (int64_t)blockIdx.x * rows.stride;
-  for (int i = threadIdx.x; i < nz; i += blockDim.x)
-    if (t.z[i] != 0.0) row[i] += t.z[i];
-  for (int i = threadIdx.x; i < nr; i += blockDim.x)
-    if (t.r[i] != 0.0) row[nz + i] += t.r[i];
-  for (int i = threadIdx.x; i < nv; i += blockDim.x)
-    if (t.v[i] != 0.0) row[nz + nr + i] += t.v[i];
-  unsigned long long *srow = rows.spec + (int64_t)blockIdx.x * 3 * kSpecBins;
-  for (int i = threadIdx.x; i < 3 * kSpecBins; i += blockDim.x)
-    if (t.spec[i]) srow[i] += t.spec[i];
-}
-
-__device__ __forceinline__ void spectrum(LdsTally &t, int species, double e) {
+__device__ __forceinline__ void spectrum(const TallyRows &r, int species, double e) {
   const int b = (int)floor((m_log10(e) + 3.0) * 8.0);
-  if (b >= 0 && b < kSpecBins) atomicAdd(&t.spec[species * kSpecBins + b], 1u);
+  if (b >= 0 && b < kSpecBins)
+    atomicAdd(&r.spec[(int64_t)(blockIdx.x % r.nrows) * 3 * kSpecBins + species * kSpecBins + b], 1ULL);
 }
 
 // ------------------------------------------------------------------ pool I/O
@@ -216,29 +242,50 @@ __device__ inline void store_new(const DevParams *P, const Pool<R> &p, int64_t i
   if (trace) p.uid[i] = uid;
 }
 
-// Slot of output idx of a region in the next pool (neutral from the bottom,
-// charged from the top); -1 when it would leave the pool.
-__device__ __forceinline__ int64_t out_slot(int region, int idx, int64_t cap) {
-  if (idx < 0 || (int64_t)idx >= cap) return -1;
-  return region == 0 ? (int64_t)idx : cap - 1 - (int64_t)idx;
+// Slot of item idx of a region in stripe st of a pool (neutral from the
+// bottom of the stripe, charged from its top); -1 when it would leave the
+// stripe.
+__device__ __forceinline__ int64_t out_slot(int region, int st, int idx, int64_t sc) {
+  if (idx < 0 || (int64_t)idx >= sc) return -1;
+  return region == 0 ? (int64_t)st * sc + idx : (int64_t)(st + 1) * sc - 1 - idx;
 }
 
-// Local deposit: into the track's own tallies and the workgroup histograms.
+// Pool slots per stripe.
 template <class R>
-__device__ __forceinline__ void deposit(const DevParams *P, LdsTally &lt, TrackRegs<R> &t, double e, const double p[3],
-                                        int vol) {
-  if (e <= 0.0) return;
-  t.aed += e;
-  if (P->geo.visible[vol]) t.avis += e;
-  unsafeAtomicAdd(&lt.v[vol], e);
-  int iz = (int)floor(p[2] / P->zbin);
-  iz = iz < 0 ? 0 : (iz >= P->nz ? P->nz - 1 : iz);
-  unsafeAtomicAdd(&lt.z[iz], e);
-  const double r = sqrt(p[0] * p[0] + p[1] * p[1]);
-  int ir = (int)(r / P->rbin);
-  ir = ir < 0 ? 0 : (ir >= P->nr ? P->nr - 1 : ir);
-  unsafeAtomicAdd(&lt.r[ir], e);
+__device__ __forceinline__ int64_t stripe_cap(const Pool<R> &p) {
+  return p.cap / kStripes;
 }
+
+// A deposit held back until the kernel's stores: the track's accumulators
+// are updated at once, the workgroup's histogram row by flush().  Every add()
+// of one Dep is at the same point.
+struct Dep {
+  double e = 0.0;
+  int iz = 0, ir = 0, vol = 0;
+  template <class R>
+  __device__ __forceinline__ void add(const DevParams *P, TrackRegs<R> &t, double de, const double p[3], int v) {
+    if (de <= 0.0) return;
+    t.aed += de;
+    if (P->geo.visible[v]) t.avis += de;
+    if (e == 0.0) {
+      int z = (int)floor(p[2] / P->zbin);
+      iz = z < 0 ? 0 : (z >= P->nz ? P->nz - 1 : z);
+      const double r = sqrt(p[0] * p[0] + p[1] * p[1]);
+      int rr = (int)(r / P->rbin);
+      ir = rr < 0 ? 0 : (rr >= P->nr ? P->nr - 1 : rr);
+      vol = v;
+    }
+    e += de;
+  }
+  __device__ __forceinline__ void flush(const StepArgs &a) const {
+    if (e <= 0.0) return;
+    double *row = tally_row(a.rows);
+    const int nz = a.P->nz, nr = a.P->nr;
+    unsafeAtomicAdd(&row[nz + nr + vol], e);
+    unsafeAtomicAdd(&row[iz], e);
+    unsafeAtomicAdd(&row[nz + ir], e);
+  }
+};
 
 // A track dies: its tallies go to its event (one set of atomics per track).
 template <class R>
@@ -273,37 +320,40 @@ __device__ __forceinline__ void trace_rec(const StepArgs &a, const TrackRegs<R> 
   r[11] = edep;
 }
 
-// Write one new track into the next pool (region 0 neutral, 1 charged) and
-// its slot into entry nk of the new-track list.
+// Write one new track into the workgroup's stripe of the next pool (region 0
+// neutral, 1 charged) and its slot into entry nk of the stripe's new-track
+// list.
 template <class R>
 __device__ __forceinline__ void emit_new(const StepArgs &a, const Pool<R> &nxt, int idx, int &nk,
                                          const TrackRegs<R> &par, int species, R e, R d0, R d1, R d2, uint64_t uid,
-                                         LdsTally &lt, bool trace) {
-  const int64_t s = out_slot(species == 0 ? 0 : 1, idx, nxt.cap);
-  if (s >= 0 && nk >= 0 && (int64_t)nk < nxt.cap) {
+                                         bool trace) {
+  const int64_t sc = stripe_cap(nxt);
+  const int st = my_stripe();
+  const int64_t s = out_slot(species == 0 ? 0 : 1, st, idx, sc);
+  if (s >= 0 && nk >= 0 && (int64_t)nk < sc) {
     store_new(a.P, nxt, s, par, species, e, d0, d1, d2, uid, trace);
-    a.newlist[nk] = (int32_t)s;
+    a.newlist[(int64_t)st * sc + nk] = (int32_t)s;
   } else {
     atomicOr(&a.cnt->overflow, 1);
   }
   ++nk;
   atomicAdd(&a.ev[par.event].nsec[species], 1ULL);
-  spectrum(lt, species, (double)e);
+  spectrum(a.rows, species, (double)e);
 }
 
 // Positron at rest: two back-to-back 511 keV photons, isotropic (2 direction
 // draws, then 2 substream draws per photon), at neutral slots idx, idx+1.
 template <class R>
 __device__ __forceinline__ void emit_at_rest(const StepArgs &a, const Pool<R> &nxt, int idx, int &nk,
-                                             TrackRegs<R> &t, LdsTally &lt, bool trace) {
+                                             TrackRegs<R> &t, bool trace) {
   const double cost = 2.0 * t.rng.uniform() - 1.0;
   const double sint = sqrt((1.0 - cost) * (1.0 + cost));
   const double phi = kTwoPi * t.rng.uniform();
   const R d0 = (R)(sint * cos(phi)), d1 = (R)(sint * sin(phi)), d2 = (R)cost;
   const uint64_t u1 = child_substream(t.rng);
-  emit_new(a, nxt, idx, nk, t, 0, (R)kMe, d0, d1, d2, u1, lt, trace);
+  emit_new(a, nxt, idx, nk, t, 0, (R)kMe, d0, d1, d2, u1, trace);
   const uint64_t u2 = child_substream(t.rng);
-  emit_new(a, nxt, idx + 1, nk, t, 0, (R)kMe, -d0, -d1, -d2, u2, lt, trace);
+  emit_new(a, nxt, idx + 1, nk, t, 0, (R)kMe, -d0, -d1, -d2, u2, trace);
 }
 
 // PostStep GPIL of process p (ProxyEmProcess.hpp:122-151): resample the
@@ -353,10 +403,11 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
         atomicAdd(&a.ev_live[t.event], 1);
       }
     }
-    const int b0 = wave_reserve(region == 0 ? 1 : 0, &a.cnt->cur[0]);
-    const int b1 = wave_reserve(region == 1 ? 1 : 0, &a.cnt->cur[1]);
+    const int st = my_stripe();
+    const int b0 = wave_reserve(region == 0 ? 1 : 0, &a.cnt->cur[0][st]);
+    const int b1 = wave_reserve(region == 1 ? 1 : 0, &a.cnt->cur[1][st]);
     if (region >= 0) {
-      const int64_t s = out_slot(region, region == 0 ? b0 : b1, cur.cap);
+      const int64_t s = out_slot(region, st, region == 0 ? b0 : b1, stripe_cap(cur));
       if (s >= 0)
         store_track(cur, s, t, trace);
       else
@@ -370,18 +421,28 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
 // class is a compile-time constant so each variant carries only its own code.
 template <class R, bool kCharged>
 __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a, PoolArgs<R> pa) {
-  __shared__ LdsTally lt;
-  lds_clear(lt);
   const DevParams *P = a.P;
   const Pool<R> &cur = pa.cur;
   const Pool<R> &nxt = pa.nxt;
   const Nodes<R> N(P);
   const bool trace = P->trace_events > 0;
-  // counts are clamped to the pool so a corrupted counter cannot index past it
-  const int64_t capc = cur.cap;
-  const int nN = (int)min((int64_t)max(a.cnt->cur[0], 0), capc);
-  const int nC = (int)min((int64_t)max(a.cnt->cur[1], 0), capc - nN);
-  const int total = kCharged ? nC : nN;
+  // counts are clamped to the stripes so a corrupted counter cannot index past them
+  const int64_t sc = stripe_cap(cur);
+  const int st = my_stripe();
+  StripeSpan span;
+  span.pre[0] = 0;
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k < kStripes; ++k) {
+    const int64_t r0 = a.cnt->cur[0][k], r1 = a.cnt->cur[1][k];
+    bad = bad || r0 < 0 || r1 < 0 || r0 + r1 > sc;
+    const int64_t n0 = min(max(r0, (int64_t)0), sc);
+    const int64_t n1 = min(max(r1, (int64_t)0), sc - n0);
+    span.pre[k + 1] = span.pre[k] + (int)(kCharged ? n1 : n0);
+  }
+  // injected primaries can fill a stripe from both ends: report, never overlap
+  if (bad && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.cnt->overflow, 1);
+  const int total = span.total();
   const R mine = (R)P->tracking_cut;
   for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < total; base += gridDim.x * blockDim.x) {
     const int i = base + lane();
@@ -389,10 +450,13 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
     TrackRegs<R> t;
     int q = -1;
     int slot = 0;
-    bool survive = false, annihilate = false;
-    int live_delta = 0;
+    bool survive = false, annihilate = false, dies = false, killed = false, escaped = false;
+    Dep dA, dB;  // along-step loss (mid-step, pre-step volume); local deposit at the end point
+    double edep = 0.0;
     if (active) {
-      slot = kCharged ? (int)(cur.cap - 1 - i) : i;
+      int is, off;
+      span.locate(i, is, off);
+      slot = (int)out_slot(kCharged ? 1 : 0, is, off, sc);
       load_track(cur, slot, t, trace);
       if (!kCharged) t.species = 0;
       const DevMaterial &m = P->mat[P->geo.slot[t.vol]];
@@ -420,87 +484,91 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
       if (xs1 > R(0)) t.nil1 -= (R)step * xs1;
       if (xs2 > R(0)) t.nil2 -= (R)step * xs2;
       t.nsteps++;
-      double edep = 0.0;
       if (charged) {
         t.alen += step;
         R loss = R(0);
         t.e = along_doit(N, m, t.species == 2, t.e, (R)step, mine, loss);
         const double mid[3] = {0.5 * (pre[0] + t.pos[0]), 0.5 * (pre[1] + t.pos[1]), 0.5 * (pre[2] + t.pos[2])};
-        deposit(P, lt, t, (double)loss, mid, prevol);
+        dA.add(P, t, (double)loss, mid, prevol);
         edep += (double)loss;
       }
       int tproc = status == kPhysics ? sel : (status == kBoundary ? -2 : (status == kEscaped ? -3 : -1));
       if (status == kPhysics && sel < 0) tproc = -1;
       if (status == kEscaped) {
-        unsafeAtomicAdd(&a.ev[t.event].leak, (double)t.e);
         if (trace) trace_rec(a, t, tproc, edep);
-        retire(a, t);
-        live_delta = -1;
+        escaped = true;
+        dies = true;
       } else if (t.e <= R(0)) {
         if (trace) trace_rec(a, t, tproc, edep);
         annihilate = t.species == 2;
-        if (!annihilate) {
-          retire(a, t);
-          live_delta = -1;
-        }
+        dies = !annihilate;
       } else if (status == kPhysics && sel >= 0) {
         t.set_nil(sel, R(-1));
         q = queue_of(t.species, sel);
-        store_track(cur, slot, t, trace);
-        if (trace) cur.tedep[slot] = edep;
       } else {
         if (trace) trace_rec(a, t, tproc, edep);
         if (t.e < mine) {
-          deposit(P, lt, t, (double)t.e, t.pos, t.vol);
+          dB.add(P, t, (double)t.e, t.pos, t.vol);
           annihilate = t.species == 2;
-          if (!annihilate) {
-            retire(a, t);
-            live_delta = -1;
-          }
+          dies = !annihilate;
         } else if (t.nsteps >= P->max_steps) {
-          deposit(P, lt, t, (double)t.e, t.pos, t.vol);
-          atomicAdd(&a.cnt->killed, 1ULL);
-          retire(a, t);
-          live_delta = -1;
+          dB.add(P, t, (double)t.e, t.pos, t.vol);
+          killed = true;
+          dies = true;
         } else {
           survive = true;
         }
       }
-    }
-    if (annihilate) {
       // positron at rest: its two photons are made by the interaction kernel
-      q = kQueueAtRest;
-      store_track(cur, slot, t, trace);
+      if (annihilate) q = kQueueAtRest;
+    }
+    // returning atomics first, all in flight together: the survivor's slot in
+    // the next pool and the queue entry
+    WaveReserve rs;
+    rs.issue(survive ? 1 : 0, &a.cnt->nxt[kCharged ? 1 : 0][st]);
+    const int64_t qpos = queue_reserve<kCharged ? 3 : 0, kCharged ? kNumQueues - 3 : 3>(q, a.cnt, cur.cap, sc);
+    const int bs = rs.get();
+    // then stores and no-return atomics
+    if (active) {
+      dA.flush(a);
+      dB.flush(a);
+      if (escaped) unsafeAtomicAdd(&a.ev[t.event].leak, (double)t.e);
+      if (killed) atomicAdd(&a.cnt->killed, 1ULL);
+      if (dies) {
+        retire(a, t);
+        atomicAdd(&a.ev_live[t.event], -1);
+      }
+      if (q >= 0) {
+        // interacting (or stopped) track: back in place for the interaction kernel
+        store_track(cur, slot, t, trace);
+        if (trace && q != kQueueAtRest) cur.tedep[slot] = edep;
+      }
+      if (survive) {
+        const int64_t s = out_slot(kCharged ? 1 : 0, st, bs, sc);
+        if (s >= 0)
+          store_track(nxt, s, t, trace);
+        else
+          atomicOr(&a.cnt->overflow, 1);
+      }
+      if (qpos >= 0)
+        a.queues[qpos] = slot;
+      else if (qpos == -2)
+        atomicOr(&a.cnt->overflow, 1);
     }
     wave_add_u64(&a.cnt->steps, active);
     if (kCharged) wave_add_u64(&a.cnt->steps_charged, active);
-    // emission: survivors into the next pool
-    const int outN = (survive && t.species == 0) ? 1 : 0;
-    const int outC = (survive && t.species != 0) ? 1 : 0;
-    const int bN = wave_reserve(outN, &a.cnt->nxt[0]);
-    const int bC = wave_reserve(outC, &a.cnt->nxt[1]);
-    if (survive) {
-      const int64_t s = t.species == 0 ? out_slot(0, bN, nxt.cap) : out_slot(1, bC, nxt.cap);
-      if (s >= 0)
-        store_track(nxt, s, t, trace);
-      else
-        atomicOr(&a.cnt->overflow, 1);
-    }
-    queue_push(q, slot, a.cnt->q, a.queues, cur.cap, &a.cnt->overflow);
-    if (live_delta != 0) atomicAdd(&a.ev_live[t.event], live_delta);
   }
-  lds_flush(P, lt, a.rows);
 }
 
 // Fate of a secondary: pushed (>= tracking cut), or deposited (and, for a
 // positron, annihilated at rest).
 template <class R>
-__device__ __forceinline__ int sec_fate(const DevParams *P, LdsTally &lt, TrackRegs<R> &t, const Sec<R> &s, R mine,
-                                        int &outN, int &outC, const StepArgs &a) {
+__device__ __forceinline__ int sec_fate(const DevParams *P, TrackRegs<R> &t, Dep &dp, const Sec<R> &s, R mine,
+                                        int &outN, int &outC, int &nannih) {
   if (s.e < mine) {
-    deposit(P, lt, t, (double)s.e, t.pos, t.vol);
+    dp.add(P, t, (double)s.e, t.pos, t.vol);
     if (s.species == 2) {
-      atomicAdd(&a.ev[t.event].nannihil, 1ULL);
+      ++nannih;
       outN += 2;
       return 2;
     }
@@ -515,22 +583,28 @@ __device__ __forceinline__ int sec_fate(const DevParams *P, LdsTally &lt, TrackR
 
 template <class R, bool kCharged>
 __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a, PoolArgs<R> pa) {
-  __shared__ LdsTally lt;
-  lds_clear(lt);
   const DevParams *P = a.P;
   const Pool<R> &cur = pa.cur;
   const Pool<R> &nxt = pa.nxt;
   const bool trace = P->trace_events > 0;
-  // photon queues 0-2 in one variant, e-/e+ queues 3-8 in the other
+  const int64_t sc = stripe_cap(cur);
+  const int st = my_stripe();
+  // photon queues 0-2 in one variant, e-/e+ queues 3-8 in the other; the
+  // variant's (queue, stripe) segments are walked in order through a prefix
+  // of their lengths (clamped to the stripe capacity)
   constexpr int kQ0 = kCharged ? 3 : 0;
-  int qc[kNumQueues];
-  int total = 0;
-#pragma unroll
-  for (int k = 0; k < kNumQueues; ++k) {
-    const bool mine_q = kCharged ? k >= 3 : k < 3;
-    qc[k] = mine_q ? (int)min((int64_t)max(a.cnt->q[k], 0), cur.cap) : 0;
-    total += qc[k];
+  constexpr int kNQ = kCharged ? kNumQueues - 3 : 3;
+  __shared__ int seg[kNQ * kStripes + 1];
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    seg[0] = 0;
+    for (int k = 0; k < kNQ * kStripes; ++k) {
+      acc += (int)min((int64_t)max(a.cnt->q[kQ0 + k / kStripes][k % kStripes], 0), sc);
+      seg[k + 1] = acc;
+    }
   }
+  __syncthreads();
+  const int total = seg[kNQ * kStripes];
   const R mine = (R)P->tracking_cut;
   for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < total; base += gridDim.x * blockDim.x) {
     const int i = base + lane();
@@ -539,23 +613,27 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
     Interaction<R> o;
     o.nsec = 0;
     int fate0 = 0, fate1 = 0;  // 0 deposited, 1 pushed, 2 annihilated at rest
-    bool survive = false, prim_rest = false;
-    int outN = 0, outC = 0;
-    int live_delta = 0;
+    bool survive = false, prim_rest = false, killed = false, pair = false;
+    int outN = 0, outC = 0, nannih = 0;
+    Dep dp;  // every deposit of an interaction is at the track's position
     if (active) {
-      int q = kQ0, j = i;
-#pragma unroll
-      for (int k = kQ0; k < kNumQueues - 1; ++k)
-        if (q == k && j >= qc[k]) {
-          j -= qc[k];
-          q = k + 1;
-        }
-      const int slot = a.queues[(int64_t)q * cur.cap + j];
+      // segment of item i: binary search over the prefix (seg[0] = 0 <= i < total)
+      int lo = 0, hi = kNQ * kStripes;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (i >= seg[mid])
+          lo = mid;
+        else
+          hi = mid;
+      }
+      const int q = kQ0 + lo / kStripes;
+      const int qs = lo % kStripes;
+      const int slot = a.queues[(int64_t)q * cur.cap + (int64_t)qs * sc + (i - seg[lo])];
       load_track(cur, slot, t, trace);
       if (!kCharged) t.species = 0;
       if (kCharged && q == kQueueAtRest) {
         // positron stopped by the continuous loss or the tracking cut
-        atomicAdd(&a.ev[t.event].nannihil, 1ULL);
+        ++nannih;
         prim_rest = true;
         outN = 2;
         t.e = R(0);
@@ -565,28 +643,28 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
           sample_charged<R>(P, m, q, t.e, t.dir, t.rng, o);
         else
           sample_neutral<R>(P, m, q, t.e, t.dir, t.rng, o);
-        if (q == 1 && o.nsec == 2) atomicAdd(&a.ev[t.event].npairs, 1ULL);
-        if (q == 7) atomicAdd(&a.ev[t.event].nannihil, 1ULL);
-        deposit(P, lt, t, (double)o.edep, t.pos, t.vol);
+        pair = q == 1 && o.nsec == 2;
+        if (q == 7) ++nannih;
+        dp.add(P, t, (double)o.edep, t.pos, t.vol);
         t.e = o.ekin;
         t.dir[0] = o.d[0];
         t.dir[1] = o.d[1];
         t.dir[2] = o.d[2];
         if (trace) trace_rec(a, t, queue_proc(q), cur.tedep[slot] + (double)o.edep);
-        if (o.nsec > 0) fate0 = sec_fate(P, lt, t, o.sec[0], mine, outN, outC, a);
-        if (o.nsec > 1) fate1 = sec_fate(P, lt, t, o.sec[1], mine, outN, outC, a);
+        if (o.nsec > 0) fate0 = sec_fate(P, t, dp, o.sec[0], mine, outN, outC, nannih);
+        if (o.nsec > 1) fate1 = sec_fate(P, t, dp, o.sec[1], mine, outN, outC, nannih);
       }
       if (t.e > R(0)) {
         if (t.e < mine) {
-          deposit(P, lt, t, (double)t.e, t.pos, t.vol);
+          dp.add(P, t, (double)t.e, t.pos, t.vol);
           if (t.species == 2) {
-            atomicAdd(&a.ev[t.event].nannihil, 1ULL);
+            ++nannih;
             prim_rest = true;
             outN += 2;
           }
         } else if (t.nsteps >= P->max_steps) {
-          deposit(P, lt, t, (double)t.e, t.pos, t.vol);
-          atomicAdd(&a.cnt->killed, 1ULL);
+          dp.add(P, t, (double)t.e, t.pos, t.vol);
+          killed = true;
         } else {
           survive = true;
           if (t.species == 0)
@@ -595,15 +673,22 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
             outC++;
         }
       }
-      if (!survive) retire(a, t);
-      live_delta = outN + outC - 1;
     }
-    wave_add_u64(&a.cnt->interactions, active);
-    const int bN = wave_reserve(outN, &a.cnt->nxt[0]);
-    const int bC = wave_reserve(outC, &a.cnt->nxt[1]);
+    // returning atomics first, all in flight together; then the stores
     const int made = outN + outC - (survive ? 1 : 0);
-    int nk = wave_reserve(made, &a.cnt->nnew);
+    WaveReserve rN, rC, rK;
+    rN.issue(outN, &a.cnt->nxt[0][st]);
+    rC.issue(outC, &a.cnt->nxt[1][st]);
+    rK.issue(made, &a.cnt->nnew[st]);
+    const int bN = rN.get(), bC = rC.get();
+    int nk = rK.get();
     if (active) {
+      dp.flush(a);
+      if (pair) atomicAdd(&a.ev[t.event].npairs, 1ULL);
+      if (nannih) atomicAdd(&a.ev[t.event].nannihil, (unsigned long long)nannih);
+      if (killed) atomicAdd(&a.cnt->killed, 1ULL);
+      if (!survive) retire(a, t);
+      if (outN + outC != 1) atomicAdd(&a.ev_live[t.event], outN + outC - 1);
       // new tracks in the oracle's draw order: secondary 0, secondary 1, then
       // the primary's annihilation at rest; the survivor last
       int kN = bN, kC = bC;
@@ -611,10 +696,10 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
         const Sec<R> &s = o.sec[0];
         if (fate0 == 1) {
           const uint64_t u = child_substream(t.rng);
-          emit_new(a, nxt, s.species == 0 ? kN++ : kC++, nk, t, s.species, s.e, s.d[0], s.d[1], s.d[2], u, lt,
+          emit_new(a, nxt, s.species == 0 ? kN++ : kC++, nk, t, s.species, s.e, s.d[0], s.d[1], s.d[2], u,
                    trace);
         } else if (fate0 == 2) {
-          emit_at_rest(a, nxt, kN, nk, t, lt, trace);
+          emit_at_rest(a, nxt, kN, nk, t, trace);
           kN += 2;
         }
       }
@@ -622,33 +707,31 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
         const Sec<R> &s = o.sec[1];
         if (fate1 == 1) {
           const uint64_t u = child_substream(t.rng);
-          emit_new(a, nxt, s.species == 0 ? kN++ : kC++, nk, t, s.species, s.e, s.d[0], s.d[1], s.d[2], u, lt,
+          emit_new(a, nxt, s.species == 0 ? kN++ : kC++, nk, t, s.species, s.e, s.d[0], s.d[1], s.d[2], u,
                    trace);
         } else if (fate1 == 2) {
-          emit_at_rest(a, nxt, kN, nk, t, lt, trace);
+          emit_at_rest(a, nxt, kN, nk, t, trace);
           kN += 2;
         }
       }
       if (prim_rest) {
-        emit_at_rest(a, nxt, kN, nk, t, lt, trace);
+        emit_at_rest(a, nxt, kN, nk, t, trace);
         kN += 2;
       }
       if (survive) {
-        const int64_t s = t.species == 0 ? out_slot(0, kN, nxt.cap) : out_slot(1, kC, nxt.cap);
+        const int64_t s = t.species == 0 ? out_slot(0, st, kN, sc) : out_slot(1, st, kC, sc);
         if (s >= 0)
           store_track(nxt, s, t, trace);
         else
           atomicOr(&a.cnt->overflow, 1);
       }
     }
+    wave_add_u64(&a.cnt->interactions, active);
     {
-      const int incl = wave_incl_scan(made);
-      const int tot = __shfl(incl, 63, 64);
+      const int tot = __shfl(rK.incl, 63, 64);
       if (tot > 0 && lane() == 63) atomicAdd(&a.cnt->created, (unsigned long long)tot);
     }
-    if (live_delta != 0) atomicAdd(&a.ev_live[t.event], live_delta);
   }
-  lds_flush(P, lt, a.rows);
 }
 
 // RNG state of the tracks created this iteration: substream uid of the seed,
@@ -658,9 +741,16 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_derive(StepArgs a, PoolArgs<R> pa) {
   const Pool<R> &nxt = pa.nxt;
-  const int64_t n = min((int64_t)max(a.cnt->nnew, 0), nxt.cap);
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t s = a.newlist[i];
+  const int64_t sc = stripe_cap(nxt);
+  StripeSpan span;
+  span.pre[0] = 0;
+#pragma unroll
+  for (int k = 0; k < kStripes; ++k) span.pre[k + 1] = span.pre[k] + (int)min((int64_t)max(a.cnt->nnew[k], 0), sc);
+  const int n = span.total();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int is, off;
+    span.locate(i, is, off);
+    const int64_t s = a.newlist[(int64_t)is * sc + off];
     if (s < 0 || s >= nxt.cap) continue;
     const uint64_t uid = (uint64_t)nxt.rng[nxt.cap + s] | ((uint64_t)nxt.rng[2 * nxt.cap + s] << 32);
     Rng r;
@@ -671,20 +761,22 @@ __global__ __launch_bounds__(kBlock) void k_derive(StepArgs a, PoolArgs<R> pa) {
 
 __global__ void k_advance(Counters *c, int64_t cap) {
   if (threadIdx.x == 0) {
-    const int64_t n0 = c->nxt[0], n1 = c->nxt[1];
-    if (c->overflow || n0 < 0 || n1 < 0 || n0 + n1 > cap) {
-      // an overflowing pool is dropped whole: the host reports GXT_ECAPACITY
-      c->overflow = 1;
-      c->cur[0] = 0;
-      c->cur[1] = 0;
-    } else {
-      c->cur[0] = (int32_t)n0;
-      c->cur[1] = (int32_t)n1;
+    const int64_t sc = cap / kStripes;
+    bool bad = c->overflow != 0;
+    for (int s = 0; s < kStripes; ++s) {
+      const int64_t n0 = c->nxt[0][s], n1 = c->nxt[1][s];
+      bad = bad || n0 < 0 || n1 < 0 || n0 + n1 > sc;
     }
-    c->nxt[0] = 0;
-    c->nxt[1] = 0;
-    c->nnew = 0;
-    for (int k = 0; k < kNumQueues; ++k) c->q[k] = 0;
+    // an overflowing pool is dropped whole: the host reports GXT_ECAPACITY
+    if (bad) c->overflow = 1;
+    for (int s = 0; s < kStripes; ++s) {
+      c->cur[0][s] = bad ? 0 : c->nxt[0][s];
+      c->cur[1][s] = bad ? 0 : c->nxt[1][s];
+      c->nxt[0][s] = 0;
+      c->nxt[1][s] = 0;
+      c->nnew[s] = 0;
+      for (int k = 0; k < kNumQueues; ++k) c->q[k][s] = 0;
+    }
   }
 }
 

@@ -16,6 +16,13 @@ constexpr int kMaxRBins = 64;
 constexpr int kNumQueues = 9;     // (species, process) interaction queues + positrons at rest
 constexpr int kQueueAtRest = 8;
 constexpr int kSpecBins = 64;     // secondary spectrum bins per species
+constexpr int kInvRangeHints = 256;  // search-hint bins per inverse-range vector
+// The pool, the interaction queues and the new-track list are split into
+// kStripes stripes of cap / kStripes slots, each with its own counters;
+// workgroup b appends to stripe b % kStripes, so each counter takes 1/kStripes
+// of the atomics (the workgroups of one stripe share an XCD under the
+// round-robin dispatch of workgroups to XCDs).
+constexpr int kStripes = 8;
 
 // One physics vector inside the node arrays (x, y, d2 at [off, off+n)).
 struct DevTable {
@@ -23,6 +30,10 @@ struct DevTable {
   int n;
   double dl, base;  // log-binning: bin = log(E)/dl - base
   float dlf, basef;
+  // search hints of a non-log vector (inverse range): hints[hint + k] is the
+  // last node below x[0] 2^(k / hscale), k < nhint (hint < 0: none)
+  int hint, nhint;
+  float hlog0, hscale;  // log2 x[0]; hint bins per octave
 };
 
 struct DevElement {
@@ -82,6 +93,7 @@ struct DevParams {
   const float *sbf;             // FP32 mirror
   const double *sandia;         // Sandia rows (edge keV, a1..a4)
   const double *shells;         // binding energies (MeV)
+  const int *hints;             // search hints of the inverse-range vectors
 };
 
 // Structure-of-arrays track pool.  R is the precision of the physics state
@@ -103,15 +115,14 @@ struct Pool {
   double *tedep;      // step deposit carried to the interaction kernel (tracing only)
 };
 
-// Device counters of the step loop.
+// Device counters of the step loop, one per stripe.
 struct Counters {
-  int32_t cur[2];          // live neutral / charged tracks in the current pool
-  int32_t nxt[2];          // written to the next pool
-  int32_t q[kNumQueues];   // interaction queue lengths
+  int32_t cur[2][kStripes];          // live neutral / charged tracks in the current pool
+  int32_t nxt[2][kStripes];          // written to the next pool
+  int32_t q[kNumQueues][kStripes];   // interaction queue lengths
+  int32_t nnew[kStripes];            // tracks created this iteration (new-track list entries)
   int32_t overflow;        // a pool or queue write would have gone out of range
-  int32_t prim_head;       // primaries consumed from the primary queue
-  int32_t prim_tail;       // primaries available in the primary queue
-  int32_t nnew;            // tracks created this iteration (entries of the new-track list)
+  int32_t pad[3];
   unsigned long long steps;   // track-steps
   unsigned long long steps_charged;  // of which e-/e+ steps
   unsigned long long killed;  // tracks killed by the step limit

@@ -4,8 +4,8 @@
 // baseline.  It restates, function by function, the reference's algorithms
 // (GeantExascalePilot; file:line citations below are relative to
 // source/Geant/) under the physics contract shared with the GPU path.  Each
-// deviation from the reference is a documented fix of a reference quirk and
-// is tagged Qn in the comments.
+// deviation from the reference is a fix of a hard-coded value or defect of
+// the reference, stated in the comment of the function that makes it.
 //
 // Parity status of each part (how it is pinned) is listed in the repository's
 // design notes; the spline second-derivative routine and the samplers' exact

@@ -233,3 +233,42 @@ def test_trace_first_50_steps(gpu_lib):
     # steps, so they inherit the libm-level drift of those (untraced) steps:
     # same sequences, kinematics within 1e-6
     np.testing.assert_allclose(g[:, 4:], c[:, 4:], rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("precision", [abi.FP32, abi.FP64])
+def test_serving_mix_equals_alone(gpu_lib, precision):
+    """Continuous batching: a job's per-event results are the same run alone
+    and admitted by the serving loop into one pool shared with jobs that
+    arrive before and after it.  Counts are exact; FP64 tallies within 1e-9
+    (the order of the FP64 atomic additions differs)."""
+    _flush("serving mix vs alone, precision %d" % precision)
+    import time
+    g = gpu_lib
+    cfg = abi.config_pbwo4_block(max_events=400, precision=precision, max_jobs=16, pool_capacity=400_000)
+    a = g.make_primaries(777, abi.ELECTRON, 2000.0, 2000.0, 0.0, 20, first=1000, n_total=1 << 40)
+    with g.Transport(cfg) as t:
+        t.submit(a, 7)
+        t.run()
+        alone = t.event_tallies(7, 20)
+    gamma = g.make_primaries(777, abi.GAMMA, 1000.0, 1000.0, 0.0, 1, first=5000, n_total=1 << 40)
+    e10 = g.make_primaries(777, abi.ELECTRON, 10000.0, 10000.0, 0.0, 10, first=6000, n_total=1 << 40)
+    e50 = g.make_primaries(777, abi.ELECTRON, 50000.0, 50000.0, 0.0, 30, first=7000, n_total=1 << 40)
+    with g.Transport(cfg) as t:
+        t.serve_start()
+        t.submit(e50, 1)
+        t.submit(gamma, 2)
+        time.sleep(0.003)
+        t.submit(a, 7)
+        t.submit(e10, 3)
+        done = []
+        t0 = time.time()
+        while len(done) < 4 and time.time() - t0 < 60:
+            done += t.poll()
+            time.sleep(0.001)
+        t.serve_stop()
+        assert sorted(r.job_id for r in done) == [1, 2, 3, 7]
+        mixed = t.event_tallies(7, 20)
+    for k in ("n_steps", "n_secondaries", "n_pairs", "n_annihil"):
+        assert np.array_equal(alone[k], mixed[k]), k
+    for k in ("edep", "visible", "leak", "charged_length"):
+        np.testing.assert_allclose(mixed[k], alone[k], rtol=1e-9, atol=1e-9)
