@@ -148,7 +148,7 @@ def cpu_baseline(abi, target_s=15.0):
         return dt, o.stats().track_steps
 
     dt, _ = timed(threads)
-    n = int(max(threads, min(4000, threads * target_s / max(dt, 1e-3))))
+    n = int(max(threads, min(10000, threads * target_s / max(dt, 1e-3))))
     dt, steps = timed(n)
     log("cpu oracle: %d events on %d threads in %.2f s" % (n, threads, dt))
     return dict(value=n / dt, unit="events/s", cores=threads, kind="port",
