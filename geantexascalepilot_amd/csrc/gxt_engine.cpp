@@ -100,9 +100,13 @@ int upload(HostData &hd, DeviceData &dd) {
   HIP_TRY(put(&dd.hints, hd.hints));
   DevParams &p = hd.params;
   p.hints = dd.hints;
+  uint32_t jump[2][64][9];
+  build_jump_matrices(jump);
   std::vector<uint32_t> j8;
-  build_jump_tables(p.jump, j8);
+  build_jump_tables(jump, j8);
   HIP_TRY(put(&dd.jump8, j8));
+  p.nsandia = (int)hd.sandia.size();
+  p.nshells = (int)hd.shells.size();
   p.jump8 = dd.jump8;
   p.xd = dd.xd;
   p.yd = dd.yd;
@@ -570,9 +574,10 @@ int rng_params(const DevParams **out) {
   if (!g_rng_dd.P) {
     DevParams p;
     std::memset(&p, 0, sizeof(p));
-    build_jump_matrices(p.jump);
+    uint32_t jump[2][64][9];
+    build_jump_matrices(jump);
     std::vector<uint32_t> j8;
-    build_jump_tables(p.jump, j8);
+    build_jump_tables(jump, j8);
     HIP_TRY(dalloc(&g_rng_dd.jump8, j8.size()));
     HIP_TRY(hipMemcpy(g_rng_dd.jump8, j8.data(), j8.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     p.jump8 = g_rng_dd.jump8;
@@ -662,7 +667,6 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
   P.trace_events = cfg->trace_tracks > 0 ? cfg->trace_tracks : 0;
   P.trace_steps = cfg->trace_steps;
   for (int i = 0; i < 6; ++i) P.seed[i] = cfg->seed;
-  build_jump_matrices(P.jump);
   rc = upload(c->hd, c->dd);
   if (rc) return rc;
   // pools, queues, counters, tallies

@@ -85,7 +85,7 @@ struct DevParams {
   int trace_steps;
   int pad;
   uint32_t seed[6];
-  uint32_t jump[2][64][9];  // A_c^(2^(76+j)) mod m_c
+  int nsandia, nshells;     // doubles in the Sandia and shell arrays
   const uint32_t *jump8;    // A_c^(d 2^(76+8k)) mod m_c at ((c * 8 + k) * 256 + d) * 9
   const double *xd, *yd, *d2d;  // node arrays, FP64
   const float *xf, *yf, *d2f;   // FP32 mirrors
