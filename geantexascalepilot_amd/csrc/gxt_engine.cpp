@@ -189,6 +189,7 @@ struct gxt_ctx {
   std::string err;
   int device = 0;
   hipStream_t stream = nullptr;
+  StepStreams ss{};  // main = stream, side stream and fork/join events
   HostData hd;
   DeviceData dd;
   int precision = GXT_FP64;
@@ -381,10 +382,10 @@ int iterate(gxt_ctx *c, int k) {
     }
     if (c->precision == GXT_FP32) {
       PoolArgs<float> pa{c->pf[c->parity].p, c->pf[1 - c->parity].p};
-      e = launch_step<float>(a, pa, c->grid, c->stream, evs);
+      e = launch_step<float>(a, pa, c->grid, c->ss, evs);
     } else {
       PoolArgs<double> pa{c->pd[c->parity].p, c->pd[1 - c->parity].p};
-      e = launch_step<double>(a, pa, c->grid, c->stream, evs);
+      e = launch_step<double>(a, pa, c->grid, c->ss, evs);
     }
     if (e != hipSuccess) return c->fail(GXT_EHIP, std::string("step: ") + hipGetErrorString(e));
     c->parity = 1 - c->parity;
@@ -403,11 +404,11 @@ int collect(gxt_ctx *c, bool &idle) {
     float t[kStepEvents - 1] = {};
     for (int k = 0; k + 1 < kStepEvents; ++k) HIP_TRY(hipEventElapsedTime(&t[k], q[k], q[k + 1]));
     c->kst.transport_neutral_ms += t[0];
-    c->kst.transport_charged_ms += t[1];
-    c->kst.interact_neutral_ms += t[2];
-    c->kst.interact_charged_ms += t[3];
-    c->kst.derive_ms += t[4];
-    c->kst.advance_ms += t[5];
+    c->kst.interact_neutral_ms += t[1];
+    c->kst.transport_charged_ms += t[3];
+    c->kst.interact_charged_ms += t[4];
+    c->kst.derive_ms += t[6];
+    c->kst.advance_ms += t[7];
     c->kst.launches++;
     c->ev_free.push_back(q);
   }
@@ -545,7 +546,10 @@ void destroy_ctx(gxt_ctx *c) {
   (void)hipFree(c->spec);
   (void)hipFree(c->trace_buf);
   (void)hipFree(c->stage);
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->ss.fork) (void)hipEventDestroy(c->ss.fork);
+  if (c->ss.join) (void)hipEventDestroy(c->ss.join);
+  if (c->ss.side) (void)hipStreamDestroy(c->ss.side);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -640,6 +644,10 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
   c->t0 = std::chrono::steady_clock::now();
   HIP_TRY(hipGetDevice(&c->device));
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&c->ss.side, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&c->ss.fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&c->ss.join, hipEventDisableTiming));
+  c->ss.main = c->stream;
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
   c->grid = std::max(cus, 1) * 4;

@@ -51,13 +51,22 @@ struct SampleArgs {
   double *sec_dir;
 };
 
+// Streams of one step-loop iteration: main carries the photon chain, the
+// RNG derivation and the counter advance; side the e-/e+ chain, forked from
+// and joined back into main with the two events.
+struct StepStreams {
+  hipStream_t main, side;
+  hipEvent_t fork, join;
+};
+
 // One step-loop iteration.  With ev (kStepEvents events) every kernel is
-// bracketed by events: photon transport, e-/e+ transport, photon
-// interactions, e-/e+ interactions, RNG derivation of the new tracks,
-// counter advance.
-constexpr int kStepEvents = 7;
+// bracketed by events on its stream: ev[0..2] photon transport and
+// interactions (main), ev[3..5] e-/e+ transport and interactions (side),
+// ev[6..8] RNG derivation and counter advance (main).
+constexpr int kStepEvents = 9;
 template <class R>
-hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, hipStream_t st, hipEvent_t *ev = nullptr);
+hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const StepStreams &s,
+                       hipEvent_t *ev = nullptr);
 template <class R>
 hipError_t launch_inject(const StepArgs &a, const PoolArgs<R> &pa, const gxt_primary *prim, int n, hipStream_t st);
 

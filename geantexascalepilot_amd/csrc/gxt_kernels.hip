@@ -866,21 +866,33 @@ __global__ void k_helix(double b0, double b1, double b2, const double *charge, c
 static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
 
 template <class R>
-hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, hipStream_t st, hipEvent_t *ev) {
-  // ev (optional): 4 events recorded before/after the transport and interaction kernels
-  if (ev) (void)hipEventRecord(ev[0], st);
-  hipLaunchKernelGGL((k_transport<R, false>), dim3(grid), dim3(kBlock), 0, st, a, pa);
-  if (ev) (void)hipEventRecord(ev[1], st);
-  hipLaunchKernelGGL((k_transport<R, true>), dim3(grid), dim3(kBlock), 0, st, a, pa);
-  if (ev) (void)hipEventRecord(ev[2], st);
-  hipLaunchKernelGGL((k_interact<R, false>), dim3(grid), dim3(kBlock), 0, st, a, pa);
-  if (ev) (void)hipEventRecord(ev[3], st);
-  hipLaunchKernelGGL((k_interact<R, true>), dim3(grid), dim3(kBlock), 0, st, a, pa);
-  if (ev) (void)hipEventRecord(ev[4], st);
-  hipLaunchKernelGGL(k_derive<R>, dim3(2 * grid), dim3(kBlock), 0, st, a, pa);
-  if (ev) (void)hipEventRecord(ev[5], st);
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, a.cnt, pa.cur.cap);
-  if (ev) (void)hipEventRecord(ev[6], st);
+hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const StepStreams &s, hipEvent_t *ev) {
+  // The photon chain (transport, then interactions) runs on s.main and the
+  // e-/e+ chain on s.side: they read disjoint regions and queues and write
+  // the next pool, queues and tallies only through atomics, so they overlap
+  // and each fills the other's tail.  RNG derivation and the counter advance
+  // follow the join.
+  hipError_t e = hipEventRecord(s.fork, s.main);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s.side, s.fork, 0);
+  if (e != hipSuccess) return e;
+  if (ev) (void)hipEventRecord(ev[0], s.main);
+  hipLaunchKernelGGL((k_transport<R, false>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
+  if (ev) (void)hipEventRecord(ev[1], s.main);
+  hipLaunchKernelGGL((k_interact<R, false>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
+  if (ev) (void)hipEventRecord(ev[2], s.main);
+  if (ev) (void)hipEventRecord(ev[3], s.side);
+  hipLaunchKernelGGL((k_transport<R, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
+  if (ev) (void)hipEventRecord(ev[4], s.side);
+  hipLaunchKernelGGL((k_interact<R, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
+  if (ev) (void)hipEventRecord(ev[5], s.side);
+  e = hipEventRecord(s.join, s.side);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s.main, s.join, 0);
+  if (e != hipSuccess) return e;
+  if (ev) (void)hipEventRecord(ev[6], s.main);
+  hipLaunchKernelGGL(k_derive<R>, dim3(2 * grid), dim3(kBlock), 0, s.main, a, pa);
+  if (ev) (void)hipEventRecord(ev[7], s.main);
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s.main, a.cnt, pa.cur.cap);
+  if (ev) (void)hipEventRecord(ev[8], s.main);
   return hipGetLastError();
 }
 
@@ -893,8 +905,10 @@ hipError_t launch_inject(const StepArgs &a, const PoolArgs<R> &pa, const gxt_pri
   return hipGetLastError();
 }
 
-template hipError_t launch_step<double>(const StepArgs &, const PoolArgs<double> &, int, hipStream_t, hipEvent_t *);
-template hipError_t launch_step<float>(const StepArgs &, const PoolArgs<float> &, int, hipStream_t, hipEvent_t *);
+template hipError_t launch_step<double>(const StepArgs &, const PoolArgs<double> &, int, const StepStreams &,
+                                        hipEvent_t *);
+template hipError_t launch_step<float>(const StepArgs &, const PoolArgs<float> &, int, const StepStreams &,
+                                       hipEvent_t *);
 template hipError_t launch_inject<double>(const StepArgs &, const PoolArgs<double> &, const gxt_primary *, int,
                                           hipStream_t);
 template hipError_t launch_inject<float>(const StepArgs &, const PoolArgs<float> &, const gxt_primary *, int,
