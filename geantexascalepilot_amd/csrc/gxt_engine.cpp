@@ -405,10 +405,10 @@ int collect(gxt_ctx *c, bool &idle) {
     for (int k = 0; k + 1 < kStepEvents; ++k) HIP_TRY(hipEventElapsedTime(&t[k], q[k], q[k + 1]));
     c->kst.transport_neutral_ms += t[0];
     c->kst.interact_neutral_ms += t[1];
-    c->kst.transport_charged_ms += t[3];
-    c->kst.interact_charged_ms += t[4];
-    c->kst.derive_ms += t[6];
-    c->kst.advance_ms += t[7];
+    c->kst.transport_charged_ms += t[4];
+    c->kst.interact_charged_ms += t[5];
+    c->kst.derive_ms += t[2] + t[6];
+    c->kst.advance_ms += t[8];
     c->kst.launches++;
     c->ev_free.push_back(q);
   }
@@ -683,7 +683,7 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
     if (rc) return rc;
   }
   HIP_TRY(dalloc(&c->queues, (size_t)kNumQueues * c->cap));
-  HIP_TRY(dalloc(&c->newlist, (size_t)c->cap));
+  HIP_TRY(dalloc(&c->newlist, (size_t)2 * c->cap));
   HIP_TRY(dalloc(&c->cnt, 1));
   HIP_TRY(hipMemset(c->cnt, 0, sizeof(Counters)));
   HIP_TRY(hipHostMalloc((void **)&c->hcnt, sizeof(Counters), hipHostMallocDefault));

@@ -25,7 +25,8 @@ struct StepArgs {
   DevEvent *ev;
   int *ev_live;      // live tracks per event slot
   int32_t *queues;  // kNumQueues x cap
-  int32_t *newlist; // next-pool slots of the tracks created this iteration (cap)
+  int32_t *newlist; // next-pool slots of the tracks created this iteration: 2 lists of cap
+                    // (photon interactions, e-/e+ interactions)
   TallyRows rows;
   double *trace;
   int64_t trace_cap;
@@ -60,10 +61,10 @@ struct StepStreams {
 };
 
 // One step-loop iteration.  With ev (kStepEvents events) every kernel is
-// bracketed by events on its stream: ev[0..2] photon transport and
-// interactions (main), ev[3..5] e-/e+ transport and interactions (side),
-// ev[6..8] RNG derivation and counter advance (main).
-constexpr int kStepEvents = 9;
+// bracketed by events on its stream: ev[0..3] photon transport,
+// interactions and RNG derivation of their new tracks (main), ev[4..7] the
+// same for e-/e+ (side), ev[8..9] the counter advance (main).
+constexpr int kStepEvents = 10;
 template <class R>
 hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const StepStreams &s,
                        hipEvent_t *ev = nullptr);

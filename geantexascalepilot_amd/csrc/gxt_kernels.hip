@@ -606,6 +606,8 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
   __syncthreads();
   const int total = seg[kNQ * kStripes];
   const R mine = (R)P->tracking_cut;
+  // this variant's new-track list
+  a.newlist += kCharged ? cur.cap : 0;
   for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < total; base += gridDim.x * blockDim.x) {
     const int i = base + lane();
     const bool active = i < total;
@@ -679,7 +681,7 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
     WaveReserve rN, rC, rK;
     rN.issue(outN, &a.cnt->nxt[0][st]);
     rC.issue(outC, &a.cnt->nxt[1][st]);
-    rK.issue(made, &a.cnt->nnew[st]);
+    rK.issue(made, &a.cnt->nnew[kCharged ? 1 : 0][st]);
     const int bN = rN.get(), bC = rC.get();
     int nk = rK.get();
     if (active) {
@@ -738,14 +740,16 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
 // one track per lane (the jump is 16 table-driven matrix-vector products, so
 // it runs here, without divergence and at full occupancy, rather than in the
 // interaction kernels).
-template <class R>
+template <class R, int kList>
 __global__ __launch_bounds__(kBlock) void k_derive(StepArgs a, PoolArgs<R> pa) {
   const Pool<R> &nxt = pa.nxt;
   const int64_t sc = stripe_cap(nxt);
+  a.newlist += kList * nxt.cap;
   StripeSpan span;
   span.pre[0] = 0;
 #pragma unroll
-  for (int k = 0; k < kStripes; ++k) span.pre[k + 1] = span.pre[k] + (int)min((int64_t)max(a.cnt->nnew[k], 0), sc);
+  for (int k = 0; k < kStripes; ++k)
+    span.pre[k + 1] = span.pre[k] + (int)min((int64_t)max(a.cnt->nnew[kList][k], 0), sc);
   const int n = span.total();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     int is, off;
@@ -774,7 +778,8 @@ __global__ void k_advance(Counters *c, int64_t cap) {
       c->cur[1][s] = bad ? 0 : c->nxt[1][s];
       c->nxt[0][s] = 0;
       c->nxt[1][s] = 0;
-      c->nnew[s] = 0;
+      c->nnew[0][s] = 0;
+      c->nnew[1][s] = 0;
       for (int k = 0; k < kNumQueues; ++k) c->q[k][s] = 0;
     }
   }
@@ -880,19 +885,21 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
   if (ev) (void)hipEventRecord(ev[1], s.main);
   hipLaunchKernelGGL((k_interact<R, false>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
   if (ev) (void)hipEventRecord(ev[2], s.main);
-  if (ev) (void)hipEventRecord(ev[3], s.side);
-  hipLaunchKernelGGL((k_transport<R, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
+  hipLaunchKernelGGL((k_derive<R, 0>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
+  if (ev) (void)hipEventRecord(ev[3], s.main);
   if (ev) (void)hipEventRecord(ev[4], s.side);
-  hipLaunchKernelGGL((k_interact<R, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
+  hipLaunchKernelGGL((k_transport<R, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
   if (ev) (void)hipEventRecord(ev[5], s.side);
+  hipLaunchKernelGGL((k_interact<R, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
+  if (ev) (void)hipEventRecord(ev[6], s.side);
+  hipLaunchKernelGGL((k_derive<R, 1>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
+  if (ev) (void)hipEventRecord(ev[7], s.side);
   e = hipEventRecord(s.join, s.side);
   if (e == hipSuccess) e = hipStreamWaitEvent(s.main, s.join, 0);
   if (e != hipSuccess) return e;
-  if (ev) (void)hipEventRecord(ev[6], s.main);
-  hipLaunchKernelGGL(k_derive<R>, dim3(2 * grid), dim3(kBlock), 0, s.main, a, pa);
-  if (ev) (void)hipEventRecord(ev[7], s.main);
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s.main, a.cnt, pa.cur.cap);
   if (ev) (void)hipEventRecord(ev[8], s.main);
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s.main, a.cnt, pa.cur.cap);
+  if (ev) (void)hipEventRecord(ev[9], s.main);
   return hipGetLastError();
 }
 

@@ -120,7 +120,8 @@ struct Counters {
   int32_t cur[2][kStripes];          // live neutral / charged tracks in the current pool
   int32_t nxt[2][kStripes];          // written to the next pool
   int32_t q[kNumQueues][kStripes];   // interaction queue lengths
-  int32_t nnew[kStripes];            // tracks created this iteration (new-track list entries)
+  int32_t nnew[2][kStripes];         // tracks created this iteration by the photon / e-/e+
+                                     // interactions (entries of the two new-track lists)
   int32_t overflow;        // a pool or queue write would have gone out of range
   int32_t pad[3];
   unsigned long long steps;   // track-steps
