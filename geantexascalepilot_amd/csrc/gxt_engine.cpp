@@ -169,7 +169,7 @@ struct Job {
   std::vector<gxt_primary> prims;   // sorted by event
   std::vector<int> ev_first;        // prims[ev_first[e] .. ev_first[e+1]) belong to event e
   std::vector<double> ev_gev;       // summed primary energy of each event (GeV)
-  std::vector<double> ev_fp;        // footprint committed for an event in flight (tracks)
+  std::vector<int> ev_max;          // highest live-track count seen for an event in flight
   std::vector<char> ev_state;       // 0 pending, 1 in flight, 2 finished
   int n_events = 0;
   int next_event = 0;               // first event not yet injected
@@ -220,9 +220,10 @@ struct gxt_ctx {
   std::map<uint64_t, std::unique_ptr<Job>> finished;
   std::vector<char> ev_used;
   double committed = 0;             // summed footprint of events in flight
-  double tracks_per_gev = 3000.0;   // admission estimate of peak live tracks per GeV (adapted)
-  double peak_ratio = 0;            // highest observed live tracks per GeV in flight
-  int64_t syncs_observed = 0;
+  double tracks_per_gev = 300.0;    // estimated peak live tracks per GeV of an event (learned)
+  double seen_ratio = 0;            // highest peak per GeV of the events past their maximum
+  int64_t seen_events = 0;          // events past their maximum seen so far
+  std::vector<char> ev_counted;     // event slot already counted in seen_events
   // kernel timing (profile_kernels)
   bool prof = false;
   std::vector<std::array<hipEvent_t, kStepEvents>> ev_free, ev_pending;
@@ -270,8 +271,23 @@ int64_t live_tracks(const Counters &h) {
 }
 
 // Estimated peak live tracks of an event, from its primaries' energy.
-double event_footprint(const gxt_ctx *c, const Job *j, int e) {
+// Peak live tracks expected of event e: 16 per primary plus the learned
+// number per GeV.
+double event_peak(const gxt_ctx *c, const Job *j, int e) {
   return 16.0 * (j->ev_first[e + 1] - j->ev_first[e]) + c->tracks_per_gev * j->ev_gev[e];
+}
+
+// An event in flight whose live count has fallen below half of the highest
+// seen, once that reached 64 tracks, is past its shower maximum.
+bool past_peak(int live, int mx) { return mx >= 64 && 2 * live < mx; }
+
+// Live tracks event e may still hold at once: past its maximum, a margin
+// over its current count; before it, its expected peak (or more, if it
+// already exceeds it).
+double event_footprint(const gxt_ctx *c, const Job *j, int e, int live) {
+  const double peak = event_peak(c, j, e);
+  if (past_peak(live, j->ev_max[e])) return std::min(peak, 1.25 * live + 16.0);
+  return std::max(peak, 1.25 * live);
 }
 
 int alloc_events(gxt_ctx *c, int n) {
@@ -309,12 +325,13 @@ int admit(gxt_ctx *c) {
         j->started = true;
         j->admit_ms = now_ms(c->t0);
         zero_ranges.emplace_back(eb, j->n_events);
+        std::fill(c->ev_counted.begin() + eb, c->ev_counted.begin() + eb + j->n_events, 0);
       }
       ++active;
       bool full = false;
       while (j->next_event < j->n_events) {
         const int e = j->next_event;
-        const double fp = event_footprint(c, j, e);
+        const double fp = event_peak(c, j, e);
         if (c->committed > 0 && c->committed + fp > budget) {
           full = true;
           break;
@@ -325,7 +342,6 @@ int admit(gxt_ctx *c) {
           buf.push_back(p);
         }
         j->ev_state[e] = 1;
-        j->ev_fp[e] = fp;
         c->committed += fp;
         j->next_event++;
       }
@@ -418,30 +434,46 @@ int collect(gxt_ctx *c, bool &idle) {
   const int64_t live = live_tracks(h);
   c->max_live = std::max<int64_t>(c->max_live, live);
   std::vector<Job *> fin;
-  double gev_in_flight = 0;
   {
     std::lock_guard<std::mutex> lk(c->mu);
+    // learn the peak per GeV from the events past their maximum or finished:
+    // once 32 have been seen, the estimate is their highest ratio plus 30 %
+    // (for peaks that fall between synchronisations); from then on it can
+    // only grow
     for (auto &jp : c->jobs) {
       Job *j = jp.get();
       if (!j->started) break;
       for (int e = 0; e < j->next_event; ++e) {
         if (j->ev_state[e] != 1) continue;
-        if (c->hev_live[j->ev_base + e] == 0) {
+        const int lv = c->hev_live[j->ev_base + e];
+        j->ev_max[e] = std::max(j->ev_max[e], lv);
+        const int mx = j->ev_max[e];
+        char &counted = c->ev_counted[j->ev_base + e];
+        if (!counted && (lv == 0 || past_peak(lv, mx)) && j->ev_gev[e] > 0) {
+          counted = 1;
+          c->seen_events++;
+          c->seen_ratio =
+              std::max(c->seen_ratio, (mx - 16.0 * (j->ev_first[e + 1] - j->ev_first[e])) / j->ev_gev[e]);
+        }
+      }
+    }
+    if (c->seen_events >= 32) c->tracks_per_gev = std::max(1.3 * c->seen_ratio, 1.0);
+    c->committed = 0;
+    for (auto &jp : c->jobs) {
+      Job *j = jp.get();
+      if (!j->started) break;
+      for (int e = 0; e < j->next_event; ++e) {
+        if (j->ev_state[e] != 1) continue;
+        const int lv = c->hev_live[j->ev_base + e];
+        if (lv == 0) {
           j->ev_state[e] = 2;
           j->n_done++;
-          c->committed -= j->ev_fp[e];
         } else {
-          for (int k = j->ev_first[e]; k < j->ev_first[e + 1]; ++k) gev_in_flight += j->prims[k].ekin * 1e-3;
+          c->committed += event_footprint(c, j, e, lv);
         }
       }
       if (j->n_done == j->n_events) fin.push_back(j);
     }
-    if (c->committed < 1e-6) c->committed = 0;
-  }
-  // adapt the admission estimate to the observed peak live tracks per GeV
-  if (gev_in_flight > 0) {
-    c->peak_ratio = std::max(c->peak_ratio, (double)live / gev_in_flight);
-    if (++c->syncs_observed >= 16) c->tracks_per_gev = std::min(3000.0, std::max(100.0, 2.0 * c->peak_ratio));
   }
   for (Job *j : fin) {
     std::vector<DevEvent> evs(j->n_events);
@@ -704,6 +736,7 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
     HIP_TRY(dalloc(&c->trace_buf, (size_t)12 * c->trace_cap));
   }
   c->ev_used.assign(cfg->max_events, 0);
+  c->ev_counted.assign(cfg->max_events, 0);
   HIP_TRY(hipDeviceSynchronize());
   *out = c.release();
   return GXT_OK;
@@ -737,7 +770,7 @@ int gxt_submit(gxt_ctx *c, const gxt_primary *prim, int n, uint64_t job_id) {
   for (const gxt_primary &p : j->prims) j->ev_first[p.event + 1]++;
   for (int e = 0; e < j->n_events; ++e) j->ev_first[e + 1] += j->ev_first[e];
   j->ev_gev.assign(j->n_events, 0.0);
-  j->ev_fp.assign(j->n_events, 0.0);
+  j->ev_max.assign(j->n_events, 0);
   for (const gxt_primary &p : j->prims) j->ev_gev[p.event] += p.ekin * 1.0e-3;
   j->ev_state.assign(j->n_events, 0);
   j->submit_ms = now_ms(c->t0);
