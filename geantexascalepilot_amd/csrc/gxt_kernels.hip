@@ -141,10 +141,48 @@ __device__ __forceinline__ double *tally_row(const TallyRows &r) {
   return r.bins + (int64_t)(blockIdx.x % r.nrows) * r.stride;
 }
 
-__device__ __forceinline__ void spectrum(const TallyRows &r, int species, double e) {
+// Histograms of one wave in LDS.  Deposits of a wave's lanes often share a
+// bin (a homogeneous block has one volume; the shower core fills the first
+// radial bins), which serialises global atomics in L2; LDS atomics resolve
+// those conflicts, and at the end of the kernel the wave adds its non-zero
+// bins to the workgroup's row with no-return atomics.  Wave-private: no
+// barrier.
+struct WaveHist {
+  double z[kMaxZBins];
+  double r[kMaxRBins];
+  double v[kMaxVol];
+  unsigned int spec[3 * kSpecBins];
+};
+
+__device__ __forceinline__ void wh_clear(WaveHist &h) {
+  const int l = lane();
+  for (int i = l; i < kMaxZBins; i += 64) h.z[i] = 0.0;
+  for (int i = l; i < kMaxRBins; i += 64) h.r[i] = 0.0;
+  for (int i = l; i < kMaxVol; i += 64) h.v[i] = 0.0;
+  for (int i = l; i < 3 * kSpecBins; i += 64) h.spec[i] = 0u;
+}
+
+__device__ __forceinline__ void wh_flush(const StepArgs &a, WaveHist &h) {
+  // the wave's LDS atomics are complete before it reads the bins back:
+  // s_waitcnt lgkmcnt(0) (vmcnt and expcnt left at their maxima)
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  const int l = lane();
+  const int nz = a.P->nz, nr = a.P->nr, nv = a.P->geo.nvol;
+  double *row = tally_row(a.rows);
+  for (int i = l; i < nz; i += 64)
+    if (h.z[i] != 0.0) unsafeAtomicAdd(&row[i], h.z[i]);
+  for (int i = l; i < nr; i += 64)
+    if (h.r[i] != 0.0) unsafeAtomicAdd(&row[nz + i], h.r[i]);
+  for (int i = l; i < nv; i += 64)
+    if (h.v[i] != 0.0) unsafeAtomicAdd(&row[nz + nr + i], h.v[i]);
+  unsigned long long *srow = a.rows.spec + (int64_t)(blockIdx.x % a.rows.nrows) * 3 * kSpecBins;
+  for (int i = l; i < 3 * kSpecBins; i += 64)
+    if (h.spec[i]) atomicAdd(&srow[i], (unsigned long long)h.spec[i]);
+}
+
+__device__ __forceinline__ void spectrum(WaveHist &h, int species, double e) {
   const int b = (int)floor((m_log10(e) + 3.0) * 8.0);
-  if (b >= 0 && b < kSpecBins)
-    atomicAdd(&r.spec[(int64_t)(blockIdx.x % r.nrows) * 3 * kSpecBins + species * kSpecBins + b], 1ULL);
+  if (b >= 0 && b < kSpecBins) atomicAdd(&h.spec[species * kSpecBins + b], 1u);
 }
 
 // ------------------------------------------------------------------ pool I/O
@@ -277,13 +315,11 @@ struct Dep {
     }
     e += de;
   }
-  __device__ __forceinline__ void flush(const StepArgs &a) const {
+  __device__ __forceinline__ void flush(WaveHist &h) const {
     if (e <= 0.0) return;
-    double *row = tally_row(a.rows);
-    const int nz = a.P->nz, nr = a.P->nr;
-    unsafeAtomicAdd(&row[nz + nr + vol], e);
-    unsafeAtomicAdd(&row[iz], e);
-    unsafeAtomicAdd(&row[nz + ir], e);
+    unsafeAtomicAdd(&h.v[vol], e);
+    unsafeAtomicAdd(&h.z[iz], e);
+    unsafeAtomicAdd(&h.r[ir], e);
   }
 };
 
@@ -326,7 +362,7 @@ __device__ __forceinline__ void trace_rec(const StepArgs &a, const TrackRegs<R> 
 template <class R>
 __device__ __forceinline__ void emit_new(const StepArgs &a, const Pool<R> &nxt, int idx, int &nk,
                                          const TrackRegs<R> &par, int species, R e, R d0, R d1, R d2, uint64_t uid,
-                                         bool trace) {
+                                         WaveHist &wh, bool trace) {
   const int64_t sc = stripe_cap(nxt);
   const int st = my_stripe();
   const int64_t s = out_slot(species == 0 ? 0 : 1, st, idx, sc);
@@ -338,22 +374,22 @@ __device__ __forceinline__ void emit_new(const StepArgs &a, const Pool<R> &nxt, 
   }
   ++nk;
   atomicAdd(&a.ev[par.event].nsec[species], 1ULL);
-  spectrum(a.rows, species, (double)e);
+  spectrum(wh, species, (double)e);
 }
 
 // Positron at rest: two back-to-back 511 keV photons, isotropic (2 direction
 // draws, then 2 substream draws per photon), at neutral slots idx, idx+1.
 template <class R>
 __device__ __forceinline__ void emit_at_rest(const StepArgs &a, const Pool<R> &nxt, int idx, int &nk,
-                                             TrackRegs<R> &t, bool trace) {
+                                             TrackRegs<R> &t, WaveHist &wh, bool trace) {
   const double cost = 2.0 * t.rng.uniform() - 1.0;
   const double sint = sqrt((1.0 - cost) * (1.0 + cost));
   const double phi = kTwoPi * t.rng.uniform();
   const R d0 = (R)(sint * cos(phi)), d1 = (R)(sint * sin(phi)), d2 = (R)cost;
   const uint64_t u1 = child_substream(t.rng);
-  emit_new(a, nxt, idx, nk, t, 0, (R)kMe, d0, d1, d2, u1, trace);
+  emit_new(a, nxt, idx, nk, t, 0, (R)kMe, d0, d1, d2, u1, wh, trace);
   const uint64_t u2 = child_substream(t.rng);
-  emit_new(a, nxt, idx + 1, nk, t, 0, (R)kMe, -d0, -d1, -d2, u2, trace);
+  emit_new(a, nxt, idx + 1, nk, t, 0, (R)kMe, -d0, -d1, -d2, u2, wh, trace);
 }
 
 // PostStep GPIL of process p (ProxyEmProcess.hpp:122-151): resample the
@@ -421,6 +457,9 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
 // class is a compile-time constant so each variant carries only its own code.
 template <class R, bool kCharged>
 __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a, PoolArgs<R> pa) {
+  __shared__ WaveHist whs[kBlock / 64];
+  WaveHist &wh = whs[threadIdx.x >> 6];
+  wh_clear(wh);
   const DevParams *P = a.P;
   const Pool<R> &cur = pa.cur;
   const Pool<R> &nxt = pa.nxt;
@@ -530,8 +569,8 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
     const int bs = rs.get();
     // then stores and no-return atomics
     if (active) {
-      dA.flush(a);
-      dB.flush(a);
+      dA.flush(wh);
+      dB.flush(wh);
       if (escaped) unsafeAtomicAdd(&a.ev[t.event].leak, (double)t.e);
       if (killed) atomicAdd(&a.cnt->killed, 1ULL);
       if (dies) {
@@ -558,6 +597,7 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
     wave_add_u64(&a.cnt->steps, active);
     if (kCharged) wave_add_u64(&a.cnt->steps_charged, active);
   }
+  wh_flush(a, wh);
 }
 
 // Fate of a secondary: pushed (>= tracking cut), or deposited (and, for a
@@ -583,6 +623,9 @@ __device__ __forceinline__ int sec_fate(const DevParams *P, TrackRegs<R> &t, Dep
 
 template <class R, bool kCharged>
 __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a, PoolArgs<R> pa) {
+  __shared__ WaveHist whs[kBlock / 64];
+  WaveHist &wh = whs[threadIdx.x >> 6];
+  wh_clear(wh);
   const DevParams *P = a.P;
   const Pool<R> &cur = pa.cur;
   const Pool<R> &nxt = pa.nxt;
@@ -685,7 +728,7 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
     const int bN = rN.get(), bC = rC.get();
     int nk = rK.get();
     if (active) {
-      dp.flush(a);
+      dp.flush(wh);
       if (pair) atomicAdd(&a.ev[t.event].npairs, 1ULL);
       if (nannih) atomicAdd(&a.ev[t.event].nannihil, (unsigned long long)nannih);
       if (killed) atomicAdd(&a.cnt->killed, 1ULL);
@@ -699,9 +742,9 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
         if (fate0 == 1) {
           const uint64_t u = child_substream(t.rng);
           emit_new(a, nxt, s.species == 0 ? kN++ : kC++, nk, t, s.species, s.e, s.d[0], s.d[1], s.d[2], u,
-                   trace);
+                   wh, trace);
         } else if (fate0 == 2) {
-          emit_at_rest(a, nxt, kN, nk, t, trace);
+          emit_at_rest(a, nxt, kN, nk, t, wh, trace);
           kN += 2;
         }
       }
@@ -710,14 +753,14 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
         if (fate1 == 1) {
           const uint64_t u = child_substream(t.rng);
           emit_new(a, nxt, s.species == 0 ? kN++ : kC++, nk, t, s.species, s.e, s.d[0], s.d[1], s.d[2], u,
-                   trace);
+                   wh, trace);
         } else if (fate1 == 2) {
-          emit_at_rest(a, nxt, kN, nk, t, trace);
+          emit_at_rest(a, nxt, kN, nk, t, wh, trace);
           kN += 2;
         }
       }
       if (prim_rest) {
-        emit_at_rest(a, nxt, kN, nk, t, trace);
+        emit_at_rest(a, nxt, kN, nk, t, wh, trace);
         kN += 2;
       }
       if (survive) {
@@ -734,6 +777,7 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
       if (tot > 0 && lane() == 63) atomicAdd(&a.cnt->created, (unsigned long long)tot);
     }
   }
+  wh_flush(a, wh);
 }
 
 // RNG state of the tracks created this iteration: substream uid of the seed,
