@@ -226,6 +226,7 @@ struct gxt_ctx {
   std::vector<char> ev_counted;     // event slot already counted in seen_events
   // kernel timing (profile_kernels)
   bool prof = false;
+  int prof_level = 1;
   std::vector<std::array<hipEvent_t, kStepEvents>> ev_free, ev_pending;
   gxt_kernel_stats kst{};
   // stats
@@ -388,8 +389,10 @@ int iterate(gxt_ctx *c, int k) {
     hipEvent_t *evs = nullptr;
     if (c->prof) {
       if (c->ev_free.empty()) {
-        std::array<hipEvent_t, kStepEvents> q;
-        for (auto &x : q) HIP_TRY(hipEventCreate(&x));
+        // level 2 records only the events around the e-/e+ transport (4, 5)
+        std::array<hipEvent_t, kStepEvents> q{};
+        for (int k = 0; k < kStepEvents; ++k)
+          if (c->prof_level == 1 || k == 4 || k == 5) HIP_TRY(hipEventCreate(&q[k]));
         c->ev_free.push_back(q);
       }
       c->ev_pending.push_back(c->ev_free.back());
@@ -418,7 +421,8 @@ int collect(gxt_ctx *c, bool &idle) {
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (auto &q : c->ev_pending) {
     float t[kStepEvents - 1] = {};
-    for (int k = 0; k + 1 < kStepEvents; ++k) HIP_TRY(hipEventElapsedTime(&t[k], q[k], q[k + 1]));
+    for (int k = 0; k + 1 < kStepEvents; ++k)
+      if (q[k] && q[k + 1]) HIP_TRY(hipEventElapsedTime(&t[k], q[k], q[k + 1]));
     c->kst.transport_neutral_ms += t[0];
     c->kst.interact_neutral_ms += t[1];
     c->kst.transport_charged_ms += t[4];
@@ -563,9 +567,11 @@ void destroy_ctx(gxt_ctx *c) {
     c->pf[i].release();
   }
   for (auto &q : c->ev_free)
-    for (auto &x : q) (void)hipEventDestroy(x);
+    for (auto &x : q)
+      if (x) (void)hipEventDestroy(x);
   for (auto &q : c->ev_pending)
-    for (auto &x : q) (void)hipEventDestroy(x);
+    for (auto &x : q)
+      if (x) (void)hipEventDestroy(x);
   c->dd.release();
   (void)hipFree(c->queues);
   (void)hipFree(c->newlist);
@@ -673,6 +679,7 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
   c->cap = (cfg->pool_capacity + kStripes - 1) / kStripes * kStripes;
   c->trace = cfg->trace_tracks > 0;
   c->prof = cfg->profile_kernels > 0;
+  c->prof_level = cfg->profile_kernels > 1 ? 2 : 1;
   c->t0 = std::chrono::steady_clock::now();
   HIP_TRY(hipGetDevice(&c->device));
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

@@ -60,8 +60,8 @@ struct StepStreams {
   hipEvent_t fork, join;
 };
 
-// One step-loop iteration.  With ev (kStepEvents events) every kernel is
-// bracketed by events on its stream: ev[0..3] photon transport,
+// One step-loop iteration.  With ev (kStepEvents events, null entries not
+// recorded) every kernel is bracketed by events on its stream: ev[0..3] photon transport,
 // interactions and RNG derivation of their new tracks (main), ev[4..7] the
 // same for e-/e+ (side), ev[8..9] the counter advance (main).
 constexpr int kStepEvents = 10;

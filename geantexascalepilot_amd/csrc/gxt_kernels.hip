@@ -231,25 +231,32 @@ __device__ __forceinline__ void load_track(const Pool<R> &p, int64_t i, TrackReg
   t.uid = trace ? p.uid[i] : 0;
 }
 
-template <class R>
+// kInPlace: the track goes back to the slot it was loaded from, so its event
+// is already there; kPhotonTransport additionally leaves out what a photon's
+// transport step never changes (direction, energy, track tallies).
+template <class R, bool kInPlace = false, bool kPhotonTransport = false>
 __device__ __forceinline__ void store_track(const Pool<R> &p, int64_t i, const TrackRegs<R> &t, bool trace) {
   p.x[i] = t.pos[0];
   p.y[i] = t.pos[1];
   p.z[i] = t.pos[2];
-  p.dx[i] = t.dir[0];
-  p.dy[i] = t.dir[1];
-  p.dz[i] = t.dir[2];
-  p.e[i] = t.e;
+  if (!kPhotonTransport) {
+    p.dx[i] = t.dir[0];
+    p.dy[i] = t.dir[1];
+    p.dz[i] = t.dir[2];
+    p.e[i] = t.e;
+  }
   p.nil0[i] = t.nil0;
   p.nil1[i] = t.nil1;
   p.nil2[i] = t.nil2;
   t.rng.store(p.rng, p.cap, i);
-  p.event[i] = t.event;
+  if (!kInPlace) p.event[i] = t.event;
   p.meta[i] = t.vol | (t.species << 16);
   p.nsteps[i] = t.nsteps;
-  p.aed[i] = t.aed;
-  p.avis[i] = t.avis;
-  p.alen[i] = t.alen;
+  if (!kPhotonTransport) {
+    p.aed[i] = t.aed;
+    p.avis[i] = t.avis;
+    p.alen[i] = t.alen;
+  }
   if (trace) p.uid[i] = t.uid;
 }
 
@@ -579,7 +586,7 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
       }
       if (q >= 0) {
         // interacting (or stopped) track: back in place for the interaction kernel
-        store_track(cur, slot, t, trace);
+        store_track<R, true, !kCharged>(cur, slot, t, trace);
         if (trace && q != kQueueAtRest) cur.tedep[slot] = edep;
       }
       if (survive) {
@@ -924,26 +931,26 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
   hipError_t e = hipEventRecord(s.fork, s.main);
   if (e == hipSuccess) e = hipStreamWaitEvent(s.side, s.fork, 0);
   if (e != hipSuccess) return e;
-  if (ev) (void)hipEventRecord(ev[0], s.main);
+  if (ev && ev[0]) (void)hipEventRecord(ev[0], s.main);
   hipLaunchKernelGGL((k_transport<R, false>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
-  if (ev) (void)hipEventRecord(ev[1], s.main);
+  if (ev && ev[1]) (void)hipEventRecord(ev[1], s.main);
   hipLaunchKernelGGL((k_interact<R, false>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
-  if (ev) (void)hipEventRecord(ev[2], s.main);
+  if (ev && ev[2]) (void)hipEventRecord(ev[2], s.main);
   hipLaunchKernelGGL((k_derive<R, 0>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
-  if (ev) (void)hipEventRecord(ev[3], s.main);
-  if (ev) (void)hipEventRecord(ev[4], s.side);
+  if (ev && ev[3]) (void)hipEventRecord(ev[3], s.main);
+  if (ev && ev[4]) (void)hipEventRecord(ev[4], s.side);
   hipLaunchKernelGGL((k_transport<R, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
-  if (ev) (void)hipEventRecord(ev[5], s.side);
+  if (ev && ev[5]) (void)hipEventRecord(ev[5], s.side);
   hipLaunchKernelGGL((k_interact<R, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
-  if (ev) (void)hipEventRecord(ev[6], s.side);
+  if (ev && ev[6]) (void)hipEventRecord(ev[6], s.side);
   hipLaunchKernelGGL((k_derive<R, 1>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
-  if (ev) (void)hipEventRecord(ev[7], s.side);
+  if (ev && ev[7]) (void)hipEventRecord(ev[7], s.side);
   e = hipEventRecord(s.join, s.side);
   if (e == hipSuccess) e = hipStreamWaitEvent(s.main, s.join, 0);
   if (e != hipSuccess) return e;
-  if (ev) (void)hipEventRecord(ev[8], s.main);
+  if (ev && ev[8]) (void)hipEventRecord(ev[8], s.main);
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s.main, a.cnt, pa.cur.cap);
-  if (ev) (void)hipEventRecord(ev[9], s.main);
+  if (ev && ev[9]) (void)hipEventRecord(ev[9], s.main);
   return hipGetLastError();
 }
 

@@ -68,7 +68,9 @@ typedef struct gxt_config {
   int32_t max_jobs;         /* concurrent job slots                                 */
   int32_t trace_tracks;     /* >0: record the first trace_steps steps of tracks     */
   int32_t trace_steps;
-  int32_t profile_kernels;  /* >0: time every transport / interaction launch (HIP events) */
+  int32_t profile_kernels;  /* 1: time every step-loop launch with HIP events; 2: only the
+                               e-/e+ transport launches (the dominant kernel; 2 events per
+                               iteration instead of 10) */
   const char *data_dir;     /* directory holding the physics data files             */
 } gxt_config;
 
