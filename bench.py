@@ -76,6 +76,44 @@ def run_batch(g, abi, precision, n_events, steps, warmup, seed=777, pool=2_000_0
     return dict(seconds=t1 - t0, ks=ks, stats=st, edep=edep)
 
 
+def precision_form(g, abi, n_events=20000, seed=2468):
+    """FP32 precision form against FP64 on the sampling calorimeter: 50 x (2.3 mm
+    Pb + 5.7 mm lAr), 40x40 cm, 4 T along +x, n_events electrons of 1-50 GeV
+    (uniform) started over a 1 cm square at the front face.  Per-observable
+    error of the FP32 run against the FP64 run on the same primaries."""
+    import numpy as np
+    prims = g.make_primaries(seed, abi.ELECTRON, 1000.0, 50000.0, 10.0, n_events)
+    res = {}
+    for prec in (abi.FP64, abi.FP32):
+        cfg = abi.config_calorimeter(precision=prec, max_events=n_events, n_zbins=40, zbin_mm=10.0)
+        with g.Transport(cfg) as t:
+            t0 = time.perf_counter()
+            t.submit(prims, 1)
+            t.run()
+            dt = time.perf_counter() - t0
+            ev = t.event_tallies(1, n_events)
+            ez, er, evol = t.profiles()
+            st = t.stats()
+        zc = (np.arange(len(ez)) + 0.5) * 10.0
+        res[prec] = dict(seconds=dt, visible=ev["visible"], layers=evol[1::2][:50], ez=ez,
+                         shower_max_mm=float(zc[np.argmax(ez)]), centroid_mm=float((zc * ez).sum() / ez.sum()),
+                         steps=int(st.track_steps))
+    a, b = res[abi.FP64], res[abi.FP32]
+    lay = (b["layers"] - a["layers"]) / a["layers"]
+    log("precision form: FP64 %.2f s, FP32 %.2f s on %d calorimeter events" % (a["seconds"], b["seconds"], n_events))
+    return dict(workload="e- 1-50 GeV x %d into 50 x (2.3 mm Pb + 5.7 mm lAr), 4 T along +x" % n_events,
+                fp64_events_per_s=n_events / a["seconds"], fp32_events_per_s=n_events / b["seconds"],
+                visible_mean_mev=dict(fp64=float(a["visible"].mean()), fp32=float(b["visible"].mean()),
+                                      rel_err=float(b["visible"].mean() / a["visible"].mean() - 1)),
+                visible_rms_mev=dict(fp64=float(a["visible"].std()), fp32=float(b["visible"].std()),
+                                     rel_err=float(b["visible"].std() / a["visible"].std() - 1)),
+                layer_deposit_rel_err=dict(max_abs=float(np.abs(lay).max()), rms=float(np.sqrt((lay ** 2).mean()))),
+                shower_max_depth_mm=dict(fp64=a["shower_max_mm"], fp32=b["shower_max_mm"]),
+                shower_centroid_mm=dict(fp64=a["centroid_mm"], fp32=b["centroid_mm"],
+                                        diff=b["centroid_mm"] - a["centroid_mm"]),
+                track_steps=dict(fp64=a["steps"], fp32=b["steps"]))
+
+
 def serving(g, abi, seconds, rate_jobs_s, seed=4242):
     """Poisson stream of jobs (40 % one 1 GeV gamma, 40 % 10 x 10 GeV e-,
     20 % 100 x 50 GeV e-) into the PbWO4 block, FP32 form, one shared pool."""
@@ -166,6 +204,7 @@ def main():
     ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp32", action="store_true")
+    ap.add_argument("--skip-precision", action="store_true", help="skip the FP32-vs-FP64 calorimeter comparison")
     ap.add_argument("--pool", type=int, default=2_000_000, help="live-track pool capacity")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a kernel of known traffic that "
@@ -235,6 +274,8 @@ def main():
                            track_steps_per_s=r32["ks"].track_steps / r32["seconds"],
                            speedup_vs_fp64=(a.steps * n / r32["seconds"]) / value,
                            edep_rel_diff_vs_fp64=float(e32.sum() / e64.sum() - 1.0))
+    if not a.skip_precision:
+        out["precision_form"] = precision_form(g, abi)
     if not a.skip_serving:
         gev_per_s = (out.get("fp32", {}).get("value", value)) * 10.0
         rate = 0.6 * gev_per_s / (0.4 * 1.0 + 0.4 * 100.0 + 0.2 * 5000.0)
