@@ -221,7 +221,7 @@ struct gxt_ctx {
   std::vector<char> ev_used;
   double committed = 0;             // summed footprint of events in flight
   double tracks_per_gev = 300.0;    // estimated peak live tracks per GeV of an event (learned)
-  double seen_ratio = 0;            // highest peak per GeV of the events past their maximum
+  double seen_sum = 0;              // summed peak per GeV of the events past their maximum
   int64_t seen_events = 0;          // events past their maximum seen so far
   std::vector<char> ev_counted;     // event slot already counted in seen_events
   // kernel timing (profile_kernels)
@@ -278,9 +278,9 @@ double event_peak(const gxt_ctx *c, const Job *j, int e) {
   return 16.0 * (j->ev_first[e + 1] - j->ev_first[e]) + c->tracks_per_gev * j->ev_gev[e];
 }
 
-// An event in flight whose live count has fallen below half of the highest
+// An event in flight whose live count has fallen below 70 % of the highest
 // seen, once that reached 64 tracks, is past its shower maximum.
-bool past_peak(int live, int mx) { return mx >= 64 && 2 * live < mx; }
+bool past_peak(int live, int mx) { return mx >= 64 && live < 0.7 * mx; }
 
 // Live tracks event e may still hold at once: past its maximum, a margin
 // over its current count; before it, its expected peak (or more, if it
@@ -441,9 +441,10 @@ int collect(gxt_ctx *c, bool &idle) {
   {
     std::lock_guard<std::mutex> lk(c->mu);
     // learn the peak per GeV from the events past their maximum or finished:
-    // once 32 have been seen, the estimate is their highest ratio plus 30 %
-    // (for peaks that fall between synchronisations); from then on it can
-    // only grow
+    // once 32 have been seen, the estimate is their mean ratio plus 30 %
+    // (for peaks that fall between synchronisations).  The budget's 20 %
+    // headroom covers the spread of single events; the sum over the events
+    // in flight follows the mean
     for (auto &jp : c->jobs) {
       Job *j = jp.get();
       if (!j->started) break;
@@ -456,12 +457,11 @@ int collect(gxt_ctx *c, bool &idle) {
         if (!counted && (lv == 0 || past_peak(lv, mx)) && j->ev_gev[e] > 0) {
           counted = 1;
           c->seen_events++;
-          c->seen_ratio =
-              std::max(c->seen_ratio, (mx - 16.0 * (j->ev_first[e + 1] - j->ev_first[e])) / j->ev_gev[e]);
+          c->seen_sum += std::max(0.0, (mx - 16.0 * (j->ev_first[e + 1] - j->ev_first[e])) / j->ev_gev[e]);
         }
       }
     }
-    if (c->seen_events >= 32) c->tracks_per_gev = std::max(1.3 * c->seen_ratio, 1.0);
+    if (c->seen_events >= 32) c->tracks_per_gev = std::max(1.3 * c->seen_sum / c->seen_events, 1.0);
     c->committed = 0;
     for (auto &jp : c->jobs) {
       Job *j = jp.get();
