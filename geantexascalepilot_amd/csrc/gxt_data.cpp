@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 
 namespace gxt {
@@ -328,6 +329,40 @@ void build_jump_matrices(uint32_t out[2][64][9]) {
       std::memcpy(M, S, sizeof(M));
     }
   }
+}
+
+void HostMrg::init(uint32_t seed, uint64_t idx) {
+  static uint32_t J[2][64][9];
+  static std::once_flag once;
+  std::call_once(once, [] { build_jump_matrices(J); });
+  const uint64_t m[2] = {4294967087ULL, 4294944443ULL};
+  for (int i = 0; i < 6; ++i) s[i] = seed;
+  for (int j = 0; j < 64; ++j)
+    if ((idx >> j) & 1ULL)
+      for (int c = 0; c < 2; ++c) {
+        uint64_t r[3];
+        for (int a = 0; a < 3; ++a) {
+          uint64_t acc = 0;
+          for (int b = 0; b < 3; ++b) acc += ((uint64_t)J[c][j][a * 3 + b] * s[3 * c + b]) % m[c];
+          r[a] = acc % m[c];
+        }
+        for (int a = 0; a < 3; ++a) s[3 * c + a] = r[a];
+      }
+}
+
+double HostMrg::uniform() {
+  int64_t p1 = (1403580LL * (int64_t)s[1] - 810728LL * (int64_t)s[0]) % 4294967087LL;
+  if (p1 < 0) p1 += 4294967087LL;
+  s[0] = s[1];
+  s[1] = s[2];
+  s[2] = (uint64_t)p1;
+  int64_t p2 = (527612LL * (int64_t)s[5] - 1370589LL * (int64_t)s[3]) % 4294944443LL;
+  if (p2 < 0) p2 += 4294944443LL;
+  s[3] = s[4];
+  s[4] = s[5];
+  s[5] = (uint64_t)p2;
+  const double norm = 1.0 / (4294967087.0 + 1.0);
+  return (p1 > p2) ? (double)(p1 - p2) * norm : (double)(p1 - p2 + 4294967087LL) * norm;
 }
 
 void build_jump_tables(const uint32_t jump[2][64][9], std::vector<uint32_t> &out) {

@@ -22,6 +22,18 @@ int load_material(const std::string &data_dir, int which, double lar_density, Ho
                   std::string &err);
 void build_geometry(const gxt_config &c, DevGeometry &g);
 void build_jump_matrices(uint32_t out[2][64][9]);
+
+// Record msg as the library's last error (gxt_last_error) and return code.
+int set_error(int code, const std::string &msg);
+
+// Host MRG32k3a: substream idx of the stream whose six seed words are all
+// seed, and its uniform draws (the recurrence, output and jumps of the
+// device generator).
+struct HostMrg {
+  uint64_t s[6];
+  void init(uint32_t seed, uint64_t idx);
+  double uniform();
+};
 // Byte-wise jump tables: out[((c * 8 + k) * 256 + d) * 9 + i] = A_c^(d 2^(76+8k)) mod m_c.
 void build_jump_tables(const uint32_t jump[2][64][9], std::vector<uint32_t> &out);
 

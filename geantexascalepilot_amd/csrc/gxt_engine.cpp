@@ -39,6 +39,13 @@ int set_err(int code, const std::string &msg) {
   return code;
 }
 
+}  // namespace
+
+// the library's last-error message, set by every failing entry point
+int gxt::set_error(int code, const std::string &msg) { return set_err(code, msg); }
+
+namespace {
+
 #define HIP_TRY(expr)                                                                        \
   do {                                                                                       \
     hipError_t e_ = (expr);                                                                  \
@@ -970,39 +977,6 @@ int gxt_serve_stop(gxt_ctx *c) {
 int gxt_make_primaries(uint32_t seed, int species, double emin, double emax, double spot_mm, int n, uint64_t first,
                        uint64_t n_total, gxt_primary *out) {
   if (n < 0 || !out || species < 0 || species > 2) return set_err(GXT_EINVAL, "bad primaries request");
-  // host MRG32k3a (same recurrence and jumps as the device)
-  static uint32_t J[2][64][9];
-  static std::once_flag once;
-  std::call_once(once, [] { build_jump_matrices(J); });
-  auto state = [&](uint64_t idx, uint64_t s[6]) {
-    for (int i = 0; i < 6; ++i) s[i] = seed;
-    const uint64_t m[2] = {4294967087ULL, 4294944443ULL};
-    for (int j = 0; j < 64; ++j)
-      if ((idx >> j) & 1ULL)
-        for (int c = 0; c < 2; ++c) {
-          uint64_t r[3];
-          for (int a = 0; a < 3; ++a) {
-            uint64_t acc = 0;
-            for (int b = 0; b < 3; ++b) acc += ((uint64_t)J[c][j][a * 3 + b] * s[3 * c + b]) % m[c];
-            r[a] = acc % m[c];
-          }
-          for (int a = 0; a < 3; ++a) s[3 * c + a] = r[a];
-        }
-  };
-  auto uniform = [](uint64_t s[6]) {
-    int64_t p1 = (1403580LL * (int64_t)s[1] - 810728LL * (int64_t)s[0]) % 4294967087LL;
-    if (p1 < 0) p1 += 4294967087LL;
-    s[0] = s[1];
-    s[1] = s[2];
-    s[2] = (uint64_t)p1;
-    int64_t p2 = (527612LL * (int64_t)s[5] - 1370589LL * (int64_t)s[3]) % 4294944443LL;
-    if (p2 < 0) p2 += 4294944443LL;
-    s[3] = s[4];
-    s[4] = s[5];
-    s[5] = (uint64_t)p2;
-    const double norm = 1.0 / (4294967087.0 + 1.0);
-    return (p1 > p2) ? (double)(p1 - p2) * norm : (double)(p1 - p2 + 4294967087LL) * norm;
-  };
   for (int i = 0; i < n; ++i) {
     gxt_primary &p = out[i];
     std::memset(&p, 0, sizeof(p));
@@ -1012,9 +986,9 @@ int gxt_make_primaries(uint32_t seed, int species, double emin, double emax, dou
     p.ekin = emin;
     p.dir[2] = 1.0;
     if (emax > emin || spot_mm > 0.0) {
-      uint64_t s[6];
-      state(n_total + first + (uint64_t)i, s);
-      const double u0 = uniform(s), u1 = uniform(s), u2 = uniform(s);
+      HostMrg r;
+      r.init(seed, n_total + first + (uint64_t)i);
+      const double u0 = r.uniform(), u1 = r.uniform(), u2 = r.uniform();
       p.ekin = emin + (emax - emin) * u0;
       p.pos[0] = (u1 - 0.5) * spot_mm;
       p.pos[1] = (u2 - 0.5) * spot_mm;

@@ -49,6 +49,33 @@ def make_primaries(seed, species, emin, emax, spot_mm, n, first=0, n_total=None)
     return p
 
 
+def read_hepevt(path, max_events=-1, vertex=(0.0, 0.0, 0.0), first=0):
+    """HEPEVT file -> (primaries, n_events, n_unsupported) (ProxyHepEvtGenerator,
+    proxy/src/ProxyHepEvtGenerator.cpp:48-132; contract in include/gxt.h)."""
+    v = (C.c_double * 3)(*vertex)
+    n_p, n_e, n_u = C.c_int(0), C.c_int(0), C.c_int(0)
+    path_b = os.fsencode(path)
+    _check(lib.gxt_read_hepevt(path_b, max_events, v, C.c_uint64(first), None, 0, C.byref(n_p), C.byref(n_e),
+                               C.byref(n_u)))
+    p = abi.primaries_array(n_p.value)
+    _check(lib.gxt_read_hepevt(path_b, max_events, v, C.c_uint64(first), p, n_p.value, C.byref(n_p), C.byref(n_e),
+                               C.byref(n_u)))
+    return p, n_e.value, n_u.value
+
+
+def egamma_gun(seed, n_events, per_event, gamma_fraction=0.5, pmin=1000.0, pmax=1.0e6, eta=(-3.0, 3.0),
+               phi=(0.0, 2.0 * 3.141592653589793), vertex=(0.0, 0.0, 0.0), first=0, n_total=None):
+    """Egamma gun (ProxyEgammaGun::GenerateOneEvent, proxy/src/ProxyEgammaGun.cpp:43-85)."""
+    n = n_events * per_event
+    p = abi.primaries_array(n)
+    v = (C.c_double * 3)(*vertex)
+    _check(lib.gxt_egamma_gun(C.c_uint32(seed), n_events, per_event, C.c_double(gamma_fraction), C.c_double(pmin),
+                              C.c_double(pmax), C.c_double(eta[0]), C.c_double(eta[1]), C.c_double(phi[0]),
+                              C.c_double(phi[1]), v, C.c_uint64(first), C.c_uint64(n if n_total is None else n_total),
+                              p))
+    return p
+
+
 def rng_init(seed6, substreams):
     seed6 = np.ascontiguousarray(seed6, dtype=np.uint32)
     idx = np.ascontiguousarray(substreams, dtype=np.uint64)

@@ -159,6 +159,33 @@ int gxt_default_config(gxt_config *cfg);
 int gxt_make_primaries(uint32_t seed, int species, double emin, double emax, double spot_mm,
                        int n, uint64_t first, uint64_t n_total, gxt_primary *out);
 
+/* HEPEVT event file -> primaries (replaces ProxyHepEvtGenerator::GenerateOneEvent,
+ * proxy/src/ProxyHepEvtGenerator.cpp:48-132, which reads one event per call).
+ * Reads up to max_events events (< 0: all).  Each event is NHEP followed by NHEP
+ * lines "ISTHEP IDHEP JDAHEP1 JDAHEP2 PX PY PZ MASS" (GeV).  The primaries of
+ * an event are its entries without daughters (JDAHEP1 = 0) that are either a
+ * daughter of an entry with daughters (the reference's rule) or stable at top
+ * level (ISTHEP = 1; the reference drops these, Geant4's G4HEPEvtInterface
+ * keeps them).  Ekin = p^2 / (sqrt(p^2 + m^2) + m) with the mass as given (the
+ * reference squares it first, ProxyHepEvtGenerator.cpp:109).  Gammas, e- and
+ * e+ (PDG 22, 11, -11) become primaries at vertex (mm) with substreams first,
+ * first+1, ...; other species and zero momenta are counted in n_unsupported.
+ * With out == NULL only the counts are returned.  A truncated event or a
+ * daughter range outside its event is GXT_EDATA. */
+int gxt_read_hepevt(const char *path, int max_events, const double vertex[3], uint64_t first,
+                    gxt_primary *out, int capacity, int *n_primaries, int *n_events,
+                    int *n_unsupported);
+
+/* Egamma gun (ProxyEgammaGun::GenerateOneEvent, proxy/src/ProxyEgammaGun.cpp:43-85):
+ * n_events x per_event particles, gamma with probability gamma_fraction else
+ * e-, momentum uniform in [pmin, pmax] (MeV), pseudorapidity uniform in
+ * [eta_min, eta_max], azimuth uniform in [phi_min, phi_max], from vertex (mm).
+ * Particle i gets substream first+i; its four draws (type, momentum, eta,
+ * phi, in the reference's order) come from substream n_total + first + i. */
+int gxt_egamma_gun(uint32_t seed, int n_events, int per_event, double gamma_fraction, double pmin,
+                   double pmax, double eta_min, double eta_max, double phi_min, double phi_max,
+                   const double vertex[3], uint64_t first, uint64_t n_total, gxt_primary *out);
+
 /* ---- batch transport --------------------------------------------------- */
 /* Queue a job of n primaries (replaces ProxyDeviceManager::UploadTrackState,
  * ProxyDeviceManager.cpp:152-204).  Non-blocking; the job is admitted into

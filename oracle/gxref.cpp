@@ -1508,6 +1508,116 @@ int gxref_destroy(gxref_ctx *c) {
   return GXT_OK;
 }
 
+// ProxyHepEvtGenerator::GenerateOneEvent (proxy/src/ProxyHepEvtGenerator.cpp:48-132),
+// repeated per event: read NHEP entries, mark the daughters (ISTHEP > 0) of
+// the entries with daughters as -1 (:76-90), and turn the marked entries
+// without daughters into primaries (:97-120).  Two fixes: stable top-level
+// entries (ISTHEP = 1, no daughters, nobody's daughter) are primaries too,
+// as in Geant4's G4HEPEvtInterface; the mass column is used as the mass
+// (:109 squares it).  Species other than gamma/e-/e+ are counted, not made.
+int gxref_read_hepevt(const char *path, int max_events, const double vertex[3], uint64_t first, gxt_primary *out,
+                      int capacity, int *n_primaries, int *n_events, int *n_unsupported) {
+  if (!path || !n_primaries || !n_events || !n_unsupported) return fail(GXT_EINVAL, "bad arguments");
+  std::ifstream f(path);
+  if (!f) return fail(GXT_EDATA, "cannot open file");
+  struct Hep {
+    int ISTHEP, IDHEP, JDAHEP1, JDAHEP2;
+    double PHEP1, PHEP2, PHEP3, PHEP4;
+  };
+  int nprim = 0, nev = 0, nunsup = 0;
+  while (max_events < 0 || nev < max_events) {
+    int NHEP = 0;
+    if (!(f >> NHEP)) break;
+    if (NHEP < 0) return fail(GXT_EDATA, "negative entry count");
+    std::vector<Hep> hepg(NHEP);
+    for (int i = 0; i < NHEP; ++i)
+      if (!(f >> hepg[i].ISTHEP >> hepg[i].IDHEP >> hepg[i].JDAHEP1 >> hepg[i].JDAHEP2 >> hepg[i].PHEP1 >>
+            hepg[i].PHEP2 >> hepg[i].PHEP3 >> hepg[i].PHEP4))
+        return fail(GXT_EDATA, "truncated event");
+    std::vector<char> top(NHEP);
+    for (int i = 0; i < NHEP; ++i) top[i] = hepg[i].ISTHEP == 1;
+    for (int i = 0; i < NHEP; ++i) {
+      if (hepg[i].JDAHEP1 > 0) {
+        const int jda1 = hepg[i].JDAHEP1 - 1, jda2 = hepg[i].JDAHEP2 - 1;
+        if (jda2 < jda1 || jda2 >= NHEP) return fail(GXT_EDATA, "daughter range outside the event");
+        for (int j = jda1; j <= jda2; ++j)
+          if (hepg[j].ISTHEP > 0) {
+            hepg[j].ISTHEP = -1;
+            top[j] = 0;
+          }
+      }
+    }
+    for (int i = 0; i < NHEP; ++i) {
+      if (!((hepg[i].ISTHEP == -1 || top[i]) && hepg[i].JDAHEP1 == 0)) continue;
+      const double px = hepg[i].PHEP1, py = hepg[i].PHEP2, pz = hepg[i].PHEP3;
+      const double p = std::sqrt(px * px + py * py + pz * pz);
+      const int id = hepg[i].IDHEP;
+      const int species = id == 22 ? 0 : (id == 11 ? 1 : (id == -11 ? 2 : -1));
+      if (species < 0 || !(p > 0.0)) {
+        ++nunsup;
+        continue;
+      }
+      if (out) {
+        if (nprim >= capacity) {
+          *n_primaries = nprim;
+          return fail(GXT_ECAPACITY, "more primaries than capacity");
+        }
+        gxt_primary &q = out[nprim];
+        std::memset(&q, 0, sizeof(q));
+        const double pm = p * 1000.0, mass = hepg[i].PHEP4 * 1000.0;
+        q.species = species;
+        q.event = nev;
+        q.ekin = pm * pm / (std::sqrt(pm * pm + mass * mass) + mass);
+        q.dir[0] = px / p;
+        q.dir[1] = py / p;
+        q.dir[2] = pz / p;
+        for (int k = 0; k < 3; ++k) q.pos[k] = vertex ? vertex[k] : 0.0;
+        q.substream = first + (uint64_t)nprim;
+      }
+      ++nprim;
+    }
+    ++nev;
+  }
+  *n_primaries = nprim;
+  *n_events = nev;
+  *n_unsupported = nunsup;
+  return GXT_OK;
+}
+
+// ProxyEgammaGun::GenerateOneEvent (proxy/src/ProxyEgammaGun.cpp:43-85): per
+// particle a gamma with probability fGammaFraction else an electron, p, eta
+// and phi uniform, theta = 2 atan(exp(-eta)), Ekin = p^2 / (sqrt(p^2+m^2)+m);
+// the draws come from substream n_total + first + i.
+int gxref_egamma_gun(uint32_t seed, int n_events, int per_event, double gamma_fraction, double pmin, double pmax,
+                     double eta_min, double eta_max, double phi_min, double phi_max, const double vertex[3],
+                     uint64_t first, uint64_t n_total, gxt_primary *out) {
+  if (!out || n_events < 0 || per_event <= 0 || !(pmax >= pmin) || !(pmin > 0.0)) return fail(GXT_EINVAL, "bad gun");
+  uint32_t s6[6];
+  for (int i = 0; i < 6; ++i) s6[i] = seed;
+  for (int i = 0; i < n_events * per_event; ++i) {
+    uint32_t st[6];
+    substream_state(s6, n_total + first + (uint64_t)i, st);
+    const int pid = mrg_uniform(st) < gamma_fraction ? 22 : 11;
+    const double charge = pid == 22 ? 0.0 : -1.0;
+    const double p = pmin + (pmax - pmin) * mrg_uniform(st);
+    const double mass = kMe * charge * charge;
+    const double eta = eta_min + (eta_max - eta_min) * mrg_uniform(st);
+    const double phi = phi_min + (phi_max - phi_min) * mrg_uniform(st);
+    const double theta = 2.0 * std::atan(std::exp(-eta));
+    gxt_primary &q = out[i];
+    std::memset(&q, 0, sizeof(q));
+    q.species = pid == 22 ? 0 : 1;
+    q.event = i / per_event;
+    q.ekin = p * p / (std::sqrt(p * p + mass * mass) + mass);
+    q.dir[0] = std::sin(theta) * std::cos(phi);
+    q.dir[1] = std::sin(theta) * std::sin(phi);
+    q.dir[2] = std::cos(theta);
+    for (int k = 0; k < 3; ++k) q.pos[k] = vertex ? vertex[k] : 0.0;
+    q.substream = first + (uint64_t)i;
+  }
+  return GXT_OK;
+}
+
 int gxref_make_primaries(uint32_t seed, int species, double emin, double emax, double spot_mm, int n,
                          uint64_t first, uint64_t n_total, gxt_primary *out) {
   if (n < 0 || !out) return fail(GXT_EINVAL, "bad primaries request");

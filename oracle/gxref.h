@@ -27,6 +27,12 @@ const char *gxref_last_error(void);
 
 int gxref_make_primaries(uint32_t seed, int species, double emin, double emax, double spot_mm, int n,
                          uint64_t first, uint64_t n_total, gxt_primary *out);
+/* Event front end, same contract as gxt_read_hepevt / gxt_egamma_gun. */
+int gxref_read_hepevt(const char *path, int max_events, const double vertex[3], uint64_t first, gxt_primary *out,
+                      int capacity, int *n_primaries, int *n_events, int *n_unsupported);
+int gxref_egamma_gun(uint32_t seed, int n_events, int per_event, double gamma_fraction, double pmin, double pmax,
+                     double eta_min, double eta_max, double phi_min, double phi_max, const double vertex[3],
+                     uint64_t first, uint64_t n_total, gxt_primary *out);
 
 /* Transport n primaries of one job (events numbered by gxt_primary.event)
  * with nthreads host threads (events handed out dynamically). */

@@ -142,6 +142,30 @@ def make_primaries(seed, species, emin, emax, spot, n, first=0, n_total=None):
     return p
 
 
+def read_hepevt(path, max_events=-1, vertex=(0.0, 0.0, 0.0), first=0):
+    v = (C.c_double * 3)(*vertex)
+    n_p, n_e, n_u = C.c_int(0), C.c_int(0), C.c_int(0)
+    pb = os.fsencode(path)
+    check(lib().gxref_read_hepevt(pb, max_events, v, C.c_uint64(first), None, 0, C.byref(n_p), C.byref(n_e),
+                                  C.byref(n_u)))
+    p = abi.primaries_array(n_p.value)
+    check(lib().gxref_read_hepevt(pb, max_events, v, C.c_uint64(first), p, n_p.value, C.byref(n_p), C.byref(n_e),
+                                  C.byref(n_u)))
+    return p, n_e.value, n_u.value
+
+
+def egamma_gun(seed, n_events, per_event, gamma_fraction, pmin, pmax, eta, phi, vertex=(0.0, 0.0, 0.0), first=0,
+               n_total=None):
+    n = n_events * per_event
+    p = abi.primaries_array(n)
+    v = (C.c_double * 3)(*vertex)
+    check(lib().gxref_egamma_gun(C.c_uint32(seed), n_events, per_event, C.c_double(gamma_fraction), C.c_double(pmin),
+                                 C.c_double(pmax), C.c_double(eta[0]), C.c_double(eta[1]), C.c_double(phi[0]),
+                                 C.c_double(phi[1]), v, C.c_uint64(first),
+                                 C.c_uint64(n if n_total is None else n_total), p))
+    return p
+
+
 def rng_init(seed6, idx):
     seed6 = np.ascontiguousarray(seed6, dtype=np.uint32)
     idx = np.ascontiguousarray(idx, dtype=np.uint64)
