@@ -31,6 +31,11 @@ constexpr int kBlock = 256;
 #define GXT_STEP_WAVES 2
 #endif
 
+// Steps a track may take in one transport launch while it neither interacts
+// nor ends; each is the same as one iteration of the step loop, and a
+// track's results do not depend on how its steps are scheduled.
+constexpr int kInnerSteps = 4;
+
 // ------------------------------------------------------------------ wave helpers
 __device__ __forceinline__ int lane() { return (int)__lane_id(); }
 
@@ -497,7 +502,7 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
     int q = -1;
     int slot = 0;
     bool survive = false, annihilate = false, dies = false, killed = false, escaped = false;
-    Dep dA, dB;  // along-step loss (mid-step, pre-step volume); local deposit at the end point
+    int nsub = 0;  // steps taken by this track in this launch
     double edep = 0.0;
     if (active) {
       int is, off;
@@ -505,10 +510,16 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
       slot = (int)out_slot(kCharged ? 1 : 0, is, off, sc);
       load_track(cur, slot, t, trace);
       if (!kCharged) t.species = 0;
-      const DevMaterial &m = P->mat[P->geo.slot[t.vol]];
       // species as a value the compiler can bound: 0 here, or 1 / 2 there
       const int sp = kCharged ? (t.species == 2 ? 2 : 1) : 0;
       const bool charged = kCharged;
+      // a track keeps stepping while it neither interacts nor ends, up to
+      // kInnerSteps steps: each step is the same as one iteration of the loop
+      for (;;) {
+      survive = false;
+      edep = 0.0;
+      Dep dA, dB;  // along-step loss (mid-step, pre-step volume); local deposit at the end point
+      const DevMaterial &m = P->mat[P->geo.slot[t.vol]];
       R xs0 = R(0), xs1 = R(0), xs2 = R(0);
       double bestd = kHuge;
       int sel = -1;
@@ -567,6 +578,13 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
       }
       // positron at rest: its two photons are made by the interaction kernel
       if (annihilate) q = kQueueAtRest;
+      // the step's deposits go to the wave's LDS histograms now (LDS
+      // atomics do not hold up the global reservations below)
+      dA.flush(wh);
+      dB.flush(wh);
+      ++nsub;
+      if (!survive || nsub >= kInnerSteps) break;
+      }
     }
     // returning atomics first, all in flight together: the survivor's slot in
     // the next pool and the queue entry
@@ -576,8 +594,6 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
     const int bs = rs.get();
     // then stores and no-return atomics
     if (active) {
-      dA.flush(wh);
-      dB.flush(wh);
       if (escaped) unsafeAtomicAdd(&a.ev[t.event].leak, (double)t.e);
       if (killed) atomicAdd(&a.cnt->killed, 1ULL);
       if (dies) {
@@ -601,8 +617,15 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
       else if (qpos == -2)
         atomicOr(&a.cnt->overflow, 1);
     }
-    wave_add_u64(&a.cnt->steps, active);
-    if (kCharged) wave_add_u64(&a.cnt->steps_charged, active);
+    {
+      // the wave's steps, one atomic
+      const int ns = wave_incl_scan(nsub);
+      const int tot = __shfl(ns, 63, 64);
+      if (tot > 0 && lane() == 63) {
+        atomicAdd(&a.cnt->steps, (unsigned long long)tot);
+        if (kCharged) atomicAdd(&a.cnt->steps_charged, (unsigned long long)tot);
+      }
+    }
   }
   wh_flush(a, wh);
 }
