@@ -15,6 +15,9 @@ DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 GEOM_BOX, GEOM_CALO = 0, 1
 MAT_PB, MAT_LAR, MAT_PBWO4 = 0, 1, 2
 FP64, FP32 = 0, 1
+STEPPER_HELIX, STEPPER_RK45 = 0, 1
+RK_ANGLE = 0.25
+CMS_FIELD_MAP = os.path.join(DATA_DIR, "field", "cmsmagfield2015.txt")
 GAMMA, ELECTRON, POSITRON = 0, 1, 2
 
 TAB = dict(LAMBDA_COMPT=0, LAMBDAPRIM_COMPT=1, LAMBDA_CONV=2, LAMBDAPRIM_PHOT=3, LAMBDA_BREM_EM=4,
@@ -38,6 +41,8 @@ class Config(C.Structure):
         ("n_rbins", C.c_int32), ("pool_capacity", C.c_int64), ("max_steps", C.c_int32),
         ("max_events", C.c_int32), ("max_jobs", C.c_int32), ("trace_tracks", C.c_int32),
         ("trace_steps", C.c_int32), ("profile_kernels", C.c_int32), ("data_dir", C.c_char_p),
+        ("field_map", C.c_char_p), ("field_origin", C.c_double * 3), ("field_stepper", C.c_int32),
+        ("pad1", C.c_int32),
     ]
 
 
@@ -116,6 +121,11 @@ def base_config(**kw):
     for k, v in kw.items():
         if k == "field":
             c.field[:] = v
+        elif k == "field_origin":
+            c.field_origin[:] = v
+        elif k == "field_map":
+            c._field_map = None if v is None else os.fsencode(v)
+            c.field_map = c._field_map
         else:
             setattr(c, k, v)
     return c
@@ -144,6 +154,16 @@ def config_calorimeter(precision=FP64, **kw):
              max_events=20000, precision=precision)
     d.update(kw)
     return base_config(**d)
+
+
+def config_calorimeter_cms(precision=FP64, stepper=STEPPER_HELIX, **kw):
+    """The sampling calorimeter in the CMS 2015 solenoid map, its front face
+    centre at r = 600 mm, z = 6 m (the end region: Bz falls from 2.8 to 2.0 T
+    across the calorimeter and Br reaches 0.9 T)."""
+    d = dict(field=(0.0, 0.0, 0.0), field_map=CMS_FIELD_MAP, field_origin=(600.0, 0.0, 6000.0),
+             field_stepper=stepper)
+    d.update(kw)
+    return config_calorimeter(precision=precision, **d)
 
 
 def primaries_array(n):

@@ -983,4 +983,244 @@ __device__ __forceinline__ int propagate(const DevParams *P, int species, R e, d
   return kLimit;
 }
 
+// ------------------------------------------------------------------ general field
+// Field of the cylindrical (r, z) map at (x, y, z) of the map frame
+// (ScalarCMSmagField::GetFieldValue / GetFieldValueRZ,
+// examples/magneticfield/simplifiedCMS/inc/Geant/ScalarCMSmagField.h:247-331):
+// r and z clamped to the grid, bilinear weights from the position inside the
+// cell, cylindrical components turned with the azimuth.  The cell index is
+// clamped to the last cell so the upper grid edge reads its own nodes, and the
+// axis (r = 0) takes azimuth 0.  The cell fractions are quotients, so nodes
+// and cell-edge midpoints of a grid with representable spacing come out exact.
+__device__ __forceinline__ void map_field(const DevField &f, double x, double y, double z, double B[3]) {
+  const double r = sqrt(x * x + y * y);
+  const double rmax = f.r0 + (double)(f.nr - 1) * f.dr;
+  const double zmax = f.z0 + (double)(f.nz - 1) * f.dz;
+  const double rc = fmin(fmax(r, f.r0), rmax);
+  const double zc = fmin(fmax(z, f.z0), zmax);
+  const double fr = (rc - f.r0) / f.dr;
+  const double fz = (zc - f.z0) / f.dz;
+  const int ir = min((int)fr, f.nr - 2);
+  const int iz = min((int)fz, f.nz - 2);
+  const double tr = fr - (double)ir, tz = fz - (double)iz;
+  const double w00 = (1.0 - tr) * (1.0 - tz), w01 = (1.0 - tr) * tz;
+  const double w10 = tr * (1.0 - tz), w11 = tr * tz;
+  const double *p = f.v + ((int64_t)ir * f.nz + iz) * 3;
+  const double *q = p + (int64_t)f.nz * 3;
+  double b[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) b[c] = w00 * p[c] + w01 * p[3 + c] + w10 * q[c] + w11 * q[3 + c];
+  double cs = 1.0, sn = 0.0;
+  if (r > 0.0) {
+    cs = x / r;
+    sn = y / r;
+  }
+  B[0] = b[0] * cs - b[1] * sn;
+  B[1] = b[0] * sn + b[1] * cs;
+  B[2] = b[2];
+}
+
+// Field (tesla) at a point of the setup frame (FieldLookup::GetFieldValue,
+// FieldPropagationHandler.hpp:27-39): the map at pos + origin, or uniform.
+__device__ __forceinline__ void field_at(const DevField &f, const double pos[3], double B[3]) {
+  if (f.v) {
+    map_field(f, pos[0] + f.origin[0], pos[1] + f.origin[1], pos[2] + f.origin[2], B);
+  } else {
+    B[0] = f.b[0];
+    B[1] = f.b[1];
+    B[2] = f.b[2];
+  }
+}
+
+// Cash-Karp embedded Runge-Kutta 4(5) tableau (J. R. Cash, A. H. Karp, ACM
+// Trans. Math. Softw. 16 (1990) 201-222); the fifth-order solution is
+// propagated, fifth minus fourth order is the error estimate.
+namespace ck {
+constexpr double a21 = 1.0 / 5.0;
+constexpr double a31 = 3.0 / 40.0, a32 = 9.0 / 40.0;
+constexpr double a41 = 3.0 / 10.0, a42 = -9.0 / 10.0, a43 = 6.0 / 5.0;
+constexpr double a51 = -11.0 / 54.0, a52 = 5.0 / 2.0, a53 = -70.0 / 27.0, a54 = 35.0 / 27.0;
+constexpr double a61 = 1631.0 / 55296.0, a62 = 175.0 / 512.0, a63 = 575.0 / 13824.0, a64 = 44275.0 / 110592.0,
+                 a65 = 253.0 / 4096.0;
+constexpr double b1 = 37.0 / 378.0, b3 = 250.0 / 621.0, b4 = 125.0 / 594.0, b6 = 512.0 / 1771.0;
+constexpr double e1 = b1 - 2825.0 / 27648.0, e3 = b3 - 18575.0 / 48384.0, e4 = b4 - 13525.0 / 55296.0,
+                 e5 = -277.0 / 14336.0, e6 = b6 - 1.0 / 4.0;
+}  // namespace ck
+
+// Stage derivative of the Lorentz equation in arc length: f = kq (u x B(x)),
+// kq = -q kB2C / p (1/(tesla mm)).
+template <class R>
+__device__ __forceinline__ void lorentz(const DevField &F, R kq, const double x[3], const R u[3], R f[3]) {
+  double B[3];
+  field_at(F, x, B);
+  const R b0 = (R)B[0], b1 = (R)B[1], b2 = (R)B[2];
+  f[0] = kq * (u[1] * b2 - u[2] * b1);
+  f[1] = kq * (u[2] * b0 - u[0] * b2);
+  f[2] = kq * (u[0] * b1 - u[1] * b0);
+}
+
+// One Cash-Karp step of length h from (pos, dir); f1 is the derivative at the
+// start.  Adds |error estimate| of position and direction to ex, eu.
+template <class R>
+__device__ __forceinline__ void ck45_step(const DevField &F, R kq, double h, const R f1[3], double pos[3], R dir[3],
+                                          double &ex, double &eu) {
+  const R hr = (R)h;
+  double x[3];
+  R u2[3], u3[3], u4[3], u5[3], u6[3];
+  R f2[3], f3[3], f4[3], f5[3], f6[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    x[c] = pos[c] + h * (ck::a21 * (double)dir[c]);
+    u2[c] = dir[c] + hr * (R(ck::a21) * f1[c]);
+  }
+  lorentz<R>(F, kq, x, u2, f2);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    x[c] = pos[c] + h * (ck::a31 * (double)dir[c] + ck::a32 * (double)u2[c]);
+    u3[c] = dir[c] + hr * (R(ck::a31) * f1[c] + R(ck::a32) * f2[c]);
+  }
+  lorentz<R>(F, kq, x, u3, f3);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    x[c] = pos[c] + h * (ck::a41 * (double)dir[c] + ck::a42 * (double)u2[c] + ck::a43 * (double)u3[c]);
+    u4[c] = dir[c] + hr * (R(ck::a41) * f1[c] + R(ck::a42) * f2[c] + R(ck::a43) * f3[c]);
+  }
+  lorentz<R>(F, kq, x, u4, f4);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    x[c] = pos[c] + h * (ck::a51 * (double)dir[c] + ck::a52 * (double)u2[c] + ck::a53 * (double)u3[c] +
+                         ck::a54 * (double)u4[c]);
+    u5[c] = dir[c] + hr * (R(ck::a51) * f1[c] + R(ck::a52) * f2[c] + R(ck::a53) * f3[c] + R(ck::a54) * f4[c]);
+  }
+  lorentz<R>(F, kq, x, u5, f5);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    x[c] = pos[c] + h * (ck::a61 * (double)dir[c] + ck::a62 * (double)u2[c] + ck::a63 * (double)u3[c] +
+                         ck::a64 * (double)u4[c] + ck::a65 * (double)u5[c]);
+    u6[c] = dir[c] + hr * (R(ck::a61) * f1[c] + R(ck::a62) * f2[c] + R(ck::a63) * f3[c] + R(ck::a64) * f4[c] +
+                           R(ck::a65) * f5[c]);
+  }
+  lorentz<R>(F, kq, x, u6, f6);
+  double ex2 = 0.0, eu2 = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double dx = h * (ck::e1 * (double)dir[c] + ck::e3 * (double)u3[c] + ck::e4 * (double)u4[c] +
+                           ck::e5 * (double)u5[c] + ck::e6 * (double)u6[c]);
+    const R du = hr * (R(ck::e1) * f1[c] + R(ck::e3) * f3[c] + R(ck::e4) * f4[c] + R(ck::e5) * f5[c] +
+                       R(ck::e6) * f6[c]);
+    ex2 += dx * dx;
+    eu2 += (double)(du * du);
+    pos[c] = pos[c] + h * (ck::b1 * (double)dir[c] + ck::b3 * (double)u3[c] + ck::b4 * (double)u4[c] +
+                           ck::b6 * (double)u6[c]);
+    dir[c] = dir[c] + hr * (R(ck::b1) * f1[c] + R(ck::b3) * f3[c] + R(ck::b4) * f4[c] + R(ck::b6) * f6[c]);
+  }
+  unit3(dir[0], dir[1], dir[2]);
+  ex += sqrt(ex2);
+  eu += sqrt(eu2);
+}
+
+// Sub-steps of the RK45 stepper within one propagation sub-step of the
+// transport, and within one call of the stepper at most.
+constexpr int kRkMaxSub = 16;
+constexpr int kRkMaxN = 65537;
+
+// Advances a charged track by s with the configured stepper: a helix in the
+// start-point field B (|B| = bm; straight line where it vanishes), or
+// ceil(s k bm / GXT_RK_ANGLE) equal Cash-Karp steps, k = 0.299792458 / p.
+template <class R>
+__device__ __forceinline__ void field_advance(const DevField &F, const double B[3], double bm, R q, R mom, double s,
+                                              double pos[3], R dir[3], double &ex, double &eu) {
+  if (F.stepper == GXT_STEPPER_RK45) {
+    // sub-steps: at most kRkMaxN, whatever the inputs (an infinite turn included)
+    const double turn = s * (-kB2C) * bm / (double)mom;
+    const int n = turn > GXT_RK_ANGLE ? (int)fmin(ceil(turn / GXT_RK_ANGLE), (double)kRkMaxN) : 1;
+    const double h = s / (double)n;
+    const R kq = -q * R(kB2C) / mom;
+    R f1[3];
+    const R u0[3] = {dir[0], dir[1], dir[2]};
+    {
+      const R b0 = (R)B[0], b1 = (R)B[1], b2 = (R)B[2];
+      f1[0] = kq * (u0[1] * b2 - u0[2] * b1);
+      f1[1] = kq * (u0[2] * b0 - u0[0] * b2);
+      f1[2] = kq * (u0[0] * b1 - u0[1] * b0);
+    }
+    for (int k = 0; k < n; ++k) {
+      if (k > 0) lorentz<R>(F, kq, pos, dir, f1);
+      ck45_step<R>(F, kq, h, f1, pos, dir, ex, eu);
+    }
+  } else if (bm > 0.0) {
+    helix_step<R>(B, bm, q, mom, (R)s, pos, dir);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pos[c] += s * (double)dir[c];
+  }
+}
+
+// Propagation of an e-/e+ in a general field (a map, or the RK45 stepper):
+// the same sub-step choice as propagate() with the field looked up at the
+// start of every sub-step (FieldPropagationHandler::Propagate,
+// FieldPropagationHandler.cpp:93-111: step min(pstep or snext, max(SafeLength
+// in the local field, safety))).  RK45 sub-steps are also kept within
+// kRkMaxSub Cash-Karp steps.
+template <class R>
+__device__ __forceinline__ int propagate_general(const DevParams *P, int species, R e, double pos[3], R dir[3],
+                                                 int &vol, double pstep, double &step) {
+  const DevGeometry &g = P->geo;
+  const DevField &F = P->field;
+  const R q = species == 1 ? R(-1) : R(1);
+  const R mom = m_sqrt(e * (e + R(2) * R(kMe)));
+  const bool rk = F.stepper == GXT_STEPPER_RK45;
+  double ex = 0.0, eu = 0.0;
+  double remaining = pstep;
+  step = 0.0;
+  for (int it = 0; it < 64; ++it) {
+    const double saf = safety(g, vol, pos);
+    double B[3];
+    field_at(F, pos, B);
+    const double bm = sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+    const double smax = rk && bm > 0.0 ? kRkMaxSub * GXT_RK_ANGLE * (double)mom / ((-kB2C) * bm) : kHuge;
+    if (remaining <= saf && remaining <= smax) {
+      field_advance<R>(F, B, bm, q, mom, remaining, pos, dir, ex, eu);
+      step += remaining;
+      return kPhysics;
+    }
+    int face;
+    const double slin = dist_out(g, vol, pos, (double)dir[0], (double)dir[1], (double)dir[2], face);
+    double sl = kHuge;
+    if (bm > 0.0) {
+      const R pl = mom * (dir[0] * (R)(B[0] / bm) + dir[1] * (R)(B[1] / bm) + dir[2] * (R)(B[2] / bm));
+      R pt2 = mom * mom - pl * pl;
+      if (pt2 < R(0)) pt2 = R(0);
+      const R pt = m_sqrt(pt2);
+      if (pt > R(0)) sl = (double)(R(2) * m_sqrt(R(0.1) / (R(0.299792458) * (R)bm / pt)));
+    }
+    if (slin <= sl && slin <= remaining && slin <= smax) {
+      // aimed step: cross the aimed face by construction, on its plane
+      const double s = fmax(slin, 1.0e-6);
+      field_advance<R>(F, B, bm, q, mom, s, pos, dir, ex, eu);
+      step += s;
+      if (face == 0 || face == 1) {
+        const int nv = face == 1 ? vol + 1 : vol - 1;
+        pos[2] = g.z[face == 1 ? vol + 1 : vol];
+        if (nv < 0 || nv >= g.nvol || fabs(pos[0]) > g.hx || fabs(pos[1]) > g.hy) return kEscaped;
+        vol = nv;
+        return kBoundary;
+      }
+      return kEscaped;
+    }
+    const double s = fmin(fmin(remaining, fmax(sl, saf)), smax);
+    field_advance<R>(F, B, bm, q, mom, s, pos, dir, ex, eu);
+    step += s;
+    remaining -= s;
+    const int v2 = locate(g, pos, (double)dir[0], (double)dir[1], (double)dir[2]);
+    if (v2 != vol) {
+      if (v2 < 0) return kEscaped;
+      vol = v2;
+      return kBoundary;
+    }
+    if (remaining <= 0.0) return kPhysics;
+  }
+  return kLimit;
+}
+
 }  // namespace gxt

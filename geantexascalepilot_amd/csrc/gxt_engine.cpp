@@ -26,6 +26,7 @@
 
 #include "../../include/gxt.h"
 #include "gxt_data.h"
+#include "gxt_field.h"
 #include "gxt_kernels.h"
 
 using namespace gxt;
@@ -220,6 +221,10 @@ struct gxt_ctx {
   int64_t trace_cap = 0;
   gxt_primary *stage = nullptr;
   int64_t stage_cap = 0;
+  // non-uniform field
+  gxt_field_map fmap;
+  std::string fmap_path;
+  int general_field = 0;
   // jobs
   std::mutex mu;
   std::deque<std::unique_ptr<Job>> jobs;   // submitted, not finished (FIFO)
@@ -261,6 +266,8 @@ struct gxt_ctx {
     a.rows.nrows = grid;
     a.trace = trace_buf;
     a.trace_cap = trace_cap;
+    a.general_field = general_field;
+    a.pad = 0;
     return a;
   }
 
@@ -580,6 +587,7 @@ void destroy_ctx(gxt_ctx *c) {
     for (auto &x : q)
       if (x) (void)hipEventDestroy(x);
   c->dd.release();
+  release_field_map(c->fmap);
   (void)hipFree(c->queues);
   (void)hipFree(c->newlist);
   (void)hipFree(c->cnt);
@@ -611,6 +619,10 @@ int check_config(const gxt_config *c) {
   if (c->precision != GXT_FP64 && c->precision != GXT_FP32) return set_err(GXT_EINVAL, "bad precision");
   if (!(c->tracking_cut > 0)) return set_err(GXT_EINVAL, "tracking_cut must be > 0");
   if (!c->data_dir) return set_err(GXT_EINVAL, "data_dir not set");
+  if (c->field_stepper != GXT_STEPPER_HELIX && c->field_stepper != GXT_STEPPER_RK45)
+    return set_err(GXT_EINVAL, "bad field_stepper");
+  for (int k = 0; k < 3; ++k)
+    if (!std::isfinite(c->field[k]) || !std::isfinite(c->field_origin[k])) return set_err(GXT_EINVAL, "bad field");
   return GXT_OK;
 }
 
@@ -712,6 +724,15 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
   P.B[1] = cfg->field[1];
   P.B[2] = cfg->field[2];
   P.bmag = std::sqrt(P.B[0] * P.B[0] + P.B[1] * P.B[1] + P.B[2] * P.B[2]);
+  if (cfg->field_map) {
+    c->fmap_path = cfg->field_map;
+    c->cfg.field_map = c->fmap_path.c_str();
+    rc = read_field_map(cfg->field_map, c->fmap);
+    if (rc == GXT_OK) rc = upload_field_map(c->fmap);
+    if (rc) return rc;
+  }
+  P.field = dev_field(cfg->field_map ? &c->fmap : nullptr, cfg->field, cfg->field_origin, cfg->field_stepper);
+  c->general_field = P.field.general;
   P.tracking_cut = cfg->tracking_cut;
   P.zbin = cfg->zbin_mm;
   P.rbin = cfg->rbin_mm;

@@ -30,6 +30,8 @@ struct StepArgs {
   TallyRows rows;
   double *trace;
   int64_t trace_cap;
+  int general_field;  // e-/e+ transport through the general-field propagation
+  int pad;
 };
 
 template <class R>
@@ -81,5 +83,8 @@ hipError_t launch_macro_xs(const DevParams *P, int precision, int slot, int spec
 hipError_t launch_sample(const DevParams *P, int precision, int slot, int model, const SampleArgs &s, hipStream_t st);
 hipError_t launch_helix(const double B[3], int precision, const double *charge, const double *mom, const double *step,
                         double *pos, double *dir, int n, hipStream_t st);
+hipError_t launch_field_value(const DevField &F, const double *pos, double *b, int n, hipStream_t st);
+hipError_t launch_field_step(const DevField &F, int precision, const double *charge, const double *mom,
+                             const double *step, double *pos, double *dir, double *err, int n, hipStream_t st);
 
 }  // namespace gxt

@@ -467,7 +467,9 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
 // One kernel per region: kCharged = false steps the photons of [0, nN),
 // kCharged = true the electrons and positrons of [cap - nC, cap); the species
 // class is a compile-time constant so each variant carries only its own code.
-template <class R, bool kCharged>
+// kGeneral: e-/e+ in a field map or with the RK45 stepper (propagate_general);
+// otherwise straight lines and the exact uniform-field helix.
+template <class R, bool kCharged, bool kGeneral = false>
 __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a, PoolArgs<R> pa) {
   __shared__ WaveHist whs[kBlock / 64];
   WaveHist &wh = whs[threadIdx.x >> 6];
@@ -536,7 +538,11 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
       const double pre[3] = {t.pos[0], t.pos[1], t.pos[2]};
       const int prevol = t.vol;
       double step = 0.0;
-      const int status = propagate<R>(P, sp, t.e, t.pos, t.dir, t.vol, bestd, step);
+      int status;
+      if constexpr (kGeneral)
+        status = propagate_general<R>(P, sp, t.e, t.pos, t.dir, t.vol, bestd, step);
+      else
+        status = propagate<R>(P, sp, t.e, t.pos, t.dir, t.vol, bestd, step);
       if (xs0 > R(0)) t.nil0 -= (R)step * xs0;
       if (xs1 > R(0)) t.nil1 -= (R)step * xs1;
       if (xs2 > R(0)) t.nil2 -= (R)step * xs2;
@@ -941,6 +947,36 @@ __global__ void k_helix(double b0, double b1, double b2, const double *charge, c
   }
 }
 
+__global__ void k_field_value(DevField F, const double *pos, double *b, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double B[3];
+  map_field(F, pos[i], pos[n + i], pos[2 * n + i], B);
+  for (int k = 0; k < 3; ++k) b[k * n + i] = B[k];
+}
+
+template <class R>
+__global__ void k_field_step(DevField F, const double *charge, const double *mom, const double *step, double *pos,
+                             double *dir, double *err, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
+  R d[3] = {(R)dir[i], (R)dir[n + i], (R)dir[2 * n + i]};
+  double B[3];
+  field_at(F, p, B);
+  const double bm = sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+  double ex = 0.0, eu = 0.0;
+  field_advance<R>(F, B, bm, (R)charge[i], (R)mom[i], step[i], p, d, ex, eu);
+  for (int k = 0; k < 3; ++k) {
+    pos[k * n + i] = p[k];
+    dir[k * n + i] = (double)d[k];
+  }
+  if (err) {
+    err[i] = ex;
+    err[n + i] = eu;
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
 
@@ -962,7 +998,10 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
   hipLaunchKernelGGL((k_derive<R, 0>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
   if (ev && ev[3]) (void)hipEventRecord(ev[3], s.main);
   if (ev && ev[4]) (void)hipEventRecord(ev[4], s.side);
-  hipLaunchKernelGGL((k_transport<R, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
+  if (a.general_field)
+    hipLaunchKernelGGL((k_transport<R, true, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
+  else
+    hipLaunchKernelGGL((k_transport<R, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
   if (ev && ev[5]) (void)hipEventRecord(ev[5], s.side);
   hipLaunchKernelGGL((k_interact<R, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
   if (ev && ev[6]) (void)hipEventRecord(ev[6], s.side);
@@ -1040,6 +1079,22 @@ hipError_t launch_helix(const double B[3], int precision, const double *charge, 
   else
     hipLaunchKernelGGL(k_helix<double>, dim3(grid_for(n)), dim3(kBlock), 0, st, B[0], B[1], B[2], charge, mom, step,
                        pos, dir, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_field_value(const DevField &F, const double *pos, double *b, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_field_value, dim3(grid_for(n)), dim3(kBlock), 0, st, F, pos, b, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_field_step(const DevField &F, int precision, const double *charge, const double *mom,
+                             const double *step, double *pos, double *dir, double *err, int n, hipStream_t st) {
+  if (precision == GXT_FP32)
+    hipLaunchKernelGGL(k_field_step<float>, dim3(grid_for(n)), dim3(kBlock), 0, st, F, charge, mom, step, pos, dir,
+                       err, n);
+  else
+    hipLaunchKernelGGL(k_field_step<double>, dim3(grid_for(n)), dim3(kBlock), 0, st, F, charge, mom, step, pos, dir,
+                       err, n);
   return hipGetLastError();
 }
 

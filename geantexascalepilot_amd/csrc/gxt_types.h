@@ -71,12 +71,25 @@ struct DevGeometry {
   int visible[kMaxVol];
 };
 
+// Magnetic field seen by the general (non-uniform or Runge-Kutta)
+// propagation: a cylindrical (r, z) map, or the uniform field b.
+struct DevField {
+  const double *v;         // (Br, Bphi, Bz) in tesla at (ir * nz + iz) * 3; null: uniform b
+  int nr, nz;
+  double r0, z0, dr, dz;   // first node and spacing (mm)
+  double origin[3];        // setup origin in the map frame (mm)
+  double b[3];             // uniform field (tesla) when v is null
+  int stepper;             // GXT_STEPPER_*
+  int general;             // e-/e+ take the general propagation (a map, or RK45)
+};
+
 // Everything a kernel needs besides the pool: lives in one device buffer.
 struct DevParams {
   DevMaterial mat[kMaxSlots];
   DevGeometry geo;
   double B[3];
   double bmag;
+  DevField field;
   double tracking_cut;
   double zbin, rbin;
   int nz, nr;

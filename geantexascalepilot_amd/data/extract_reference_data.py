@@ -6,7 +6,9 @@ The reference keeps its Geant4-dumped physics tables as ASCII data under
 and its Sandia photo-absorption coefficients and atomic-shell binding energies
 as numeric arrays in ``proxy/ProxySandiaData.hpp`` (``fSandiaTable``,
 ``fNbOfIntervals``, ``fZtoAratio``, ``fNumberOfShells``, ``fIndexOfShells``,
-``fBindingEnergies``).  This script copies the table files verbatim and writes
+``fBindingEnergies``); the CMS 2015 solenoid field map is
+``examples/magneticfield/simplifiedCMS/cmsmagfield2015.txt``.  This script
+copies the table files and the field map verbatim and writes
 the numeric arrays, for the elements the configured materials use, into plain
 text files beside it.  It only runs where the reference tree is present; the
 files it writes are committed so the library never needs the reference.
@@ -90,6 +92,12 @@ def main(ref):
         for z in ELEMENTS:
             vals = be[ish[z]:ish[z] + nsh[z]]
             f.write("%d %d %s\n" % (z, nsh[z], " ".join("%.6g" % v for v in vals)))
+
+    # 4. CMS (r, z) field map (181 x 161 nodes, tesla).
+    fout = os.path.join(here, "field")
+    os.makedirs(fout, exist_ok=True)
+    shutil.copyfile(os.path.join(ref, "examples/magneticfield/simplifiedCMS/cmsmagfield2015.txt"),
+                    os.path.join(fout, "cmsmagfield2015.txt"))
 
 
 if __name__ == "__main__":

@@ -105,6 +105,58 @@ def helix_step(field, charge, mom, step, pos, dirs, precision=abi.FP64):
     return pos, dirs
 
 
+class FieldMap:
+    """A cylindrical (r, z) field map resident on the current HIP device."""
+
+    def __init__(self, path=abi.CMS_FIELD_MAP):
+        self.h = C.c_void_p()
+        _check(lib.gxt_field_map_load(os.fsencode(path), C.byref(self.h)))
+
+    def info(self):
+        nr, nz = C.c_int(), C.c_int()
+        d = [C.c_double() for _ in range(5)]
+        _check(lib.gxt_field_map_info(self.h, C.byref(nr), C.byref(nz), *[C.byref(x) for x in d]))
+        return dict(nr=nr.value, nz=nz.value, r0=d[0].value, z0=d[1].value, dr=d[2].value, dz=d[3].value,
+                    bmax=d[4].value)
+
+    def value(self, pos):
+        """Cartesian field (tesla) at points of the map frame; pos: [3, n] (SoA)."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        n = pos.shape[1]
+        b = np.zeros((3, n))
+        _check(lib.gxt_field_map_value(self.h, abi.ptr(pos), abi.ptr(b), n))
+        return b
+
+    def close(self):
+        if self.h:
+            _check(lib.gxt_field_map_free(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def field_step(field_map, field, charge, mom, step, pos, dirs, stepper=abi.STEPPER_HELIX, precision=abi.FP64,
+               origin=(0.0, 0.0, 0.0)):
+    """One propagation sub-step per track in a map (field_map) or a uniform field; returns (pos, dir, err)."""
+    charge = np.ascontiguousarray(charge, dtype=np.float64)
+    mom = np.ascontiguousarray(mom, dtype=np.float64)
+    step = np.ascontiguousarray(step, dtype=np.float64)
+    pos = np.ascontiguousarray(pos, dtype=np.float64).copy()
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64).copy()
+    n = len(charge)
+    err = np.zeros(2 * n)
+    f = None if field is None else (C.c_double * 3)(*field)
+    o = (C.c_double * 3)(*origin)
+    h = field_map.h if field_map is not None else None
+    _check(lib.gxt_field_step(h, f, o, stepper, precision, abi.ptr(charge), abi.ptr(mom), abi.ptr(step),
+                              abi.ptr(pos), abi.ptr(dirs), abi.ptr(err), n))
+    return pos, dirs, err.reshape(2, n)
+
+
 class Transport:
     """One device context: physics tables, track pool, tallies, one HIP stream."""
 

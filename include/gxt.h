@@ -42,6 +42,19 @@ extern "C" {
 #define GXT_FP64 0
 #define GXT_FP32 1 /* precision form: FP32 tables/sampling/helix, FP64 position+tallies */
 
+/* Charged-track steppers.  HELIX: a helix in the field at the start of each
+ * propagation sub-step (the reference's ConstFieldHelixStepper driven by
+ * FieldLookup, FieldPropagationHandler.cpp:101-111,247-253; exact in a
+ * uniform field).  RK45: Cash-Karp embedded Runge-Kutta 4(5) on the Lorentz
+ * equation, the field evaluated at every stage, sub-steps of at most
+ * GXT_RK_ANGLE radians of turning (the integrator the reference reaches
+ * through GUFieldPropagator::DoStep under ENABLE_MORE_COMPLEX_FIELD,
+ * FieldPropagationHandler.cpp:201-208, and checks the helix against in
+ * magneticfield/test/testConstFieldStepper.cpp). */
+#define GXT_STEPPER_HELIX 0
+#define GXT_STEPPER_RK45 1
+#define GXT_RK_ANGLE 0.25
+
 typedef struct gxt_config {
   int32_t geometry;         /* GXT_GEOM_*                                           */
   int32_t material;         /* box material (GXT_MAT_*)                             */
@@ -72,6 +85,12 @@ typedef struct gxt_config {
                                e-/e+ transport launches (the dominant kernel; 2 events per
                                iteration instead of 10) */
   const char *data_dir;     /* directory holding the physics data files             */
+  const char *field_map;    /* NULL: the uniform field above; else a cylindrical (r, z)
+                               field-map file (see gxt_field_map_load), which replaces it */
+  double field_origin[3];   /* position of the setup origin in the map frame (mm)   */
+  int32_t field_stepper;    /* GXT_STEPPER_* for e-/e+ (a uniform field with HELIX
+                               takes the exact constant-field path)                  */
+  int32_t pad1;
 } gxt_config;
 
 /* One primary particle. */
@@ -295,6 +314,41 @@ int gxt_sample(gxt_ctx *ctx, int model, int material_slot, int precision, gxt_sa
  * mm/MeV/tesla instead of cm/GeV/kilogauss).  pos, dir: [3n] SoA, updated in place. */
 int gxt_helix_step(const double field[3], int precision, const double *charge,
                    const double *momentum, const double *step, double *pos, double *dir, int n);
+
+/* ---- non-uniform field ------------------------------------------------- */
+/* Cylindrical (r, z) field map on a regular grid, resident in device memory
+ * of the current HIP device.  The file has one node per line,
+ *   "R(mm)= r phi(deg)= 0 Z(mm)= z Bz(tesla)= bz Br(tesla)= br Bphi(tesla)= bphi",
+ * r-major (all z of the first r, then the next r), r and z ascending with
+ * constant spacing (the CMS 2015 map: 181 x 161 nodes, r <= 9 m, |z| <= 16 m).
+ * Replaces ScalarCMSmagField::ReadVectorData
+ * (examples/magneticfield/simplifiedCMS/inc/Geant/ScalarCMSmagField.h:174-206),
+ * which assumes the 181 x 161 CMS grid; this reader takes the grid from the
+ * file and rejects irregular or truncated files with GXT_EDATA. */
+typedef struct gxt_field_map gxt_field_map;
+int gxt_field_map_load(const char *path, gxt_field_map **out);
+int gxt_field_map_free(gxt_field_map *map);
+/* Grid: nr x nz nodes from (r0, z0) with spacing (dr, dz) in mm; bmax: the
+ * largest |B| of the nodes (tesla). */
+int gxt_field_map_info(const gxt_field_map *map, int *nr, int *nz, double *r0, double *z0, double *dr,
+                       double *dz, double *bmax);
+/* Cartesian field (tesla) at n points of the map frame: bilinear in (r, z)
+ * inside the grid cell, r and z clamped to the grid, cylindrical components
+ * turned with the point's azimuth (on the axis: Bx = Br, By = Bphi).
+ * Replaces ScalarCMSmagField::GetFieldValue / GetFieldValueRZ
+ * (ScalarCMSmagField.h:208-331).  pos, b: [3n] SoA. */
+int gxt_field_map_value(const gxt_field_map *map, const double *pos, double *b, int n);
+/* Advances n charged tracks by step[i] (mm) with the stepper the transport
+ * uses for one propagation sub-step: HELIX, a helix in the field at the start
+ * point; RK45, ceil(step k |B(start)| / GXT_RK_ANGLE) equal Cash-Karp steps
+ * (k = 0.299792458 / momentum), direction renormalised after each.  The field
+ * is the map at pos + origin (map != NULL) or the uniform field[] (map ==
+ * NULL).  pos, dir: [3n] SoA, updated in place.  err (NULL or [2n]): summed
+ * Cash-Karp error estimates, position (mm) at [i] and direction at [n+i]
+ * (0 for HELIX).  Steps of more than 2^16 RK45 sub-steps are rejected. */
+int gxt_field_step(const gxt_field_map *map, const double field[3], const double origin[3], int stepper,
+                   int precision, const double *charge, const double *momentum, const double *step,
+                   double *pos, double *dir, double *err, int n);
 
 #ifdef __cplusplus
 }

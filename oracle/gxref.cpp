@@ -646,6 +646,213 @@ static void helix_step(const double B[3], double bmag, double charge, double mom
   unit3(dir);
 }
 
+// ---------------------------------------------------------------- field map
+// Cylindrical (r, z) field map (ScalarCMSmagField,
+// examples/magneticfield/simplifiedCMS/inc/Geant/ScalarCMSmagField.h): nodes
+// r-major, (Br, Bphi, Bz) in tesla per node.
+struct FieldMap {
+  int nr = 0, nz = 0;
+  double r0 = 0, z0 = 0, dr = 0, dz = 0, bmax = 0;
+  std::vector<double> v;
+};
+
+// ScalarCMSmagField::ReadVectorData (ScalarCMSmagField.h:174-206): one node
+// per line, "R(mm)= r phi(deg)= phi Z(mm)= z Bz(tesla)= bz Br(tesla)= br
+// Bphi(tesla)= bphi".  The grid (181 x 161, 50 mm x 200 mm from (0, -16 m) in
+// the reference) is taken from the file: nz = the lines sharing the first r,
+// spacings from the first differences; a line off the grid, an unparsable
+// line or a count that is not nr * nz is an error.
+static int read_field_map(const char *path, FieldMap &m) {
+  std::ifstream in(path ? path : "");
+  if (!path || !in) return fail(GXT_EDATA, std::string("cannot open field map ") + (path ? path : "(null)"));
+  std::vector<double> r, z, b;
+  std::string line;
+  long lineno = 0;
+  while (std::getline(in, line)) {
+    ++lineno;
+    if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+    char l0[32], l1[32], l2[32], l3[32], l4[32], l5[32], tail[8];
+    double rv, pv, zv, bz, br, bp;
+    const int k = std::sscanf(line.c_str(), "%31s %lf %31s %lf %31s %lf %31s %lf %31s %lf %31s %lf %7s", l0, &rv, l1,
+                              &pv, l2, &zv, l3, &bz, l4, &br, l5, &bp, tail);
+    if (k != 12 || std::strcmp(l0, "R(mm)=") || std::strcmp(l1, "phi(deg)=") || std::strcmp(l2, "Z(mm)=") ||
+        std::strcmp(l3, "Bz(tesla)=") || std::strcmp(l4, "Br(tesla)=") || std::strcmp(l5, "Bphi(tesla)=") ||
+        !std::isfinite(rv) || !std::isfinite(zv) || !std::isfinite(bz) || !std::isfinite(br) || !std::isfinite(bp))
+      return fail(GXT_EDATA, "malformed field-map line " + std::to_string(lineno));
+    r.push_back(rv);
+    z.push_back(zv);
+    b.insert(b.end(), {br, bp, bz});
+  }
+  const size_t n = r.size();
+  size_t nz = 1;
+  while (nz < n && r[nz] == r[0]) ++nz;
+  if (n < 4 || nz < 2 || nz == n || n % nz) return fail(GXT_EDATA, "field map is not a regular (r, z) grid");
+  const size_t nr = n / nz;
+  const double dr = r[nz] - r[0], dz = z[1] - z[0];
+  if (!(dr > 0) || !(dz > 0) || r[0] < 0) return fail(GXT_EDATA, "field-map grid must start at r >= 0 and ascend");
+  for (size_t i = 0; i < n; ++i) {
+    const double wr = r[0] + (double)(i / nz) * dr, wz = z[0] + (double)(i % nz) * dz;
+    if (std::fabs(r[i] - wr) > 1e-9 * std::max(std::fabs(wr), dr) ||
+        std::fabs(z[i] - wz) > 1e-9 * std::max(std::fabs(wz), dz))
+      return fail(GXT_EDATA, "field-map node " + std::to_string(i + 1) + " is off the regular grid");
+  }
+  m.nr = (int)nr;
+  m.nz = (int)nz;
+  m.r0 = r[0];
+  m.z0 = z[0];
+  m.dr = dr;
+  m.dz = dz;
+  m.bmax = 0;
+  for (size_t i = 0; i < n; ++i)
+    m.bmax = std::max(m.bmax, std::sqrt(b[3 * i] * b[3 * i] + b[3 * i + 1] * b[3 * i + 1] + b[3 * i + 2] * b[3 * i + 2]));
+  m.v.swap(b);
+  return GXT_OK;
+}
+
+// ScalarCMSmagField::GetFieldValueRZ / GetFieldValue (ScalarCMSmagField.h:247-331):
+// r and z clamped to the grid; bilinear area weights of the cell (here as
+// products of the fractional cell position, the same weights divided by the
+// cell area); Bx = Br cos - Bphi sin, By = Br sin + Bphi cos with the point's
+// azimuth.  Fixed here: the cell index is clamped to the last cell (the
+// reference reads past the grid at r = rmax or z = zmax), and the azimuth on
+// the axis is 0 (the reference leaves 1/r unset there).
+static void map_field(const FieldMap &f, double x, double y, double z, double B[3]) {
+  const double r = std::sqrt(x * x + y * y);
+  const double rmax = f.r0 + (double)(f.nr - 1) * f.dr;
+  const double zmax = f.z0 + (double)(f.nz - 1) * f.dz;
+  const double rc = std::fmin(std::fmax(r, f.r0), rmax);
+  const double zc = std::fmin(std::fmax(z, f.z0), zmax);
+  const double fr = (rc - f.r0) / f.dr;
+  const double fz = (zc - f.z0) / f.dz;
+  const int ir = std::min((int)fr, f.nr - 2);
+  const int iz = std::min((int)fz, f.nz - 2);
+  const double tr = fr - (double)ir, tz = fz - (double)iz;
+  const double w00 = (1.0 - tr) * (1.0 - tz), w01 = (1.0 - tr) * tz;
+  const double w10 = tr * (1.0 - tz), w11 = tr * tz;
+  const double *p = f.v.data() + ((size_t)ir * f.nz + iz) * 3;
+  const double *q = p + (size_t)f.nz * 3;
+  double b[3];
+  for (int c = 0; c < 3; ++c) b[c] = w00 * p[c] + w01 * p[3 + c] + w10 * q[c] + w11 * q[3 + c];
+  double cs = 1.0, sn = 0.0;
+  if (r > 0.0) {
+    cs = x / r;
+    sn = y / r;
+  }
+  B[0] = b[0] * cs - b[1] * sn;
+  B[1] = b[0] * sn + b[1] * cs;
+  B[2] = b[2];
+}
+
+// The field a charged track sees: the map at pos + origin, or uniform.
+struct Field {
+  const FieldMap *map = nullptr;
+  double b[3] = {0, 0, 0};
+  double origin[3] = {0, 0, 0};
+  int stepper = GXT_STEPPER_HELIX;
+  void at(const double pos[3], double B[3]) const {
+    if (map) {
+      map_field(*map, pos[0] + origin[0], pos[1] + origin[1], pos[2] + origin[2], B);
+    } else {
+      B[0] = b[0];
+      B[1] = b[1];
+      B[2] = b[2];
+    }
+  }
+};
+
+// Cash-Karp embedded Runge-Kutta 4(5) (J. R. Cash, A. H. Karp, ACM Trans.
+// Math. Softw. 16 (1990) 201-222) on dx/ds = u, du/ds = kq (u x B(x)),
+// kq = -q kB2C / p: six stages, the fifth-order solution propagated, the
+// difference to the fourth-order one as the error estimate.
+static const double kCkA[6][5] = {{0, 0, 0, 0, 0},
+                                  {1.0 / 5.0, 0, 0, 0, 0},
+                                  {3.0 / 40.0, 9.0 / 40.0, 0, 0, 0},
+                                  {3.0 / 10.0, -9.0 / 10.0, 6.0 / 5.0, 0, 0},
+                                  {-11.0 / 54.0, 5.0 / 2.0, -70.0 / 27.0, 35.0 / 27.0, 0},
+                                  {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0}};
+static const double kCkB5[6] = {37.0 / 378.0, 0, 250.0 / 621.0, 125.0 / 594.0, 0, 512.0 / 1771.0};
+static const double kCkB4[6] = {2825.0 / 27648.0, 0, 18575.0 / 48384.0, 13525.0 / 55296.0, 277.0 / 14336.0, 1.0 / 4.0};
+
+static void lorentz(const Field &F, double kq, const double x[3], const double u[3], double f[3]) {
+  double B[3];
+  F.at(x, B);
+  f[0] = kq * (u[1] * B[2] - u[2] * B[1]);
+  f[1] = kq * (u[2] * B[0] - u[0] * B[2]);
+  f[2] = kq * (u[0] * B[1] - u[1] * B[0]);
+}
+
+static void ck45_step(const Field &F, double kq, double h, const double f1[3], double pos[3], double dir[3], double &ex,
+                      double &eu) {
+  double u[6][3], f[6][3];
+  for (int c = 0; c < 3; ++c) {
+    u[0][c] = dir[c];
+    f[0][c] = f1[c];
+  }
+  for (int s = 1; s < 6; ++s) {
+    double x[3];
+    for (int c = 0; c < 3; ++c) {
+      // sums in stage order, as the restatement's arithmetic must follow one fixed order
+      double sx = kCkA[s][0] * u[0][c], su = kCkA[s][0] * f[0][c];
+      for (int j = 1; j < s; ++j) {
+        sx += kCkA[s][j] * u[j][c];
+        su += kCkA[s][j] * f[j][c];
+      }
+      x[c] = pos[c] + h * sx;
+      u[s][c] = dir[c] + h * su;
+    }
+    lorentz(F, kq, x, u[s], f[s]);
+  }
+  double ex2 = 0, eu2 = 0;
+  for (int c = 0; c < 3; ++c) {
+    const int st[5] = {0, 2, 3, 4, 5};  // stages with a nonzero b5 - b4 (stage 2's is 0)
+    double dx = (kCkB5[0] - kCkB4[0]) * u[0][c], du = (kCkB5[0] - kCkB4[0]) * f[0][c];
+    for (int k = 1; k < 5; ++k) {
+      dx += (kCkB5[st[k]] - kCkB4[st[k]]) * u[st[k]][c];
+      du += (kCkB5[st[k]] - kCkB4[st[k]]) * f[st[k]][c];
+    }
+    dx *= h;
+    du *= h;
+    ex2 += dx * dx;
+    eu2 += du * du;
+    const int sb[4] = {0, 2, 3, 5};  // stages with a nonzero b5
+    double sx = kCkB5[0] * u[0][c], su = kCkB5[0] * f[0][c];
+    for (int k = 1; k < 4; ++k) {
+      sx += kCkB5[sb[k]] * u[sb[k]][c];
+      su += kCkB5[sb[k]] * f[sb[k]][c];
+    }
+    pos[c] = pos[c] + h * sx;
+    dir[c] = dir[c] + h * su;
+  }
+  unit3(dir);
+  ex += std::sqrt(ex2);
+  eu += std::sqrt(eu2);
+}
+
+static const int kRkMaxSub = 16;
+static const int kRkMaxN = 65537;
+
+// One propagation sub-step of length s: helix in the start-point field B, or
+// RK45 in ceil(s k |B| / GXT_RK_ANGLE) equal Cash-Karp steps (k = 0.299792458 / p).
+static void field_advance(const Field &F, const double B[3], double bm, double q, double mom, double s, double pos[3],
+                          double dir[3], double &ex, double &eu) {
+  if (F.stepper == GXT_STEPPER_RK45) {
+    const double turn = s * (-kB2C) * bm / mom;
+    const int n = turn > GXT_RK_ANGLE ? (int)std::fmin(std::ceil(turn / GXT_RK_ANGLE), (double)kRkMaxN) : 1;
+    const double h = s / (double)n;
+    const double kq = -q * kB2C / mom;
+    double f1[3] = {kq * (dir[1] * B[2] - dir[2] * B[1]), kq * (dir[2] * B[0] - dir[0] * B[2]),
+                    kq * (dir[0] * B[1] - dir[1] * B[0])};
+    for (int k = 0; k < n; ++k) {
+      if (k > 0) lorentz(F, kq, pos, dir, f1);
+      ck45_step(F, kq, h, f1, pos, dir, ex, eu);
+    }
+  } else if (bm > 0.0) {
+    helix_step(B, bm, q, mom, s, pos, dir);
+  } else {
+    for (int c = 0; c < 3; ++c) pos[c] += s * dir[c];
+  }
+}
+
 // ---------------------------------------------------------------- physics
 struct Secondary {
   int species;
@@ -1175,6 +1382,9 @@ struct gxref_ctx {
   Physics phys;
   Geometry geo;
   double bfield[3], bmag;
+  FieldMap fmap;
+  Field field;          // map or uniform field seen by e-/e+ on the general path
+  bool general = false; // e-/e+ propagate through propagate_general (a map, or RK45)
   uint32_t seed[6];
   std::vector<EventTally> events;
   Worker tot;
@@ -1263,9 +1473,74 @@ static void trace_step(const gxref_ctx &c, Worker &w, const Track &t, int proc, 
 // FieldPropagationHandler.cpp:87-360; LinearPropagationHandler::Propagate,
 // core/src/LinearPropagationHandler.cpp:13-63; driver loop
 // proxy/test/test_BasicCpuTransport.cpp:319-336).
+// Propagation of an e-/e+ in a general field (a map, or the RK45 stepper):
+// the sub-step choice of the uniform case below with the field looked up at
+// the start of every sub-step (FieldPropagationHandler::Propagate,
+// FieldPropagationHandler.cpp:93-111: step min(pstep or snext, max(SafeLength
+// in the local field, safety))); RK45 sub-steps also kept within kRkMaxSub
+// Cash-Karp steps.
+static int propagate_general(const gxref_ctx &c, Track &t, double pstep, double &step) {
+  const Geometry &g = c.geo;
+  const Field &F = c.field;
+  const double q = t.species == 1 ? -1.0 : 1.0;
+  const double mom = std::sqrt(t.e * (t.e + 2.0 * kMe));
+  const bool rk = F.stepper == GXT_STEPPER_RK45;
+  double ex = 0.0, eu = 0.0;
+  double remaining = pstep;
+  step = 0.0;
+  for (int it = 0; it < 64; ++it) {
+    const double saf = safety(g, t.vol, t.pos);
+    double B[3];
+    F.at(t.pos, B);
+    const double bm = std::sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+    const double smax = rk && bm > 0.0 ? kRkMaxSub * GXT_RK_ANGLE * mom / ((-kB2C) * bm) : kHuge;
+    if (remaining <= saf && remaining <= smax) {
+      field_advance(F, B, bm, q, mom, remaining, t.pos, t.dir, ex, eu);
+      step += remaining;
+      return kPhysics;
+    }
+    int face;
+    const double slin = dist_out(g, t.vol, t.pos, t.dir, face);
+    double sl = kHuge;
+    if (bm > 0.0) {
+      const double pl = mom * (t.dir[0] * (B[0] / bm) + t.dir[1] * (B[1] / bm) + t.dir[2] * (B[2] / bm));
+      double pt2 = mom * mom - pl * pl;
+      if (pt2 < 0.0) pt2 = 0.0;
+      const double pt = std::sqrt(pt2);
+      if (pt > 0.0) sl = 2.0 * std::sqrt(0.1 / (0.299792458 * bm / pt));
+    }
+    if (slin <= sl && slin <= remaining && slin <= smax) {
+      const double s = std::max(slin, 1.0e-6);
+      field_advance(F, B, bm, q, mom, s, t.pos, t.dir, ex, eu);
+      step += s;
+      if (face == 0 || face == 1) {
+        const int nv = face == 1 ? t.vol + 1 : t.vol - 1;
+        t.pos[2] = g.z[face == 1 ? t.vol + 1 : t.vol];
+        if (nv < 0 || nv >= g.nvol() || std::fabs(t.pos[0]) > g.hx || std::fabs(t.pos[1]) > g.hy) return kEscaped;
+        t.vol = nv;
+        return kBoundary;
+      }
+      return kEscaped;
+    }
+    const double s = std::min(std::min(remaining, std::max(sl, saf)), smax);
+    field_advance(F, B, bm, q, mom, s, t.pos, t.dir, ex, eu);
+    step += s;
+    remaining -= s;
+    const int v2 = locate(g, t.pos, t.dir);
+    if (v2 != t.vol) {
+      if (v2 < 0) return kEscaped;
+      t.vol = v2;
+      return kBoundary;
+    }
+    if (remaining <= 0.0) return kPhysics;
+  }
+  return kLimit;
+}
+
 static int propagate(const gxref_ctx &c, Track &t, double pstep, double &step) {
   const Geometry &g = c.geo;
   const bool charged = t.species != 0;
+  if (charged && c.general) return propagate_general(c, t, pstep, step);
   if (!charged || c.bmag == 0.0) {
     int face;
     const double sn = dist_out(g, t.vol, t.pos, t.dir, face);
@@ -1494,6 +1769,25 @@ int gxref_create(const gxt_config *cfg, gxref_ctx **out) {
   c->bfield[1] = cfg->field[1];
   c->bfield[2] = cfg->field[2];
   c->bmag = std::sqrt(cfg->field[0] * cfg->field[0] + cfg->field[1] * cfg->field[1] + cfg->field[2] * cfg->field[2]);
+  if (cfg->field_stepper != GXT_STEPPER_HELIX && cfg->field_stepper != GXT_STEPPER_RK45) {
+    delete c;
+    return fail(GXT_EINVAL, "bad field_stepper");
+  }
+  if (cfg->field_map) {
+    const int rc = read_field_map(cfg->field_map, c->fmap);
+    if (rc) {
+      delete c;
+      return rc;
+    }
+    c->field.map = &c->fmap;
+  }
+  c->cfg.field_map = nullptr;
+  for (int k = 0; k < 3; ++k) {
+    c->field.b[k] = cfg->field[k];
+    c->field.origin[k] = cfg->field_origin[k];
+  }
+  c->field.stepper = cfg->field_stepper;
+  c->general = cfg->field_map != nullptr || cfg->field_stepper == GXT_STEPPER_RK45;
   c->events.assign(std::max(cfg->max_events, 1), EventTally());
   c->tot.ez.assign(cfg->n_zbins, 0.0);
   c->tot.er.assign(cfg->n_rbins, 0.0);
@@ -1842,3 +2136,82 @@ int gxref_rotate_to_lab(double *u, double *v, double *w, const double *u1, const
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- field entry points
+struct gxref_field_map {
+  FieldMap m;
+};
+
+int gxref_field_map_load(const char *path, gxref_field_map **out) {
+  if (!out) return fail(GXT_EINVAL, "null output");
+  auto *f = new gxref_field_map();
+  const int rc = read_field_map(path, f->m);
+  if (rc) {
+    delete f;
+    return rc;
+  }
+  *out = f;
+  return GXT_OK;
+}
+
+int gxref_field_map_free(gxref_field_map *map) {
+  delete map;
+  return GXT_OK;
+}
+
+int gxref_field_map_info(const gxref_field_map *map, int *nr, int *nz, double *r0, double *z0, double *dr, double *dz,
+                         double *bmax) {
+  if (!map) return fail(GXT_EINVAL, "null field map");
+  const FieldMap &m = map->m;
+  *nr = m.nr;
+  *nz = m.nz;
+  *r0 = m.r0;
+  *z0 = m.z0;
+  *dr = m.dr;
+  *dz = m.dz;
+  *bmax = m.bmax;
+  return GXT_OK;
+}
+
+int gxref_field_map_value(const gxref_field_map *map, const double *pos, double *b, int n) {
+  if (!map || !pos || !b || n <= 0) return fail(GXT_EINVAL, "bad field-map value arguments");
+  for (int i = 0; i < n; ++i) {
+    double B[3];
+    map_field(map->m, pos[i], pos[n + i], pos[2 * n + i], B);
+    for (int k = 0; k < 3; ++k) b[k * n + i] = B[k];
+  }
+  return GXT_OK;
+}
+
+int gxref_field_step(const gxref_field_map *map, const double field[3], const double origin[3], int stepper,
+                     const double *charge, const double *mom, const double *step, double *pos, double *dir,
+                     double *err, int n) {
+  if ((!map && !field) || !charge || !mom || !step || !pos || !dir || n <= 0)
+    return fail(GXT_EINVAL, "bad field-step arguments");
+  if (stepper != GXT_STEPPER_HELIX && stepper != GXT_STEPPER_RK45) return fail(GXT_EINVAL, "bad stepper");
+  Field F;
+  F.map = map ? &map->m : nullptr;
+  for (int k = 0; k < 3; ++k) {
+    F.b[k] = field ? field[k] : 0.0;
+    F.origin[k] = origin ? origin[k] : 0.0;
+  }
+  F.stepper = stepper;
+  for (int i = 0; i < n; ++i) {
+    double p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
+    double d[3] = {dir[i], dir[n + i], dir[2 * n + i]};
+    double B[3];
+    F.at(p, B);
+    const double bm = std::sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+    double ex = 0.0, eu = 0.0;
+    field_advance(F, B, bm, charge[i], mom[i], step[i], p, d, ex, eu);
+    for (int k = 0; k < 3; ++k) {
+      pos[k * n + i] = p[k];
+      dir[k * n + i] = d[k];
+    }
+    if (err) {
+      err[i] = ex;
+      err[n + i] = eu;
+    }
+  }
+  return GXT_OK;
+}

@@ -58,6 +58,16 @@ int gxref_macro_xs(gxref_ctx *ctx, int material_slot, int species, int process, 
 int gxref_sample(gxref_ctx *ctx, int model, int material_slot, gxt_sample_io *io);
 int gxref_helix_step(const double field[3], const double *charge, const double *momentum,
                      const double *step, double *pos, double *dir, int n);
+/* Field map and field stepping, same contract as gxt_field_map_* / gxt_field_step (FP64). */
+typedef struct gxref_field_map gxref_field_map;
+int gxref_field_map_load(const char *path, gxref_field_map **out);
+int gxref_field_map_free(gxref_field_map *map);
+int gxref_field_map_info(const gxref_field_map *map, int *nr, int *nz, double *r0, double *z0, double *dr,
+                         double *dz, double *bmax);
+int gxref_field_map_value(const gxref_field_map *map, const double *pos, double *b, int n);
+int gxref_field_step(const gxref_field_map *map, const double field[3], const double origin[3], int stepper,
+                     const double *charge, const double *momentum, const double *step, double *pos, double *dir,
+                     double *err, int n);
 int gxref_rotate_to_lab(double *u, double *v, double *w, const double *u1, const double *u2,
                         const double *u3, int n);
 /* Sandia photo-absorption macroscopic cross-section (1/mm). */

@@ -201,6 +201,49 @@ def helix_step(field, charge, mom, step, pos, dirs):
     return pos, dirs
 
 
+class FieldMap:
+    """Oracle copy of a cylindrical (r, z) field map (host memory)."""
+
+    def __init__(self, path=abi.CMS_FIELD_MAP):
+        self.h = C.c_void_p()
+        check(lib().gxref_field_map_load(os.fsencode(path), C.byref(self.h)))
+
+    def info(self):
+        nr, nz = C.c_int(), C.c_int()
+        d = [C.c_double() for _ in range(5)]
+        check(lib().gxref_field_map_info(self.h, C.byref(nr), C.byref(nz), *[C.byref(x) for x in d]))
+        return dict(nr=nr.value, nz=nz.value, r0=d[0].value, z0=d[1].value, dr=d[2].value, dz=d[3].value,
+                    bmax=d[4].value)
+
+    def value(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        n = pos.shape[1]
+        b = np.zeros((3, n))
+        check(lib().gxref_field_map_value(self.h, abi.ptr(pos), abi.ptr(b), n))
+        return b
+
+    def __del__(self):
+        if self.h and _lib is not None:
+            _lib.gxref_field_map_free(self.h)
+            self.h = C.c_void_p()
+
+
+def field_step(field_map, field, charge, mom, step, pos, dirs, stepper=abi.STEPPER_HELIX, origin=(0.0, 0.0, 0.0)):
+    charge = np.ascontiguousarray(charge, dtype=np.float64)
+    mom = np.ascontiguousarray(mom, dtype=np.float64)
+    step = np.ascontiguousarray(step, dtype=np.float64)
+    pos = np.ascontiguousarray(pos, dtype=np.float64).copy()
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64).copy()
+    n = len(charge)
+    err = np.zeros(2 * n)
+    f = None if field is None else (C.c_double * 3)(*field)
+    o = (C.c_double * 3)(*origin)
+    h = field_map.h if field_map is not None else None
+    check(lib().gxref_field_step(h, f, o, stepper, abi.ptr(charge), abi.ptr(mom), abi.ptr(step), abi.ptr(pos),
+                                 abi.ptr(dirs), abi.ptr(err), n))
+    return pos, dirs, err.reshape(2, n)
+
+
 def rotate_to_lab(u, v, w, u1, u2, u3):
     arrs = [np.ascontiguousarray(a, dtype=np.float64).copy() for a in (u, v, w)]
     ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (u1, u2, u3)]
