@@ -114,6 +114,49 @@ def precision_form(g, abi, n_events=20000, seed=2468):
                 track_steps=dict(fp64=a["steps"], fp32=b["steps"]))
 
 
+def field_forms(g, abi, n_events=20000, seed=2468, cpu=False):
+    """The calorimeter workload of the precision form (FP64) in three fields:
+    uniform 4 T along +x with the exact helix; the CMS 2015 solenoid map with
+    the calorimeter's front face at r = 600 mm, z = 6 m, with the helix in the
+    local field and with Cash-Karp RK45; and the uniform field with RK45.
+    Throughput and device time per track-step of each."""
+    prims = g.make_primaries(seed, abi.ELECTRON, 1000.0, 50000.0, 10.0, n_events)
+    forms = [("uniform_helix", lambda: abi.config_calorimeter(max_events=n_events)),
+             ("cms_map_helix", lambda: abi.config_calorimeter_cms(stepper=abi.STEPPER_HELIX, max_events=n_events)),
+             ("cms_map_rk45", lambda: abi.config_calorimeter_cms(stepper=abi.STEPPER_RK45, max_events=n_events)),
+             ("uniform_rk45", lambda: abi.config_calorimeter(field_stepper=abi.STEPPER_RK45, max_events=n_events))]
+    out = dict(workload="e- 1-50 GeV x %d into 50 x (2.3 mm Pb + 5.7 mm lAr), FP64" % n_events)
+    for name, make in forms:
+        with g.Transport(make()) as t:
+            t0 = time.perf_counter()
+            t.submit(prims, 1)
+            t.run()
+            dt = time.perf_counter() - t0
+            ev = t.event_tallies(1, n_events)
+            st = t.stats()
+        out[name] = dict(events_per_s=n_events / dt, seconds=dt, track_steps=int(st.track_steps),
+                         ns_per_track_step=1e9 * dt / max(int(st.track_steps), 1),
+                         visible_mean_mev=float(ev["visible"].mean()))
+        log("field %s: %.2f s, %d track-steps" % (name, dt, st.track_steps))
+    if cpu:
+        # the oracle (FP64, host threads) on the first primaries of the same batch in the map
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_harness as O
+        threads = max(1, min(16, os.cpu_count() or 1))
+        n = 16 * threads
+        o = O.Oracle(abi.config_calorimeter_cms(stepper=abi.STEPPER_HELIX, max_events=n))
+        p = O.make_primaries(seed, abi.ELECTRON, 1000.0, 50000.0, 10.0, n)
+        t0 = time.perf_counter()
+        o.run(p, threads)
+        dt = time.perf_counter() - t0
+        steps = int(o.stats().track_steps)
+        o.close()
+        out["cpu_oracle_cms_map_helix"] = dict(events_per_s=n / dt, cores=threads, track_steps_per_s=steps / dt,
+                                               sample="first %d primaries, %d track-steps, %.1f s" % (n, steps, dt))
+        log("field cpu oracle: %d events on %d threads in %.2f s" % (n, threads, dt))
+    return out
+
+
 def serving(g, abi, seconds, rate_jobs_s, seed=4242):
     """Poisson stream of jobs (40 % one 1 GeV gamma, 40 % 10 x 10 GeV e-,
     20 % 100 x 50 GeV e-) into the PbWO4 block, FP32 form, one shared pool."""
@@ -205,6 +248,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp32", action="store_true")
     ap.add_argument("--skip-precision", action="store_true", help="skip the FP32-vs-FP64 calorimeter comparison")
+    ap.add_argument("--skip-field", action="store_true", help="skip the field-map and RK45 calorimeter runs")
     ap.add_argument("--pool", type=int, default=2_000_000, help="live-track pool capacity")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a kernel of known traffic that "
@@ -276,6 +320,8 @@ def main():
                            edep_rel_diff_vs_fp64=float(e32.sum() / e64.sum() - 1.0))
     if not a.skip_precision:
         out["precision_form"] = precision_form(g, abi)
+    if not a.skip_field:
+        out["field_forms"] = field_forms(g, abi, cpu=not a.skip_cpu)
     if not a.skip_serving:
         gev_per_s = (out.get("fp32", {}).get("value", value)) * 10.0
         rate = 0.6 * gev_per_s / (0.4 * 1.0 + 0.4 * 100.0 + 0.2 * 5000.0)
