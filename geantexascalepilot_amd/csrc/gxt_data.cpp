@@ -253,6 +253,14 @@ int load_material(const std::string &data, int which, double lar_density, HostDa
     dt.base = std::log(v.x[0]) / dt.dl;
     dt.dlf = (float)dt.dl;
     dt.basef = (float)dt.base;
+    dt.x0 = v.x.front();
+    dt.y0 = v.y.front();
+    dt.xn = v.x.back();
+    dt.yn = v.y.back();
+    dt.x0f = (float)dt.x0;
+    dt.y0f = (float)dt.y0;
+    dt.xnf = (float)dt.xn;
+    dt.ynf = (float)dt.yn;
     dt.hint = -1;
     dt.nhint = 0;
     dt.hlog0 = 0.0f;

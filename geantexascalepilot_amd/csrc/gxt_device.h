@@ -156,14 +156,14 @@ __device__ __forceinline__ uint64_t child_substream(Rng &r) {
 // ------------------------------------------------------------------ tables
 template <class R> struct Nodes;
 template <> struct Nodes<double> {
-  const double *x, *y, *d2;
+  const Node64 *v;
   const int *hints;
-  __device__ Nodes(const DevParams *P) : x(P->xd), y(P->yd), d2(P->d2d), hints(P->hints) {}
+  __device__ Nodes(const DevParams *P) : v(P->nd), hints(P->hints) {}
 };
 template <> struct Nodes<float> {
-  const float *x, *y, *d2;
+  const Node32 *v;
   const int *hints;
-  __device__ Nodes(const DevParams *P) : x(P->xf), y(P->yf), d2(P->d2f), hints(P->hints) {}
+  __device__ Nodes(const DevParams *P) : v(P->nf), hints(P->hints) {}
 };
 
 template <class R> __device__ __forceinline__ R tdl(const DevTable &t);
@@ -172,29 +172,51 @@ template <> __device__ __forceinline__ float tdl<float>(const DevTable &t) { ret
 template <class R> __device__ __forceinline__ R tbase(const DevTable &t);
 template <> __device__ __forceinline__ double tbase<double>(const DevTable &t) { return t.base; }
 template <> __device__ __forceinline__ float tbase<float>(const DevTable &t) { return t.basef; }
+// first and last node of a vector: (x0, y0), (xn, yn)
+template <class R> __device__ __forceinline__ void tends(const DevTable &t, R &x0, R &y0, R &xn, R &yn);
+template <> __device__ __forceinline__ void tends<double>(const DevTable &t, double &x0, double &y0, double &xn,
+                                                          double &yn) {
+  x0 = t.x0;
+  y0 = t.y0;
+  xn = t.xn;
+  yn = t.yn;
+}
+template <> __device__ __forceinline__ void tends<float>(const DevTable &t, float &x0, float &y0, float &xn,
+                                                         float &yn) {
+  x0 = t.x0f;
+  y0 = t.y0f;
+  xn = t.xnf;
+  yn = t.ynf;
+}
 
-// Cubic spline on a bin (ProxyPhysicsVector.cu:396-429).
-template <class R>
-__device__ __forceinline__ R spline(const Nodes<R> &N, int o, R e) {
-  const R x1 = N.x[o], x2 = N.x[o + 1];
+// Cubic spline between two adjacent nodes (ProxyPhysicsVector.cu:396-429).
+template <class R, class V>
+__device__ __forceinline__ R spline(const V &lo, const V &hi, R e) {
+  const R x1 = lo.x, x2 = hi.x;
   const R delta = x2 - x1;
   const R a = (x2 - e) / delta;
   const R b = (e - x1) / delta;
-  return a * N.y[o] + b * N.y[o + 1] +
-         ((a * a * a - a) * N.d2[o] + (b * b * b - b) * N.d2[o + 1]) * delta * delta / R(6.0);
+  return a * lo.y + b * hi.y + ((a * a * a - a) * lo.d2 + (b * b * b - b) * hi.d2) * delta * delta / R(6.0);
 }
 
 // ProxyPhysicsVector::Value on a log-binned vector (ProxyPhysicsVector.cu:388-394,469-483).
 template <class R>
 __device__ __forceinline__ R pvec_value(const Nodes<R> &N, const DevTable &t, R e) {
   const int o = t.off, n = t.n;
-  if (e <= N.x[o]) return N.y[o];
-  if (e >= N.x[o + n - 1]) return N.y[o + n - 1];
+  R x0, y0, xn, yn;
+  tends<R>(t, x0, y0, xn, yn);
+  if (e <= x0) return y0;
+  if (e >= xn) return yn;
   int bin = (int)(m_log(e) / tdl<R>(t) - tbase<R>(t));
   if (bin > n - 2) bin = n - 2;
   if (bin < 0) bin = 0;
-  if (bin > 0 && e < N.x[o + bin]) --bin;
-  return spline(N, o + bin, e);
+  auto lo = N.v[o + bin];
+  auto hi = N.v[o + bin + 1];
+  if (bin > 0 && e < lo.x) {
+    hi = lo;
+    lo = N.v[o + bin - 1];
+  }
+  return spline<R>(lo, hi, e);
 }
 
 // InvRangeValue (ProxyPhysicsVector.cu:491-524): binary search (the reference calls the
@@ -202,8 +224,10 @@ __device__ __forceinline__ R pvec_value(const Nodes<R> &N, const DevTable &t, R 
 template <class R>
 __device__ __forceinline__ R pvec_inv_value(const Nodes<R> &N, const DevTable &t, R r) {
   const int o = t.off, n = t.n;
-  if (r <= N.x[o]) return N.y[o];
-  if (r >= N.x[o + n - 1]) return N.y[o + n - 1];
+  R x0, y0, xn, yn;
+  tends<R>(t, x0, y0, xn, yn);
+  if (r <= x0) return y0;
+  if (r >= xn) return yn;
   // The binary search ends at the last node below r, j with x[j] < r <= x[j+1].
   // The same node is reached from the hint of the log2 bin one below r's
   // (the FP32 log2 is off by far less than a bin), by stepping up while the
@@ -211,9 +235,9 @@ __device__ __forceinline__ R pvec_inv_value(const Nodes<R> &N, const DevTable &t
   int k = (int)floorf((__log2f((float)r) - t.hlog0) * t.hscale) - 1;
   k = k < 0 ? 0 : (k >= t.nhint ? t.nhint - 1 : k);
   int j = N.hints[t.hint + k];
-  while (j > 0 && !(N.x[o + j] < r)) --j;
-  while (j < n - 2 && N.x[o + j + 1] < r) ++j;
-  return spline(N, o + j, r);
+  while (j > 0 && !(N.v[o + j].x < r)) --j;
+  while (j < n - 2 && N.v[o + j + 1].x < r) ++j;
+  return spline<R>(N.v[o + j], N.v[o + j + 1], r);
 }
 
 // Seltzer-Berger grid: x[32], y[57], v[57][32] at offset sb.

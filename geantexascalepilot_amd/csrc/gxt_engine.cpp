@@ -61,23 +61,21 @@ hipError_t dalloc(T **p, size_t n) {
 
 struct DeviceData {
   DevParams *P = nullptr;
-  double *xd = nullptr, *yd = nullptr, *d2d = nullptr, *sbd = nullptr, *sandia = nullptr, *shells = nullptr;
-  float *xf = nullptr, *yf = nullptr, *d2f = nullptr, *sbf = nullptr;
+  Node64 *nd = nullptr;
+  Node32 *nf = nullptr;
+  double *sbd = nullptr, *sandia = nullptr, *shells = nullptr;
+  float *sbf = nullptr;
   uint32_t *jump8 = nullptr;
   int *hints = nullptr;
   void release() {
     (void)hipFree(hints);
     (void)hipFree(P);
     (void)hipFree(jump8);
-    (void)hipFree(xd);
-    (void)hipFree(yd);
-    (void)hipFree(d2d);
+    (void)hipFree(nd);
+    (void)hipFree(nf);
     (void)hipFree(sbd);
     (void)hipFree(sandia);
     (void)hipFree(shells);
-    (void)hipFree(xf);
-    (void)hipFree(yf);
-    (void)hipFree(d2f);
     (void)hipFree(sbf);
     *this = DeviceData();
   }
@@ -95,12 +93,14 @@ int upload(HostData &hd, DeviceData &dd) {
     if (e != hipSuccess) return e;
     return hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice);
   };
-  HIP_TRY(put(&dd.xd, hd.x));
-  HIP_TRY(put(&dd.yd, hd.y));
-  HIP_TRY(put(&dd.d2d, hd.d2));
-  HIP_TRY(put(&dd.xf, tof(hd.x)));
-  HIP_TRY(put(&dd.yf, tof(hd.y)));
-  HIP_TRY(put(&dd.d2f, tof(hd.d2)));
+  std::vector<Node64> n64(hd.x.size());
+  std::vector<Node32> n32(hd.x.size());
+  for (size_t i = 0; i < hd.x.size(); ++i) {
+    n64[i] = Node64{hd.x[i], hd.y[i], hd.d2[i], 0.0};
+    n32[i] = Node32{(float)hd.x[i], (float)hd.y[i], (float)hd.d2[i], 0.0f};
+  }
+  HIP_TRY(put(&dd.nd, n64));
+  HIP_TRY(put(&dd.nf, n32));
   HIP_TRY(put(&dd.sbd, hd.sb));
   HIP_TRY(put(&dd.sbf, tof(hd.sb)));
   HIP_TRY(put(&dd.sandia, hd.sandia));
@@ -116,12 +116,8 @@ int upload(HostData &hd, DeviceData &dd) {
   p.nsandia = (int)hd.sandia.size();
   p.nshells = (int)hd.shells.size();
   p.jump8 = dd.jump8;
-  p.xd = dd.xd;
-  p.yd = dd.yd;
-  p.d2d = dd.d2d;
-  p.xf = dd.xf;
-  p.yf = dd.yf;
-  p.d2f = dd.d2f;
+  p.nd = dd.nd;
+  p.nf = dd.nf;
   p.sbd = dd.sbd;
   p.sbf = dd.sbf;
   p.sandia = dd.sandia;

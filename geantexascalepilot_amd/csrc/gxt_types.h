@@ -24,12 +24,23 @@ constexpr int kInvRangeHints = 256;  // search-hint bins per inverse-range vecto
 // round-robin dispatch of workgroups to XCDs).
 constexpr int kStripes = 8;
 
-// One physics vector inside the node arrays (x, y, d2 at [off, off+n)).
+// One node of a physics vector: abscissa (energy or range), value, second
+// derivative; a spline lookup reads two adjacent nodes, 64 B (32 B in FP32).
+struct alignas(32) Node64 {
+  double x, y, d2, pad;
+};
+struct alignas(16) Node32 {
+  float x, y, d2, pad;
+};
+
+// One physics vector inside the node array (nodes [off, off+n)).
 struct DevTable {
   int off;
   int n;
   double dl, base;  // log-binning: bin = log(E)/dl - base
+  double x0, xn, y0, yn;  // first and last node (the clamps of a lookup)
   float dlf, basef;
+  float x0f, xnf, y0f, ynf;
   // search hints of a non-log vector (inverse range): hints[hint + k] is the
   // last node below x[0] 2^(k / hscale), k < nhint (hint < 0: none)
   int hint, nhint;
@@ -100,8 +111,8 @@ struct DevParams {
   uint32_t seed[6];
   int nsandia, nshells;     // doubles in the Sandia and shell arrays
   const uint32_t *jump8;    // A_c^(d 2^(76+8k)) mod m_c at ((c * 8 + k) * 256 + d) * 9
-  const double *xd, *yd, *d2d;  // node arrays, FP64
-  const float *xf, *yf, *d2f;   // FP32 mirrors
+  const Node64 *nd;             // physics-vector nodes, FP64
+  const Node32 *nf;             // FP32 mirror
   const double *sbd;            // Seltzer-Berger grids, FP64
   const float *sbf;             // FP32 mirror
   const double *sandia;         // Sandia rows (edge keV, a1..a4)
