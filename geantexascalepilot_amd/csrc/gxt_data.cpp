@@ -375,13 +375,13 @@ double HostMrg::uniform() {
 
 void build_jump_tables(const uint32_t jump[2][64][9], std::vector<uint32_t> &out) {
   const uint64_t m[2] = {4294967087ULL, 4294944443ULL};
-  out.assign((size_t)2 * 8 * 256 * 9, 0);
+  out.assign((size_t)2 * 8 * 256 * kJumpRow, 0);
   for (int c = 0; c < 2; ++c)
     for (int k = 0; k < 8; ++k) {
       const uint32_t *J = jump[c][8 * k];
       uint64_t M[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
       for (int d = 0; d < 256; ++d) {
-        uint32_t *o = &out[((size_t)(c * 8 + k) * 256 + d) * 9];
+        uint32_t *o = &out[((size_t)(c * 8 + k) * 256 + d) * kJumpRow];
         for (int i = 0; i < 9; ++i) o[i] = (uint32_t)M[i];
         uint64_t S[9];
         for (int i = 0; i < 3; ++i)

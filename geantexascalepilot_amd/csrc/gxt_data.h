@@ -34,7 +34,8 @@ struct HostMrg {
   void init(uint32_t seed, uint64_t idx);
   double uniform();
 };
-// Byte-wise jump tables: out[((c * 8 + k) * 256 + d) * 9 + i] = A_c^(d 2^(76+8k)) mod m_c.
+// Byte-wise jump tables: out[((c * 8 + k) * 256 + d) * kJumpRow + i] = A_c^(d 2^(76+8k)) mod m_c
+// for i < 9 (rows padded with zeros to kJumpRow words).
 void build_jump_tables(const uint32_t jump[2][64][9], std::vector<uint32_t> &out);
 
 }  // namespace gxt

@@ -88,55 +88,75 @@ struct Rng {
 };
 
 // a * b mod m for a, b < m < 2^32, exactly: the 64-bit product wraps
-// correctly, the FP64 quotient estimate is within one of the true quotient,
-// and one correction each way restores the residue (no 64-bit division).
-__device__ __forceinline__ uint64_t mulmod32(uint64_t a, uint64_t b, uint64_t m, double invm) {
-  const uint64_t p = a * b;
-  const double q = floor((double)a * (double)b * invm);
+// correctly, the FP64 quotient estimate (below 2^32) is within one of the
+// true quotient, and one correction each way restores the residue (no 64-bit
+// division; the operands stay 32-bit, so each conversion is one instruction).
+__device__ __forceinline__ uint32_t mulmod32(uint32_t a, uint32_t b, uint32_t m, double invm) {
+  const uint64_t p = (uint64_t)a * b;
+  const uint32_t q = (uint32_t)floor((double)a * (double)b * invm);
   int64_t r = (int64_t)(p - (uint64_t)q * m);
   if (r < 0) r += (int64_t)m;
   if (r >= (int64_t)m) r -= (int64_t)m;
-  return (uint64_t)r;
+  return (uint32_t)r;
 }
 
-// (M v) mod m for 32-bit residues.
-__device__ __forceinline__ void matvec_mod(const uint32_t *M, uint64_t v[3], uint64_t m, double invm) {
-  uint64_t r[3];
+// (M v) mod m for 32-bit residues (M row-major 3x3).
+__device__ __forceinline__ void matvec_mod(const uint32_t M[9], uint32_t v[3], uint32_t m, double invm) {
+  uint32_t r[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    uint64_t acc = mulmod32(M[i * 3], v[0], m, invm) + mulmod32(M[i * 3 + 1], v[1], m, invm) +
+    uint64_t acc = (uint64_t)mulmod32(M[i * 3], v[0], m, invm) + mulmod32(M[i * 3 + 1], v[1], m, invm) +
                    mulmod32(M[i * 3 + 2], v[2], m, invm);
     if (acc >= m) acc -= m;
     if (acc >= m) acc -= m;
-    r[i] = acc;
+    r[i] = (uint32_t)acc;
   }
   v[0] = r[0];
   v[1] = r[1];
   v[2] = r[2];
 }
 
+// One jump-table row: three 16-byte loads of a 48-byte aligned row.
+__device__ __forceinline__ void jump_row(const uint32_t *T, int c, int k, int d, uint32_t M[9]) {
+  const uint4 *row = reinterpret_cast<const uint4 *>(T + ((c * 8 + k) * 256 + d) * kJumpRow);
+  const uint4 a = row[0], b = row[1], e = row[2];
+  M[0] = a.x;
+  M[1] = a.y;
+  M[2] = a.z;
+  M[3] = a.w;
+  M[4] = b.x;
+  M[5] = b.y;
+  M[6] = b.z;
+  M[7] = b.w;
+  M[8] = e.x;
+}
+
 // Substream idx of the stream seeded with s: (A^(2^76))^idx s, as the
 // product over the 8 bytes d_k of idx of A^(d_k 2^(76+8k)) (powers of one
-// matrix commute).  T[((c * 8 + k) * 256 + d) * 9 + i] holds A_c^(d 2^(76+8k))
-// mod m_c, the identity for d = 0.
+// matrix commute).  T[((c * 8 + k) * 256 + d) * kJumpRow + i], i < 9, holds
+// A_c^(d 2^(76+8k)) mod m_c, the identity for d = 0.  The loop is unrolled
+// so the rows' loads, which depend only on idx, issue ahead of the products.
 __device__ __forceinline__ void jump_substream(const uint32_t *T, const uint32_t s[6], uint64_t idx,
-                                               uint64_t v1[3], uint64_t v2[3]) {
+                                               uint32_t v1[3], uint32_t v2[3]) {
   v1[0] = s[0];
   v1[1] = s[1];
   v1[2] = s[2];
   v2[0] = s[3];
   v2[1] = s[4];
   v2[2] = s[5];
-#pragma unroll 1
+#pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int d = (int)((idx >> (8 * k)) & 255u);
-    matvec_mod(T + ((0 * 8 + k) * 256 + d) * 9, v1, 4294967087ULL, 1.0 / 4294967087.0);
-    matvec_mod(T + ((1 * 8 + k) * 256 + d) * 9, v2, 4294944443ULL, 1.0 / 4294944443.0);
+    uint32_t M1[9], M2[9];
+    jump_row(T, 0, k, d, M1);
+    jump_row(T, 1, k, d, M2);
+    matvec_mod(M1, v1, 4294967087u, 1.0 / 4294967087.0);
+    matvec_mod(M2, v2, 4294944443u, 1.0 / 4294944443.0);
   }
 }
 
 __device__ __forceinline__ void substream_state(const DevParams *P, uint64_t idx, double out[6]) {
-  uint64_t v1[3], v2[3];
+  uint32_t v1[3], v2[3];
   jump_substream(P->jump8, P->seed, idx, v1, v2);
   out[0] = (double)v1[0];
   out[1] = (double)v1[1];

@@ -869,14 +869,14 @@ __global__ void k_advance(Counters *c, int64_t cap) {
 __global__ void k_rng_init(const DevParams *P, const uint32_t *seed, const uint64_t *idx, int n, uint32_t *out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint64_t v1[3], v2[3];
+  uint32_t v1[3], v2[3];
   jump_substream(P->jump8, seed, idx[i], v1, v2);
-  out[6 * i + 0] = (uint32_t)v1[0];
-  out[6 * i + 1] = (uint32_t)v1[1];
-  out[6 * i + 2] = (uint32_t)v1[2];
-  out[6 * i + 3] = (uint32_t)v2[0];
-  out[6 * i + 4] = (uint32_t)v2[1];
-  out[6 * i + 5] = (uint32_t)v2[2];
+  out[6 * i + 0] = v1[0];
+  out[6 * i + 1] = v1[1];
+  out[6 * i + 2] = v1[2];
+  out[6 * i + 3] = v2[0];
+  out[6 * i + 4] = v2[1];
+  out[6 * i + 5] = v2[2];
 }
 
 __global__ void k_rng_draw(uint32_t *states, int n, int k, double *out) {

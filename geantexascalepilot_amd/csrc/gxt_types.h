@@ -23,6 +23,8 @@ constexpr int kInvRangeHints = 256;  // search-hint bins per inverse-range vecto
 // of the atomics (the workgroups of one stripe share an XCD under the
 // round-robin dispatch of workgroups to XCDs).
 constexpr int kStripes = 8;
+// Words per row of the substream jump tables: a 3x3 matrix padded to 48 bytes.
+constexpr int kJumpRow = 12;
 
 // One node of a physics vector: abscissa (energy or range), value, second
 // derivative; a spline lookup reads two adjacent nodes, 64 B (32 B in FP32).
@@ -110,7 +112,7 @@ struct DevParams {
   int pad;
   uint32_t seed[6];
   int nsandia, nshells;     // doubles in the Sandia and shell arrays
-  const uint32_t *jump8;    // A_c^(d 2^(76+8k)) mod m_c at ((c * 8 + k) * 256 + d) * 9
+  const uint32_t *jump8;    // A_c^(d 2^(76+8k)) mod m_c at ((c * 8 + k) * 256 + d) * kJumpRow
   const Node64 *nd;             // physics-vector nodes, FP64
   const Node32 *nf;             // FP32 mirror
   const double *sbd;            // Seltzer-Berger grids, FP64
