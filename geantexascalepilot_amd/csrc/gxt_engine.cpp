@@ -217,6 +217,10 @@ struct gxt_ctx {
   int64_t trace_cap = 0;
   gxt_primary *stage = nullptr;
   int64_t stage_cap = 0;
+  gxt_primary *hstage = nullptr;  // pinned host staging of the primaries of one admission
+  int64_t hstage_cap = 0;
+  hipEvent_t snap = nullptr;      // recorded after a round's counter snapshot
+  size_t ev_snap = 0;             // profiling entries recorded before that snapshot
   // non-uniform field
   gxt_field_map fmap;
   std::string fmap_path;
@@ -321,7 +325,6 @@ int alloc_events(gxt_ctx *c, int n) {
 int admit(gxt_ctx *c) {
   const double budget = 0.8 * (double)c->cap;
   std::vector<gxt_primary> buf;
-  std::vector<std::vector<DevEvent>> evbufs;  // alive until the stream synchronises
   std::vector<std::pair<int, int>> zero_ranges;
   {
     std::lock_guard<std::mutex> lk(c->mu);
@@ -359,25 +362,34 @@ int admit(gxt_ctx *c) {
       if (full) break;
     }
   }
+  // everything below is queued on the stream behind the work in flight; the
+  // slots of a new job belong to events without tracks, so the look-ahead
+  // iterations still running do not touch them
   for (auto &r : zero_ranges) {
-    evbufs.emplace_back(r.second);
-    std::memset(evbufs.back().data(), 0, r.second * sizeof(DevEvent));
-    HIP_TRY(hipMemcpyAsync(c->ev + r.first, evbufs.back().data(), r.second * sizeof(DevEvent),
-                           hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemsetAsync(c->ev + r.first, 0, r.second * sizeof(DevEvent), c->stream));
     HIP_TRY(hipMemsetAsync(c->ev_live + r.first, 0, r.second * sizeof(int), c->stream));
   }
-  if (buf.empty()) {
-    if (!zero_ranges.empty()) HIP_TRY(hipStreamSynchronize(c->stream));
-    return GXT_OK;
-  }
+  if (buf.empty()) return GXT_OK;
   const int64_t total = (int64_t)buf.size();
-  if (total > c->stage_cap) {
+  if (total > c->stage_cap || total > c->hstage_cap) {
     HIP_TRY(hipStreamSynchronize(c->stream));
-    (void)hipFree(c->stage);
-    c->stage_cap = std::max<int64_t>(total, 4096);
-    HIP_TRY(dalloc(&c->stage, c->stage_cap));
+    if (total > c->stage_cap) {
+      (void)hipFree(c->stage);
+      c->stage = nullptr;
+      c->stage_cap = std::max<int64_t>(total, 4096);
+      HIP_TRY(dalloc(&c->stage, c->stage_cap));
+    }
+    if (total > c->hstage_cap) {
+      (void)hipHostFree(c->hstage);
+      c->hstage = nullptr;
+      c->hstage_cap = std::max<int64_t>(total, 4096);
+      HIP_TRY(hipHostMalloc((void **)&c->hstage, c->hstage_cap * sizeof(gxt_primary), hipHostMallocDefault));
+    }
   }
-  HIP_TRY(hipMemcpyAsync(c->stage, buf.data(), buf.size() * sizeof(gxt_primary), hipMemcpyHostToDevice, c->stream));
+  // the previous admission's copy from hstage finished before the last
+  // snapshot, which the host has waited for
+  std::memcpy(c->hstage, buf.data(), buf.size() * sizeof(gxt_primary));
+  HIP_TRY(hipMemcpyAsync(c->stage, c->hstage, buf.size() * sizeof(gxt_primary), hipMemcpyHostToDevice, c->stream));
   const StepArgs a = c->args();
   hipError_t e;
   if (c->precision == GXT_FP32) {
@@ -388,7 +400,6 @@ int admit(gxt_ctx *c) {
     e = launch_inject<double>(a, pa, c->stage, (int)total, c->stream);
   }
   if (e != hipSuccess) return c->fail(GXT_EHIP, std::string("inject: ") + hipGetErrorString(e));
-  HIP_TRY(hipStreamSynchronize(c->stream));
   return GXT_OK;
 }
 
@@ -423,13 +434,11 @@ int iterate(gxt_ctx *c, int k) {
   return GXT_OK;
 }
 
-// Read counters and the per-event live counts; finish events and jobs.
+// Read the snapshot of the counters and per-event live counts (complete:
+// the caller waited for its event); finish events and jobs.
 int collect(gxt_ctx *c, bool &idle) {
-  HIP_TRY(hipMemcpyAsync(c->hcnt, c->cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->hev_live, c->ev_live, c->cfg.max_events * sizeof(int), hipMemcpyDeviceToHost,
-                         c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  for (auto &q : c->ev_pending) {
+  for (size_t qi = 0; qi < c->ev_snap; ++qi) {
+    auto &q = c->ev_pending[qi];
     float t[kStepEvents - 1] = {};
     for (int k = 0; k + 1 < kStepEvents; ++k)
       if (q[k] && q[k + 1]) HIP_TRY(hipEventElapsedTime(&t[k], q[k], q[k + 1]));
@@ -442,7 +451,8 @@ int collect(gxt_ctx *c, bool &idle) {
     c->kst.launches++;
     c->ev_free.push_back(q);
   }
-  c->ev_pending.clear();
+  c->ev_pending.erase(c->ev_pending.begin(), c->ev_pending.begin() + c->ev_snap);
+  c->ev_snap = 0;
   const Counters &h = *c->hcnt;
   if (h.overflow) return c->fail(GXT_ECAPACITY, "track pool or queue overflow (increase pool_capacity)");
   const int64_t live = live_tracks(h);
@@ -533,7 +543,14 @@ int collect(gxt_ctx *c, bool &idle) {
   return GXT_OK;
 }
 
-// One scheduling round: admit, run k iterations, collect.
+// Iterations queued behind a round's counter snapshot: the device runs them
+// while the host reads the snapshot and plans the next admission.
+constexpr int kLookahead = 1;
+
+// One scheduling round of k iterations: admit, run k - kLookahead iterations,
+// snapshot the counters, queue the look-ahead iterations, then collect from
+// the snapshot.  Admission thus sees counts kLookahead iterations old; each
+// event in flight reserves its expected peak, which covers the growth.
 int round_trip(gxt_ctx *c, int k, bool &idle) {
   int rc = admit(c);
   if (rc) return rc;
@@ -544,9 +561,19 @@ int round_trip(gxt_ctx *c, int k, bool &idle) {
       if (jp->started) any = true;
   }
   if (any) {
-    rc = iterate(c, k);
+    rc = iterate(c, std::max(k - kLookahead, 1));
     if (rc) return rc;
   }
+  HIP_TRY(hipMemcpyAsync(c->hcnt, c->cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->hev_live, c->ev_live, c->cfg.max_events * sizeof(int), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipEventRecord(c->snap, c->stream));
+  c->ev_snap = c->ev_pending.size();
+  if (any) {
+    rc = iterate(c, kLookahead);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipEventSynchronize(c->snap));
   return collect(c, idle);
 }
 
@@ -572,6 +599,8 @@ void destroy_ctx(gxt_ctx *c) {
     c->stop = true;
     if (c->server.joinable()) c->server.join();
   }
+  // look-ahead iterations may still be in flight
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (int i = 0; i < 2; ++i) {
     c->pd[i].release();
     c->pf[i].release();
@@ -584,6 +613,8 @@ void destroy_ctx(gxt_ctx *c) {
       if (x) (void)hipEventDestroy(x);
   c->dd.release();
   release_field_map(c->fmap);
+  if (c->snap) (void)hipEventDestroy(c->snap);
+  (void)hipHostFree(c->hstage);
   (void)hipFree(c->queues);
   (void)hipFree(c->newlist);
   (void)hipFree(c->cnt);
@@ -701,6 +732,7 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
   HIP_TRY(hipStreamCreateWithFlags(&c->ss.side, hipStreamNonBlocking));
   HIP_TRY(hipEventCreateWithFlags(&c->ss.fork, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&c->ss.join, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&c->snap, hipEventDisableTiming));
   c->ss.main = c->stream;
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
@@ -988,6 +1020,8 @@ int gxt_serve_stop(gxt_ctx *c) {
   c->stop = true;
   if (c->server.joinable()) c->server.join();
   c->serving = false;
+  // the last round's look-ahead iterations
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return c->serve_rc;
 }
 
