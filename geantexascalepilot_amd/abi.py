@@ -22,8 +22,8 @@ GAMMA, ELECTRON, POSITRON = 0, 1, 2
 
 TAB = dict(LAMBDA_COMPT=0, LAMBDAPRIM_COMPT=1, LAMBDA_CONV=2, LAMBDAPRIM_PHOT=3, LAMBDA_BREM_EM=4,
            LAMBDA_IONI_EM=5, DEDX_EM=6, RANGE_EM=7, INVRANGE_EM=8, LAMBDA_BREM_EP=9, LAMBDA_IONI_EP=10,
-           DEDX_EP=11, RANGE_EP=12, INVRANGE_EP=13)
-NTAB = 14
+           DEDX_EP=11, RANGE_EP=12, INVRANGE_EP=13, MSC_EM=14, MSC_EP=15)
+NTAB = 16
 MODEL = dict(KLEIN_NISHINA=0, BETHE_HEITLER=1, SAUTER_GAVRILA=2, SELTZER_BERGER_EM=3, MOLLER=4,
              SELTZER_BERGER_EP=5, BHABHA=6, ANNIHIL=7)
 
@@ -42,7 +42,7 @@ class Config(C.Structure):
         ("max_events", C.c_int32), ("max_jobs", C.c_int32), ("trace_tracks", C.c_int32),
         ("trace_steps", C.c_int32), ("profile_kernels", C.c_int32), ("data_dir", C.c_char_p),
         ("field_map", C.c_char_p), ("field_origin", C.c_double * 3), ("field_stepper", C.c_int32),
-        ("pad1", C.c_int32),
+        ("msc", C.c_int32),
     ]
 
 

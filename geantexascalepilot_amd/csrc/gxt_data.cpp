@@ -27,9 +27,12 @@ const char *kFiles[GXT_NTAB] = {"Lambda.compt.gamma.asc", "LambdaPrim.compt.gamm
                                 "LambdaPrim.phot.gamma.asc", "Lambda.eBrem.e-.asc", "Lambda.eIoni.e-.asc",
                                 "DEDX.eIoni.e-.asc", "Range.eIoni.e-.asc", "InverseRange.eIoni.e-.asc",
                                 "Lambda.eBrem.e+.asc", "Lambda.eIoni.e+.asc", "DEDX.eIoni.e+.asc",
-                                "Range.eIoni.e+.asc", "InverseRange.eIoni.e+.asc"};
+                                "Range.eIoni.e+.asc", "InverseRange.eIoni.e+.asc", "LambdaMod1.msc.e-.asc",
+                                "LambdaMod1.msc.e+.asc"};
 // density scaling: 0 multiply y, 1 divide y, 2 divide x
-const int kScale[GXT_NTAB] = {0, 0, 0, 0, 0, 0, 0, 1, 2, 0, 0, 0, 1, 2};
+const int kScale[GXT_NTAB] = {0, 0, 0, 0, 0, 0, 0, 1, 2, 0, 0, 0, 1, 2, 0, 0};
+// tables a material may lack (then zero nodes): the multiple-scattering ones
+bool optional_table(int t) { return t == GXT_TAB_MSC_EM || t == GXT_TAB_MSC_EP; }
 
 struct Vec {
   std::vector<double> x, y, d2;
@@ -234,6 +237,13 @@ int load_material(const std::string &data, int which, double lar_density, HostDa
   const double f = rho / tab_rho;
   for (int t = 0; t < GXT_NTAB; ++t) {
     std::vector<Vec> vecs;
+    if (optional_table(t) && !std::ifstream(dir + "/" + kFiles[t])) {
+      DevTable &dt = m.tab[t];
+      std::memset(&dt, 0, sizeof(dt));
+      dt.off = (int)hd.x.size();
+      dt.hint = -1;
+      continue;
+    }
     if (!parse_tables(dir + "/" + kFiles[t], vecs) || couple >= (int)vecs.size()) {
       err = "cannot read " + dir + "/" + kFiles[t];
       return GXT_EDATA;

@@ -1027,6 +1027,94 @@ __device__ __forceinline__ int propagate(const DevParams *P, int species, R e, d
   return kLimit;
 }
 
+// ------------------------------------------------------------------ multiple scattering
+// e-/e+ multiple scattering on the first transport mean free path lambda1,
+// tabulated as E^2 / lambda1 (the LambdaMod1.msc tables; the reference
+// reserves the process, ProxyProcessIndex.cuh:27, but has no model):
+//  - step limit max(0.04 max(range, lambda1), 0.6 safety, 1e-5 mm);
+//  - geometric path z = lambda1 (1 - exp(-t/lambda1)) for a true path t
+//    (Lewis' mean longitudinal displacement, loss over the step neglected),
+//    and its inverse when geometry ends the step;
+//  - deflection: x = 1 - cos(theta) from p(x) ~ 1/(x + a)^2 on [0, 2] with a
+//    set so that <cos theta> = exp(-t/lambda1) (Lewis' first moment),
+//    azimuth uniform; no lateral displacement.
+constexpr double kMscFacRange = 0.04, kMscFacSafety = 0.6, kMscTmin = 1.0e-5;
+constexpr double kMscNone = 1.0e30;  // lambda1 where a material has no msc table
+
+template <class R>
+__device__ __forceinline__ double msc_lambda1(const Nodes<R> &N, const DevMaterial &m, bool pos, R e) {
+  const DevTable &t = m.tab[pos ? GXT_TAB_MSC_EP : GXT_TAB_MSC_EM];
+  if (t.n == 0) return kMscNone;
+  const R v = pvec_value(N, t, e);
+  return v > R(0) ? (double)(e * e / v) : kMscNone;
+}
+
+// (expm1 and log1p keep full precision for short steps)
+__device__ __forceinline__ double msc_geom(double t, double lam) { return -lam * expm1(-t / lam); }
+
+// true path of a geometric path z, at most tmax
+__device__ __forceinline__ double msc_true(double z, double lam, double tmax) {
+  const double x = z / lam;
+  if (x >= 1.0) return tmax;
+  return fmin(-lam * log1p(-x), tmax);
+}
+
+// Screening of the deflection x = 1 - cos(theta) ~ 1/(x + a)^2 on [0, 2],
+// as u = 2/a, for a mean 1 - cos(theta) of 1 - q, q = exp(-tau):
+// 1 - q = mean(a) = a (a + 2) ln(1 + 2/a) / 2 - a, which rises from 0 (a -> 0)
+// to 1 (a -> inf).  For q >= 0.05 (a below ~12): twelve Newton steps on ln a
+// (d mean / d ln a = a ((a + 1) ln(1 + 2/a) - 2) > 0, steps at most e^2) from
+// the small-a form a (ln(2/a) - 1) or the large-a form q = 2/(3a).  For
+// q < 0.05 the direct form cancels; q = sum_{j>=1} 2 (-1)^(j+1) u^j /
+// ((j + 1)(j + 2)) (from the series of ln(1 + u)) is solved for u < 0.17 by
+// twelve Newton steps from u = 3q with 20 terms.
+__device__ __forceinline__ double msc_u(double tau) {
+  const double q = exp(-tau);
+  if (q >= 0.05) {
+    const double mean = -expm1(-tau);
+    double a = mean < 0.5 ? mean / fmax(log(2.0 / mean) - 1.0, 0.5) : 2.0 / (3.0 * q);
+    for (int k = 0; k < 12; ++k) {
+      const double L = log1p(2.0 / a);
+      const double f = 0.5 * a * (a + 2.0) * L - a - mean;
+      const double df = a * ((a + 1.0) * L - 2.0);
+      const double du = fmin(fmax(-f / df, -2.0), 2.0);
+      a *= exp(du);
+    }
+    return 2.0 / a;
+  }
+  double u = 3.0 * q;
+  for (int k = 0; k < 12; ++k) {
+    double h = 0.0, dh = 0.0;
+    for (int j = 20; j >= 1; --j) {
+      const double c = (j % 2 ? 2.0 : -2.0) / ((j + 1.0) * (j + 2.0));
+      dh = dh * u + j * c;
+      h = h * u + c;
+    }
+    h *= u;
+    u = fmin(fmax(u - (h - q) / dh, 0.0), 0.2);
+  }
+  return u;
+}
+
+// Deflection after a true path of tau transport mean free paths (two draws):
+// x = 2 r1 / (1 + u (1 - r1)) inverts the distribution's CDF.
+template <class R>
+__device__ __forceinline__ void msc_deflect(double tau, R dir[3], Rng &rng) {
+  const double r1 = rng.uniform();
+  const double r2 = rng.uniform();
+  const double u = msc_u(tau);
+  const double x = 2.0 * r1 / (1.0 + u * (1.0 - r1));
+  const double cost = 1.0 - x;
+  const double sint = sqrt(fmax(x * (2.0 - x), 0.0));
+  const double phi = kTwoPi * r2;
+  R uu = (R)(sint * cos(phi)), v = (R)(sint * sin(phi)), w = (R)cost;
+  rotate_to_lab<R>(uu, v, w, dir[0], dir[1], dir[2]);
+  dir[0] = uu;
+  dir[1] = v;
+  dir[2] = w;
+  unit3(dir[0], dir[1], dir[2]);
+}
+
 // ------------------------------------------------------------------ general field
 // Field of the cylindrical (r, z) map at (x, y, z) of the map frame
 // (ScalarCMSmagField::GetFieldValue / GetFieldValueRZ,

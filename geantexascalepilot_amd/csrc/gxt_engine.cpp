@@ -760,7 +760,12 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
     if (rc) return rc;
   }
   P.field = dev_field(cfg->field_map ? &c->fmap : nullptr, cfg->field, cfg->field_origin, cfg->field_stepper);
-  c->general_field = P.field.general;
+  P.msc = cfg->msc ? 1 : 0;
+  if (P.msc)
+    for (size_t s = 0; s < mats.size(); ++s)
+      if (P.mat[s].tab[GXT_TAB_MSC_EM].n == 0 || P.mat[s].tab[GXT_TAB_MSC_EP].n == 0)
+        return set_err(GXT_EINVAL, "multiple scattering needs the LambdaMod1.msc tables (none for PbWO4)");
+  c->general_field = P.field.general || P.msc;
   P.tracking_cut = cfg->tracking_cut;
   P.zbin = cfg->zbin_mm;
   P.rbin = cfg->rbin_mm;

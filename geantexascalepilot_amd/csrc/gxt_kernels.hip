@@ -467,8 +467,9 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
 // One kernel per region: kCharged = false steps the photons of [0, nN),
 // kCharged = true the electrons and positrons of [cap - nC, cap); the species
 // class is a compile-time constant so each variant carries only its own code.
-// kGeneral: e-/e+ in a field map or with the RK45 stepper (propagate_general);
-// otherwise straight lines and the exact uniform-field helix.
+// kGeneral: e-/e+ with multiple scattering, in a field map or with the RK45
+// stepper (propagate_general); otherwise straight lines and the exact
+// uniform-field helix, no multiple scattering.
 template <class R, bool kCharged, bool kGeneral = false>
 __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a, PoolArgs<R> pa) {
   __shared__ WaveHist whs[kBlock / 64];
@@ -535,14 +536,40 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
           sel = -1;
         }
       }
+      // multiple scattering: its step limit; then the geometric length of the
+      // true path is propagated (the step is the true path for interaction
+      // lengths and loss)
+      double lam1 = kMscNone;
+      if constexpr (kGeneral) {
+        if (charged && P->msc) {
+          lam1 = msc_lambda1<R>(N, m, sp == 2, t.e);
+          if (lam1 < kMscNone) {
+            const double range = (double)pvec_value(N, m.tab[sp == 2 ? GXT_TAB_RANGE_EP : GXT_TAB_RANGE_EM], t.e);
+            const double tlim =
+                fmax(fmax(kMscFacRange * fmax(range, lam1), kMscFacSafety * safety(P->geo, t.vol, t.pos)), kMscTmin);
+            if (tlim < bestd) {
+              bestd = tlim;
+              sel = -1;
+            }
+          }
+        }
+      }
       const double pre[3] = {t.pos[0], t.pos[1], t.pos[2]};
       const int prevol = t.vol;
       double step = 0.0;
       int status;
-      if constexpr (kGeneral)
-        status = propagate_general<R>(P, sp, t.e, t.pos, t.dir, t.vol, bestd, step);
-      else
+      if constexpr (kGeneral) {
+        const bool msc_on = lam1 < kMscNone;
+        const double request = msc_on ? msc_geom(bestd, lam1) : bestd;
+        double gstep = 0.0;
+        if (P->field.general)
+          status = propagate_general<R>(P, sp, t.e, t.pos, t.dir, t.vol, request, gstep);
+        else
+          status = propagate<R>(P, sp, t.e, t.pos, t.dir, t.vol, request, gstep);
+        step = msc_on ? (status == kPhysics ? bestd : msc_true(gstep, lam1, bestd)) : gstep;
+      } else {
         status = propagate<R>(P, sp, t.e, t.pos, t.dir, t.vol, bestd, step);
+      }
       if (xs0 > R(0)) t.nil0 -= (R)step * xs0;
       if (xs1 > R(0)) t.nil1 -= (R)step * xs1;
       if (xs2 > R(0)) t.nil2 -= (R)step * xs2;
@@ -554,6 +581,10 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
         const double mid[3] = {0.5 * (pre[0] + t.pos[0]), 0.5 * (pre[1] + t.pos[1]), 0.5 * (pre[2] + t.pos[2])};
         dA.add(P, t, (double)loss, mid, prevol);
         edep += (double)loss;
+        if constexpr (kGeneral) {
+          if (lam1 < kMscNone && status != kEscaped && t.e > R(0) && step > 0.0)
+            msc_deflect<R>(step / lam1, t.dir, t.rng);
+        }
       }
       int tproc = status == kPhysics ? sel : (status == kBoundary ? -2 : (status == kEscaped ? -3 : -1));
       if (status == kPhysics && sel < 0) tproc = -1;

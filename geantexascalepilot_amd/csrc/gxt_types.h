@@ -109,7 +109,7 @@ struct DevParams {
   int max_steps;
   int trace_events;   // events [0, trace_events) are traced
   int trace_steps;
-  int pad;
+  int msc;            // e-/e+ multiple scattering
   uint32_t seed[6];
   int nsandia, nshells;     // doubles in the Sandia and shell arrays
   const uint32_t *jump8;    // A_c^(d 2^(76+8k)) mod m_c at ((c * 8 + k) * 256 + d) * kJumpRow

@@ -90,7 +90,8 @@ typedef struct gxt_config {
   double field_origin[3];   /* position of the setup origin in the map frame (mm)   */
   int32_t field_stepper;    /* GXT_STEPPER_* for e-/e+ (a uniform field with HELIX
                                takes the exact constant-field path)                  */
-  int32_t pad1;
+  int32_t msc;              /* 1: multiple scattering of e-/e+ (see GXT_TAB_MSC_EM);
+                               0: none, as in the reference's physics list           */
 } gxt_config;
 
 /* One primary particle. */
@@ -269,7 +270,12 @@ int gxt_rng_draw(uint32_t *states, int n, int k, double *out);
 #define GXT_TAB_DEDX_EP 11
 #define GXT_TAB_RANGE_EP 12
 #define GXT_TAB_INVRANGE_EP 13
-#define GXT_NTAB 14
+/* multiple scattering: E^2 / lambda1 (MeV^2/mm), lambda1 the first transport
+ * mean free path (LambdaMod1.msc.e-/e+, ProxyPhysicsTableIndex.hpp:39-40);
+ * present for Pb and lAr, absent (zero nodes) for PbWO4 */
+#define GXT_TAB_MSC_EM 14
+#define GXT_TAB_MSC_EP 15
+#define GXT_NTAB 16
 int gxt_table_value(gxt_ctx *ctx, int material_slot, int table, int precision, const double *x,
                     double *out, int n);
 /* Macroscopic cross-section (1/mm) of discrete process p of a species at the

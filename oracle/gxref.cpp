@@ -223,6 +223,7 @@ static inline double spline(const PVec &p, double e, int bin) {
 // relies on E < xN to stay in range).
 static double pvec_value(const PVec &p, double e) {
   const int N = (int)p.x.size();
+  if (N == 0) return 0.0;
   if (e <= p.x[0]) return p.y[0];
   if (e >= p.x[N - 1]) return p.y[N - 1];
   int bin = (int)(std::log(e) / p.dl - p.base);
@@ -237,6 +238,7 @@ static double pvec_value(const PVec &p, double e) {
 // Value() on it, ProxyIonization.hpp:100-104).
 static double pvec_inv_value(const PVec &p, double r) {
   const int N = (int)p.x.size();
+  if (N == 0) return 0.0;
   if (r <= p.x[0]) return p.y[0];
   if (r >= p.x[N - 1]) return p.y[N - 1];
   int n1 = 0, n2 = N / 2, n3 = N - 1;
@@ -324,11 +326,13 @@ static const char *kTableFile[GXT_NTAB] = {
     "LambdaPrim.phot.gamma.asc", "Lambda.eBrem.e-.asc",      "Lambda.eIoni.e-.asc",
     "DEDX.eIoni.e-.asc",         "Range.eIoni.e-.asc",        "InverseRange.eIoni.e-.asc",
     "Lambda.eBrem.e+.asc",       "Lambda.eIoni.e+.asc",       "DEDX.eIoni.e+.asc",
-    "Range.eIoni.e+.asc",        "InverseRange.eIoni.e+.asc"};
+    "Range.eIoni.e+.asc",        "InverseRange.eIoni.e+.asc", "LambdaMod1.msc.e-.asc",
+    "LambdaMod1.msc.e+.asc"};
 
-// kind of scaling with density: 0 = proportional (cross-sections, dE/dx),
-// 1 = inverse (range), 2 = inverse on x (inverse range)
-static const int kTableScale[GXT_NTAB] = {0, 0, 0, 0, 0, 0, 0, 1, 2, 0, 0, 0, 1, 2};
+// kind of scaling with density: 0 = proportional (cross-sections, dE/dx,
+// inverse transport mean free path), 1 = inverse (range), 2 = inverse on x
+// (inverse range)
+static const int kTableScale[GXT_NTAB] = {0, 0, 0, 0, 0, 0, 0, 1, 2, 0, 0, 0, 1, 2, 0, 0};
 
 struct ElemSpec {
   int Z;
@@ -452,6 +456,11 @@ static int build_material(const std::string &data, int which, double lar_density
   const double f = m.density / tab_density;
   for (int t = 0; t < GXT_NTAB; ++t) {
     std::vector<PVec> vecs;
+    // the multiple-scattering tables are optional (none for PbWO4)
+    if ((t == GXT_TAB_MSC_EM || t == GXT_TAB_MSC_EP) && !std::ifstream(tabdir + "/" + kTableFile[t])) {
+      m.tab[t] = PVec();
+      continue;
+    }
     if (!read_asc(tabdir + "/" + kTableFile[t], vecs) || couple >= (int)vecs.size())
       return fail(GXT_EDATA, tabdir + "/" + kTableFile[t]);
     PVec p = vecs[couple];
@@ -1327,6 +1336,94 @@ static double along_limit(const Material &m, bool pos, double e, double mine) {
   return range > finR ? range * 0.2 + finR * 0.8 * (2.0 - finR / range) : range;
 }
 
+// ---------------------------------------------------------------- multiple scattering
+// e-/e+ multiple scattering on the first transport mean free path lambda1 of
+// the reference's LambdaMod1.msc tables (stored as E^2 / lambda1, the table
+// convention of Geant4's G4VMscModel).  The reference reserves the process
+// (ProxyProcessIndex.cuh:27) but has no model; this one is:
+//  - step limit max(0.04 max(range, lambda1), 0.6 safety, 1e-5 mm) (the
+//    "UseSafety" form of Geant4's Urban model);
+//  - geometric path z = lambda1 (1 - exp(-t/lambda1)) for a true path t (Lewis'
+//    mean longitudinal displacement, energy loss over the step neglected), and
+//    its inverse when geometry ends the step;
+//  - deflection: x = 1 - cos(theta) from p(x) ~ 1/(x + a)^2 on [0, 2] (the
+//    screened-Rutherford form) with a set so that <cos theta> = exp(-t/lambda1)
+//    exactly (Lewis' first moment), azimuth uniform; no lateral displacement.
+constexpr double kMscFacRange = 0.04, kMscFacSafety = 0.6, kMscTmin = 1.0e-5;
+constexpr double kMscNone = 1.0e30;  // lambda1 where a material has no msc table
+
+static double msc_lambda1(const Material &m, bool pos, double e) {
+  const PVec &p = m.tab[pos ? GXT_TAB_MSC_EP : GXT_TAB_MSC_EM];
+  if (p.x.empty()) return kMscNone;
+  const double v = pvec_value(p, e);
+  return v > 0.0 ? e * e / v : kMscNone;
+}
+
+// (expm1 and log1p keep full precision for short steps)
+static double msc_geom(double t, double lam) { return -lam * std::expm1(-t / lam); }
+
+// true path of a geometric path z, at most tmax
+static double msc_true(double z, double lam, double tmax) {
+  const double x = z / lam;
+  if (x >= 1.0) return tmax;
+  return std::min(-lam * std::log1p(-x), tmax);
+}
+
+// Screening of the deflection x = 1 - cos(theta) ~ 1/(x + a)^2 on [0, 2],
+// as u = 2/a, for a mean 1 - cos(theta) of 1 - q, q = exp(-tau):
+// 1 - q = mean(a) = a (a + 2) ln(1 + 2/a) / 2 - a, which rises from 0 (a -> 0)
+// to 1 (a -> inf).  For q >= 0.05 (a below ~12): twelve Newton steps on ln a
+// (d mean / d ln a = a ((a + 1) ln(1 + 2/a) - 2) > 0, steps at most e^2) from
+// the small-a form a (ln(2/a) - 1) or the large-a form q = 2/(3a).  For
+// q < 0.05 the direct form cancels; q = sum_{j>=1} 2 (-1)^(j+1) u^j /
+// ((j + 1)(j + 2)) (from the series of ln(1 + u)) is solved for u < 0.17 by
+// twelve Newton steps from u = 3q with 20 terms.
+static double msc_u(double tau) {
+  const double q = std::exp(-tau);
+  if (q >= 0.05) {
+    const double mean = -std::expm1(-tau);
+    double a = mean < 0.5 ? mean / std::max(std::log(2.0 / mean) - 1.0, 0.5) : 2.0 / (3.0 * q);
+    for (int k = 0; k < 12; ++k) {
+      const double L = std::log1p(2.0 / a);
+      const double f = 0.5 * a * (a + 2.0) * L - a - mean;
+      const double df = a * ((a + 1.0) * L - 2.0);
+      const double du = std::min(std::max(-f / df, -2.0), 2.0);
+      a *= std::exp(du);
+    }
+    return 2.0 / a;
+  }
+  double u = 3.0 * q;
+  for (int k = 0; k < 12; ++k) {
+    double h = 0.0, dh = 0.0;
+    for (int j = 20; j >= 1; --j) {
+      const double c = (j % 2 ? 2.0 : -2.0) / ((j + 1.0) * (j + 2.0));
+      dh = dh * u + j * c;
+      h = h * u + c;
+    }
+    h *= u;  // h = sum c_j u^j, dh = sum j c_j u^(j-1)
+    u = std::min(std::max(u - (h - q) / dh, 0.0), 0.2);
+  }
+  return u;
+}
+
+// Deflection after a true path of tau transport mean free paths (two draws):
+// x = 2 r1 / (1 + u (1 - r1)) inverts the distribution's CDF.
+static void msc_deflect(double tau, double dir[3], uint32_t rng[6]) {
+  const double r1 = mrg_uniform(rng);
+  const double r2 = mrg_uniform(rng);
+  const double u = msc_u(tau);
+  const double x = 2.0 * r1 / (1.0 + u * (1.0 - r1));
+  const double cost = 1.0 - x;
+  const double sint = std::sqrt(std::max(x * (2.0 - x), 0.0));
+  const double phi = kTwoPi * r2;
+  double uu = sint * std::cos(phi), v = sint * std::sin(phi), w = cost;
+  rotate_to_lab(uu, v, w, dir[0], dir[1], dir[2]);
+  dir[0] = uu;
+  dir[1] = v;
+  dir[2] = w;
+  unit3(dir);
+}
+
 // Continuous energy loss (ProxyIonization.hpp:130-168) with the tracking cut
 // as minimum energy and the binary-search inverse range.
 static double along_doit(const Material &m, bool pos, double e, double step, double mine, double &eloss) {
@@ -1385,6 +1482,7 @@ struct gxref_ctx {
   FieldMap fmap;
   Field field;          // map or uniform field seen by e-/e+ on the general path
   bool general = false; // e-/e+ propagate through propagate_general (a map, or RK45)
+  bool msc = false;     // e-/e+ multiple scattering
   uint32_t seed[6];
   std::vector<EventTally> events;
   Worker tot;
@@ -1639,10 +1737,27 @@ static void one_step(gxref_ctx &c, Worker &w, EventTally &ev, std::vector<Track>
       sel = -1;
     }
   }
+  // multiple scattering: its step limit, then the geometric length of the
+  // true path (the step is the true path for interaction lengths and loss)
+  double lam1 = kMscNone;
+  if (charged && c.msc) {
+    lam1 = msc_lambda1(m, t.species == 2, t.e);
+    if (lam1 < kMscNone) {
+      const double range = pvec_value(m.tab[t.species == 2 ? GXT_TAB_RANGE_EP : GXT_TAB_RANGE_EM], t.e);
+      const double tlim = std::max(
+          std::max(kMscFacRange * std::max(range, lam1), kMscFacSafety * safety(c.geo, t.vol, t.pos)), kMscTmin);
+      if (tlim < best) {
+        best = tlim;
+        sel = -1;
+      }
+    }
+  }
+  const bool msc_on = lam1 < kMscNone;
   const double pre[3] = {t.pos[0], t.pos[1], t.pos[2]};
   const int prevol = t.vol;
-  double step = 0.0;
-  const int status = propagate(c, t, best, step);
+  double gstep = 0.0;
+  const int status = propagate(c, t, msc_on ? msc_geom(best, lam1) : best, gstep);
+  const double step = msc_on ? (status == kPhysics ? best : msc_true(gstep, lam1, best)) : gstep;
   for (int p = 0; p < np; ++p)
     if (xs[p] > 0.0) t.nil[p] -= step * xs[p];
   t.nsteps++;
@@ -1656,6 +1771,7 @@ static void one_step(gxref_ctx &c, Worker &w, EventTally &ev, std::vector<Track>
     const double mid[3] = {0.5 * (pre[0] + t.pos[0]), 0.5 * (pre[1] + t.pos[1]), 0.5 * (pre[2] + t.pos[2])};
     deposit(c, w, ev, loss, mid, prevol);
     edep += loss;
+    if (msc_on && status != kEscaped && t.e > 0.0 && step > 0.0) msc_deflect(step / lam1, t.dir, t.rng);
   }
   int tproc = status == kPhysics ? sel : (status == kBoundary ? -2 : (status == kEscaped ? -3 : -1));
   if (status == kPhysics && sel < 0) tproc = -1;
@@ -1788,6 +1904,13 @@ int gxref_create(const gxt_config *cfg, gxref_ctx **out) {
   }
   c->field.stepper = cfg->field_stepper;
   c->general = cfg->field_map != nullptr || cfg->field_stepper == GXT_STEPPER_RK45;
+  c->msc = cfg->msc != 0;
+  if (c->msc)
+    for (const Material &mm : c->phys.mat)
+      if (mm.tab[GXT_TAB_MSC_EM].x.empty() || mm.tab[GXT_TAB_MSC_EP].x.empty()) {
+        delete c;
+        return fail(GXT_EINVAL, "multiple scattering needs the LambdaMod1.msc tables (none for PbWO4)");
+      }
   c->events.assign(std::max(cfg->max_events, 1), EventTally());
   c->tot.ez.assign(cfg->n_zbins, 0.0);
   c->tot.er.assign(cfg->n_rbins, 0.0);
@@ -2213,5 +2336,35 @@ int gxref_field_step(const gxref_field_map *map, const double field[3], const do
       err[n + i] = eu;
     }
   }
+  return GXT_OK;
+}
+
+// ---------------------------------------------------------------- msc test hooks
+int gxref_msc_deflect(double tau, uint32_t seed, int n, double *cos_theta, double *phi) {
+  if (!(tau > 0.0) || n <= 0 || !cos_theta) return fail(GXT_EINVAL, "bad msc deflect arguments");
+  const uint32_t s6[6] = {seed, seed, seed, seed, seed, seed};
+  for (int i = 0; i < n; ++i) {
+    uint32_t st[6];
+    substream_state(s6, (uint64_t)i, st);
+    double d[3] = {0.0, 0.0, 1.0};
+    msc_deflect(tau, d, st);
+    cos_theta[i] = d[2];
+    if (phi) phi[i] = std::atan2(d[1], d[0]);
+  }
+  return GXT_OK;
+}
+
+int gxref_msc_lengths(const double *t, const double *lam, int n, double *z, double *t_back) {
+  if (!t || !lam || !z || !t_back || n <= 0) return fail(GXT_EINVAL, "bad msc length arguments");
+  for (int i = 0; i < n; ++i) {
+    z[i] = msc_geom(t[i], lam[i]);
+    t_back[i] = msc_true(z[i], lam[i], 1.0e300);
+  }
+  return GXT_OK;
+}
+
+int gxref_msc_screening(const double *tau, int n, double *u) {
+  if (!tau || !u || n <= 0) return fail(GXT_EINVAL, "bad msc screening arguments");
+  for (int i = 0; i < n; ++i) u[i] = msc_u(tau[i]);
   return GXT_OK;
 }

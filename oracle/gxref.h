@@ -58,6 +58,15 @@ int gxref_macro_xs(gxref_ctx *ctx, int material_slot, int species, int process, 
 int gxref_sample(gxref_ctx *ctx, int model, int material_slot, gxt_sample_io *io);
 int gxref_helix_step(const double field[3], const double *charge, const double *momentum,
                      const double *step, double *pos, double *dir, int n);
+/* Multiple-scattering pieces of the oracle's e-/e+ step (test hooks; the
+ * library has them only inside its transport kernel):
+ * cos(theta) of n deflections of (0,0,1) after tau transport mean free paths,
+ * deflection i drawn from substream i of seed (two draws each);
+ * z(t) and t(z(t)) for the geometric-path conversion;
+ * u = 2/a of the screening a after tau transport mean free paths. */
+int gxref_msc_deflect(double tau, uint32_t seed, int n, double *cos_theta, double *phi);
+int gxref_msc_lengths(const double *t, const double *lambda1, int n, double *z, double *t_back);
+int gxref_msc_screening(const double *tau, int n, double *u);
 /* Field map and field stepping, same contract as gxt_field_map_* / gxt_field_step (FP64). */
 typedef struct gxref_field_map gxref_field_map;
 int gxref_field_map_load(const char *path, gxref_field_map **out);

@@ -201,6 +201,27 @@ def helix_step(field, charge, mom, step, pos, dirs):
     return pos, dirs
 
 
+def msc_deflect(tau, seed, n):
+    c, phi = np.zeros(n), np.zeros(n)
+    check(lib().gxref_msc_deflect(C.c_double(tau), C.c_uint32(seed), n, abi.ptr(c), abi.ptr(phi)))
+    return c, phi
+
+
+def msc_lengths(t, lam):
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    z, tb = np.zeros_like(t), np.zeros_like(t)
+    check(lib().gxref_msc_lengths(abi.ptr(t), abi.ptr(lam), len(t), abi.ptr(z), abi.ptr(tb)))
+    return z, tb
+
+
+def msc_screening(mean):
+    mean = np.ascontiguousarray(mean, dtype=np.float64)
+    a = np.zeros_like(mean)
+    check(lib().gxref_msc_screening(abi.ptr(mean), len(mean), abi.ptr(a)))
+    return a
+
+
 class FieldMap:
     """Oracle copy of a cylindrical (r, z) field map (host memory)."""
 
