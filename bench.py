@@ -118,13 +118,15 @@ def field_forms(g, abi, n_events=20000, seed=2468, cpu=False):
     """The calorimeter workload of the precision form (FP64) in three fields:
     uniform 4 T along +x with the exact helix; the CMS 2015 solenoid map with
     the calorimeter's front face at r = 600 mm, z = 6 m, with the helix in the
-    local field and with Cash-Karp RK45; and the uniform field with RK45.
-    Throughput and device time per track-step of each."""
+    local field and with Cash-Karp RK45; the uniform field with RK45; and the
+    uniform field with multiple scattering of e-/e+.  Throughput and device
+    time per track-step of each."""
     prims = g.make_primaries(seed, abi.ELECTRON, 1000.0, 50000.0, 10.0, n_events)
     forms = [("uniform_helix", lambda: abi.config_calorimeter(max_events=n_events)),
              ("cms_map_helix", lambda: abi.config_calorimeter_cms(stepper=abi.STEPPER_HELIX, max_events=n_events)),
              ("cms_map_rk45", lambda: abi.config_calorimeter_cms(stepper=abi.STEPPER_RK45, max_events=n_events)),
-             ("uniform_rk45", lambda: abi.config_calorimeter(field_stepper=abi.STEPPER_RK45, max_events=n_events))]
+             ("uniform_rk45", lambda: abi.config_calorimeter(field_stepper=abi.STEPPER_RK45, max_events=n_events)),
+             ("uniform_helix_msc", lambda: abi.config_calorimeter(msc=1, max_events=n_events))]
     out = dict(workload="e- 1-50 GeV x %d into 50 x (2.3 mm Pb + 5.7 mm lAr), FP64" % n_events)
     for name, make in forms:
         with g.Transport(make()) as t:

@@ -1068,7 +1068,7 @@ __device__ __forceinline__ double msc_true(double z, double lam, double tmax) {
 // q < 0.05 the direct form cancels; q = sum_{j>=1} 2 (-1)^(j+1) u^j /
 // ((j + 1)(j + 2)) (from the series of ln(1 + u)) is solved for u < 0.17 by
 // twelve Newton steps from u = 3q with 20 terms.
-__device__ __forceinline__ double msc_u(double tau) {
+__device__ __attribute__((noinline)) double msc_u(double tau) {
   const double q = exp(-tau);
   if (q >= 0.05) {
     const double mean = -expm1(-tau);
