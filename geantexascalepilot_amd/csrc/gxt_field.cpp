@@ -155,6 +155,16 @@ struct DevBuf {
 
 int hip_fail(hipError_t e, const char *what) { return set_error(GXT_EHIP, std::string(what) + ": " + hipGetErrorString(e)); }
 
+// A map's nodes live on the device that was current when it was loaded.
+int check_device(const gxt_field_map *m) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return set_error(GXT_EHIP, "hipGetDevice failed");
+  if (dev != m->device)
+    return set_error(GXT_EINVAL, "field map loaded on device " + std::to_string(m->device) + ", current device is " +
+                                     std::to_string(dev));
+  return GXT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -194,6 +204,7 @@ int gxt_field_map_info(const gxt_field_map *m, int *nr, int *nz, double *r0, dou
 
 int gxt_field_map_value(const gxt_field_map *m, const double *pos, double *b, int n) {
   if (!m || !m->dev || !pos || !b || n <= 0) return set_error(GXT_EINVAL, "bad field-map value arguments");
+  if (const int rc = check_device(m)) return rc;
   const DevField f = dev_field(m, nullptr, nullptr, GXT_STEPPER_HELIX);
   DevBuf<double> dp, db;
   hipError_t e = dp.alloc(3 * (size_t)n);
@@ -210,6 +221,8 @@ int gxt_field_step(const gxt_field_map *m, const double field[3], const double o
   if ((!m && !field) || !charge || !mom || !step || !pos || !dir || n <= 0)
     return set_error(GXT_EINVAL, "bad field-step arguments");
   if (m && !m->dev) return set_error(GXT_EINVAL, "field map not on a device");
+  if (m)
+    if (const int rc = check_device(m)) return rc;
   if (stepper != GXT_STEPPER_HELIX && stepper != GXT_STEPPER_RK45) return set_error(GXT_EINVAL, "bad stepper");
   if (precision != GXT_FP64 && precision != GXT_FP32) return set_error(GXT_EINVAL, "bad precision");
   // the RK45 sub-step count ceil(step 0.3 |B| / (p GXT_RK_ANGLE)) is bounded with the largest |B|

@@ -338,6 +338,8 @@ int gxt_field_map_free(gxt_field_map *map);
  * largest |B| of the nodes (tesla). */
 int gxt_field_map_info(const gxt_field_map *map, int *nr, int *nz, double *r0, double *z0, double *dr,
                        double *dz, double *bmax);
+/* gxt_field_map_value and gxt_field_step need the map's device to be the
+ * current HIP device (GXT_EINVAL otherwise). */
 /* Cartesian field (tesla) at n points of the map frame: bilinear in (r, z)
  * inside the grid cell, r and z clamped to the grid, cylindrical components
  * turned with the point's azimuth (on the axis: Bx = Br, By = Bphi).
