@@ -122,11 +122,12 @@ def field_forms(g, abi, n_events=20000, seed=2468, cpu=False):
     uniform field with multiple scattering of e-/e+.  Throughput and device
     time per track-step of each."""
     prims = g.make_primaries(seed, abi.ELECTRON, 1000.0, 50000.0, 10.0, n_events)
-    forms = [("uniform_helix", lambda: abi.config_calorimeter(max_events=n_events)),
-             ("cms_map_helix", lambda: abi.config_calorimeter_cms(stepper=abi.STEPPER_HELIX, max_events=n_events)),
-             ("cms_map_rk45", lambda: abi.config_calorimeter_cms(stepper=abi.STEPPER_RK45, max_events=n_events)),
-             ("uniform_rk45", lambda: abi.config_calorimeter(field_stepper=abi.STEPPER_RK45, max_events=n_events)),
-             ("uniform_helix_msc", lambda: abi.config_calorimeter(msc=1, max_events=n_events))]
+    kw = dict(max_events=n_events, profile_kernels=2)
+    forms = [("uniform_helix", lambda: abi.config_calorimeter(**kw)),
+             ("cms_map_helix", lambda: abi.config_calorimeter_cms(stepper=abi.STEPPER_HELIX, **kw)),
+             ("cms_map_rk45", lambda: abi.config_calorimeter_cms(stepper=abi.STEPPER_RK45, **kw)),
+             ("uniform_rk45", lambda: abi.config_calorimeter(field_stepper=abi.STEPPER_RK45, **kw)),
+             ("uniform_helix_msc", lambda: abi.config_calorimeter(msc=1, **kw))]
     out = dict(workload="e- 1-50 GeV x %d into 50 x (2.3 mm Pb + 5.7 mm lAr), FP64" % n_events)
     for name, make in forms:
         with g.Transport(make()) as t:
@@ -136,9 +137,16 @@ def field_forms(g, abi, n_events=20000, seed=2468, cpu=False):
             dt = time.perf_counter() - t0
             ev = t.event_tallies(1, n_events)
             st = t.stats()
+            ks = t.kernel_stats()
+        # the e-/e+ transport against the HBM roofline, as for the headline
+        # (2 x 148 B per e-/e+ step over its HIP-event-timed launches)
+        gbs = 2 * 148 * ks.charged_steps / max(ks.transport_charged_ms * 1e-3, 1e-12) / 1e9
         out[name] = dict(events_per_s=n_events / dt, seconds=dt, track_steps=int(st.track_steps),
                          ns_per_track_step=1e9 * dt / max(int(st.track_steps), 1),
-                         visible_mean_mev=float(ev["visible"].mean()))
+                         visible_mean_mev=float(ev["visible"].mean()),
+                         charged_transport=dict(kernel_ms=ks.transport_charged_ms, launches=int(ks.launches),
+                                                steps=int(ks.charged_steps), achieved_gbs=gbs,
+                                                frac=gbs / PEAK_HBM_GBS))
         log("field %s: %.2f s, %d track-steps" % (name, dt, st.track_steps))
     if cpu:
         # the oracle (FP64, host threads) on the first primaries of the same batch in the map
@@ -251,6 +259,8 @@ def main():
     ap.add_argument("--skip-fp32", action="store_true")
     ap.add_argument("--skip-precision", action="store_true", help="skip the FP32-vs-FP64 calorimeter comparison")
     ap.add_argument("--skip-field", action="store_true", help="skip the field-map and RK45 calorimeter runs")
+    ap.add_argument("--field-only", action="store_true",
+                    help="run only the field-map, RK45 and multiple-scattering calorimeter runs and print them")
     ap.add_argument("--pool", type=int, default=2_000_000, help="live-track pool capacity")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a kernel of known traffic that "
@@ -266,6 +276,9 @@ def main():
     from geantexascalepilot_amd import abi
     import geantexascalepilot_amd.gxt as g
 
+    if a.field_only:
+        print(json.dumps(dict(field_forms=field_forms(g, abi, cpu=not a.skip_cpu))), flush=True)
+        return
     if a.copy_probe > 0:
         src = torch.ones(a.copy_probe << 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
