@@ -7,6 +7,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from geantexascalepilot_amd import abi
@@ -89,6 +90,45 @@ def test_validation_without_device():
     d = abi.Config()
     assert lib.gxt_default_config(C.byref(d)) == 0
     assert d.tracking_cut == 1e-3 and d.pool_capacity == 2_000_000
+
+
+@pytest.mark.parametrize("field,value,needle", [
+    ("field_stepper", 2, b"stepper"),
+    ("field_origin", (0.0, float("nan"), 0.0), b"field"),
+    ("field", (float("inf"), 0.0, 0.0), b"field"),
+    ("precision", 3, b"precision"),
+    ("pool_capacity", 10, b"pool_capacity"),
+])
+def test_config_validation_without_device(field, value, needle):
+    lib = C.CDLL(LIB)
+    lib.gxt_last_error.restype = C.c_char_p
+    cfg = abi.config_calorimeter(**{field: value})
+    ctx = C.c_void_p()
+    assert lib.gxt_create(C.byref(cfg), C.byref(ctx)) == -1
+    assert needle in lib.gxt_last_error()
+
+
+def test_field_step_argument_checks_without_device():
+    """Argument checks of the field stepper come before any device call."""
+    lib = C.CDLL(LIB)
+    lib.gxt_last_error.restype = C.c_char_p
+    pos = np.zeros(3)
+    d = np.array([1.0, 0.0, 0.0])
+    f = (C.c_double * 3)(0.0, 0.0, 4.0)
+
+    def step(stepper=abi.STEPPER_RK45, charge=1.0, mom=100.0, length=1.0, field=f):
+        q, p, s = np.array([charge]), np.array([mom]), np.array([length])
+        return lib.gxt_field_step(None, field, None, stepper, abi.FP64, abi.ptr(q), abi.ptr(p), abi.ptr(s),
+                                  abi.ptr(pos), abi.ptr(d), None, 1)
+
+    assert step(stepper=5) == -1 and b"stepper" in lib.gxt_last_error()
+    assert step(charge=0.5) == -1 and b"charge" in lib.gxt_last_error()
+    assert step(mom=0.0) == -1 and b"momentum" in lib.gxt_last_error()
+    assert step(length=float("nan")) == -1
+    assert step(mom=1.0, length=1e9) == -1 and b"2^16" in lib.gxt_last_error()
+    assert step(field=None) == -1
+    assert lib.gxt_field_map_value(None, abi.ptr(pos), abi.ptr(pos), 1) == -1
+    assert lib.gxt_field_map_free(None) == -1
 
 
 def test_host_gun_matches_oracle():
