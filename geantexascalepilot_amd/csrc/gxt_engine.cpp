@@ -224,7 +224,7 @@ struct gxt_ctx {
   // non-uniform field
   gxt_field_map fmap;
   std::string fmap_path;
-  int general_field = 0;
+  int transport_mode = 0;  // e-/e+ transport kernel (StepArgs::transport_mode)
   // jobs
   std::mutex mu;
   std::deque<std::unique_ptr<Job>> jobs;   // submitted, not finished (FIFO)
@@ -266,7 +266,7 @@ struct gxt_ctx {
     a.rows.nrows = grid;
     a.trace = trace_buf;
     a.trace_cap = trace_cap;
-    a.general_field = general_field;
+    a.transport_mode = transport_mode;
     a.pad = 0;
     return a;
   }
@@ -765,7 +765,7 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
     for (size_t s = 0; s < mats.size(); ++s)
       if (P.mat[s].tab[GXT_TAB_MSC_EM].n == 0 || P.mat[s].tab[GXT_TAB_MSC_EP].n == 0)
         return set_err(GXT_EINVAL, "multiple scattering needs the LambdaMod1.msc tables (none for PbWO4)");
-  c->general_field = P.field.general || P.msc;
+  c->transport_mode = P.field.general ? 2 : (P.msc ? 1 : 0);
   P.tracking_cut = cfg->tracking_cut;
   P.zbin = cfg->zbin_mm;
   P.rbin = cfg->rbin_mm;

@@ -30,7 +30,7 @@ struct StepArgs {
   TallyRows rows;
   double *trace;
   int64_t trace_cap;
-  int general_field;  // e-/e+ transport through the general-field propagation
+  int transport_mode;  // e-/e+ transport: 0 plain, 1 with multiple scattering, 2 general field
   int pad;
 };
 

@@ -467,10 +467,12 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
 // One kernel per region: kCharged = false steps the photons of [0, nN),
 // kCharged = true the electrons and positrons of [cap - nC, cap); the species
 // class is a compile-time constant so each variant carries only its own code.
-// kGeneral: e-/e+ with multiple scattering, in a field map or with the RK45
-// stepper (propagate_general); otherwise straight lines and the exact
-// uniform-field helix, no multiple scattering.
-template <class R, bool kCharged, bool kGeneral = false>
+// kMode 0: straight lines and the exact uniform-field helix, no multiple
+// scattering; 1: the same propagation with e-/e+ multiple scattering; 2: e-/e+
+// in a field map or with the RK45 stepper (propagate_general), with or without
+// multiple scattering.  Each mode is its own kernel, so the simpler ones keep
+// their register budgets.
+template <class R, bool kCharged, int kMode = 0>
 __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a, PoolArgs<R> pa) {
   __shared__ WaveHist whs[kBlock / 64];
   WaveHist &wh = whs[threadIdx.x >> 6];
@@ -540,7 +542,7 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
       // true path is propagated (the step is the true path for interaction
       // lengths and loss)
       double lam1 = kMscNone;
-      if constexpr (kGeneral) {
+      if constexpr (kMode != 0) {
         if (charged && P->msc) {
           lam1 = msc_lambda1<R>(N, m, sp == 2, t.e);
           if (lam1 < kMscNone) {
@@ -558,11 +560,11 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
       const int prevol = t.vol;
       double step = 0.0;
       int status;
-      if constexpr (kGeneral) {
+      if constexpr (kMode != 0) {
         const bool msc_on = lam1 < kMscNone;
         const double request = msc_on ? msc_geom(bestd, lam1) : bestd;
         double gstep = 0.0;
-        if (P->field.general)
+        if constexpr (kMode == 2)
           status = propagate_general<R>(P, sp, t.e, t.pos, t.dir, t.vol, request, gstep);
         else
           status = propagate<R>(P, sp, t.e, t.pos, t.dir, t.vol, request, gstep);
@@ -581,7 +583,7 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
         const double mid[3] = {0.5 * (pre[0] + t.pos[0]), 0.5 * (pre[1] + t.pos[1]), 0.5 * (pre[2] + t.pos[2])};
         dA.add(P, t, (double)loss, mid, prevol);
         edep += (double)loss;
-        if constexpr (kGeneral) {
+        if constexpr (kMode != 0) {
           if (lam1 < kMscNone && status != kEscaped && t.e > R(0) && step > 0.0)
             msc_deflect<R>(step / lam1, t.dir, t.rng);
         }
@@ -1029,8 +1031,10 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
   hipLaunchKernelGGL((k_derive<R, 0>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
   if (ev && ev[3]) (void)hipEventRecord(ev[3], s.main);
   if (ev && ev[4]) (void)hipEventRecord(ev[4], s.side);
-  if (a.general_field)
-    hipLaunchKernelGGL((k_transport<R, true, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
+  if (a.transport_mode == 2)
+    hipLaunchKernelGGL((k_transport<R, true, 2>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
+  else if (a.transport_mode == 1)
+    hipLaunchKernelGGL((k_transport<R, true, 1>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
   else
     hipLaunchKernelGGL((k_transport<R, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
   if (ev && ev[5]) (void)hipEventRecord(ev[5], s.side);
