@@ -224,27 +224,47 @@ def serving(g, abi, seconds, rate_jobs_s, seed=4242):
                 rate_jobs_s=rate_jobs_s, max_live=int(st.max_live))
 
 
-def cpu_baseline(abi, target_s=15.0):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(abi, target_s=15.0, single_s=4.0):
+    """The oracle on the host: one worker per core (up to the 16 the GPU host
+    allots this run), events handed out dynamically, on a bounded first slice
+    of the same batch; then on one core."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_harness as O
     threads = max(1, min(16, os.cpu_count() or 1))
     cfg = abi.config_pbwo4_block(max_events=100000)
 
-    def timed(n):
+    def timed(n, nt):
         o = O.Oracle(cfg)
         p = O.make_primaries(777, abi.ELECTRON, 10000.0, 10000.0, 0.0, n)
         t0 = time.perf_counter()
-        o.run(p, threads)
+        o.run(p, nt)
         dt = time.perf_counter() - t0
         return dt, o.stats().track_steps
 
-    dt, _ = timed(threads)
+    dt, _ = timed(threads, threads)
     n = int(max(threads, min(10000, threads * target_s / max(dt, 1e-3))))
-    dt, steps = timed(n)
+    dt, steps = timed(n, threads)
     log("cpu oracle: %d events on %d threads in %.2f s" % (n, threads, dt))
+    per_event_1 = dt * threads / n  # estimate of one core's time per event
+    n1 = int(max(1, min(n, single_s / max(per_event_1, 1e-6))))
+    dt1, steps1 = timed(n1, 1)
+    log("cpu oracle: %d events on 1 thread in %.2f s" % (n1, dt1))
     return dict(value=n / dt, unit="events/s", cores=threads, kind="port",
                 sample="first %d primaries of the same 10 GeV e- / PbWO4 batch (FP64 oracle, %d track-steps, %.1f s)"
-                % (n, steps, dt), track_steps_per_s=steps / dt)
+                % (n, steps, dt), track_steps_per_s=steps / dt, cpu_model=cpu_model(),
+                host_cores=os.cpu_count(),
+                single_core=dict(value=n1 / dt1, track_steps_per_s=steps1 / dt1,
+                                 sample="first %d primaries (%d track-steps, %.1f s)" % (n1, steps1, dt1)))
 
 
 def main():
@@ -311,7 +331,9 @@ def main():
     roofline = dict(bound="hbm", achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s", frac=achieved / PEAK_HBM_GBS,
                     traffic=profiled_traffic(name), kernel=name, launches=int(ks.launches), avg_launch_us=avg_us,
                     bytes_per_launch=nbytes / launches, per_unit=per_unit, record_bytes_in_pool=int(ks.record_bytes),
-                    kernel_ms=ms)
+                    kernel_ms=ms,
+                    # every track-step of the run (photons included) at 2 x 148 B over the wall time
+                    whole_run_gbs=ks.track_steps * 2 * rec / r64["seconds"] / 1e9)
     e64 = r64["edep"]["edep"]
     out = dict(metric="events/s and track-steps/s, FP32 vs FP64 on 1 MI355X; p99 job latency", value=value,
                unit="events/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
