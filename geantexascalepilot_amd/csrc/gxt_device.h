@@ -1256,13 +1256,15 @@ __device__ __forceinline__ void ck45_step(const DevField &F, R kq, double h, con
 constexpr int kRkMaxSub = 16;
 constexpr int kRkMaxN = 65537;
 
-// Advances a charged track by s with the configured stepper: a helix in the
-// start-point field B (|B| = bm; straight line where it vanishes), or
-// ceil(s k bm / GXT_RK_ANGLE) equal Cash-Karp steps, k = 0.299792458 / p.
-template <class R>
+// Advances a charged track by s with the stepper (kStep, or F.stepper when
+// kStep is -1): a helix in the start-point field B (|B| = bm; straight line
+// where it vanishes), or ceil(s k bm / GXT_RK_ANGLE) equal Cash-Karp steps,
+// k = 0.299792458 / p.
+template <class R, int kStep = -1>
 __device__ __forceinline__ void field_advance(const DevField &F, const double B[3], double bm, R q, R mom, double s,
                                               double pos[3], R dir[3], double &ex, double &eu) {
-  if (F.stepper == GXT_STEPPER_RK45) {
+  const bool rk = kStep < 0 ? F.stepper == GXT_STEPPER_RK45 : kStep == GXT_STEPPER_RK45;
+  if (rk) {
     // sub-steps: at most kRkMaxN, whatever the inputs (an infinite turn included)
     const double turn = s * (-kB2C) * bm / (double)mom;
     const int n = turn > GXT_RK_ANGLE ? (int)fmin(ceil(turn / GXT_RK_ANGLE), (double)kRkMaxN) : 1;
@@ -1293,15 +1295,16 @@ __device__ __forceinline__ void field_advance(const DevField &F, const double B[
 // start of every sub-step (FieldPropagationHandler::Propagate,
 // FieldPropagationHandler.cpp:93-111: step min(pstep or snext, max(SafeLength
 // in the local field, safety))).  RK45 sub-steps are also kept within
-// kRkMaxSub Cash-Karp steps.
-template <class R>
+// kRkMaxSub Cash-Karp steps.  kStep: the stepper (GXT_STEPPER_*), fixed at
+// compile time so each kernel carries one.
+template <class R, int kStep>
 __device__ __forceinline__ int propagate_general(const DevParams *P, int species, R e, double pos[3], R dir[3],
                                                  int &vol, double pstep, double &step) {
   const DevGeometry &g = P->geo;
   const DevField &F = P->field;
   const R q = species == 1 ? R(-1) : R(1);
   const R mom = m_sqrt(e * (e + R(2) * R(kMe)));
-  const bool rk = F.stepper == GXT_STEPPER_RK45;
+  constexpr bool rk = kStep == GXT_STEPPER_RK45;
   double ex = 0.0, eu = 0.0;
   double remaining = pstep;
   step = 0.0;
@@ -1312,7 +1315,7 @@ __device__ __forceinline__ int propagate_general(const DevParams *P, int species
     const double bm = sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
     const double smax = rk && bm > 0.0 ? kRkMaxSub * GXT_RK_ANGLE * (double)mom / ((-kB2C) * bm) : kHuge;
     if (remaining <= saf && remaining <= smax) {
-      field_advance<R>(F, B, bm, q, mom, remaining, pos, dir, ex, eu);
+      field_advance<R, kStep>(F, B, bm, q, mom, remaining, pos, dir, ex, eu);
       step += remaining;
       return kPhysics;
     }
@@ -1329,7 +1332,7 @@ __device__ __forceinline__ int propagate_general(const DevParams *P, int species
     if (slin <= sl && slin <= remaining && slin <= smax) {
       // aimed step: cross the aimed face by construction, on its plane
       const double s = fmax(slin, 1.0e-6);
-      field_advance<R>(F, B, bm, q, mom, s, pos, dir, ex, eu);
+      field_advance<R, kStep>(F, B, bm, q, mom, s, pos, dir, ex, eu);
       step += s;
       if (face == 0 || face == 1) {
         const int nv = face == 1 ? vol + 1 : vol - 1;
@@ -1341,7 +1344,7 @@ __device__ __forceinline__ int propagate_general(const DevParams *P, int species
       return kEscaped;
     }
     const double s = fmin(fmin(remaining, fmax(sl, saf)), smax);
-    field_advance<R>(F, B, bm, q, mom, s, pos, dir, ex, eu);
+    field_advance<R, kStep>(F, B, bm, q, mom, s, pos, dir, ex, eu);
     step += s;
     remaining -= s;
     const int v2 = locate(g, pos, (double)dir[0], (double)dir[1], (double)dir[2]);

@@ -765,7 +765,8 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
     for (size_t s = 0; s < mats.size(); ++s)
       if (P.mat[s].tab[GXT_TAB_MSC_EM].n == 0 || P.mat[s].tab[GXT_TAB_MSC_EP].n == 0)
         return set_err(GXT_EINVAL, "multiple scattering needs the LambdaMod1.msc tables (none for PbWO4)");
-  c->transport_mode = P.field.general ? 2 : (P.msc ? 1 : 0);
+  c->transport_mode =
+      P.field.general ? (P.field.stepper == GXT_STEPPER_RK45 ? 3 : 2) : (P.msc ? 1 : 0);
   P.tracking_cut = cfg->tracking_cut;
   P.zbin = cfg->zbin_mm;
   P.rbin = cfg->rbin_mm;

@@ -30,7 +30,8 @@ struct StepArgs {
   TallyRows rows;
   double *trace;
   int64_t trace_cap;
-  int transport_mode;  // e-/e+ transport: 0 plain, 1 with multiple scattering, 2 general field
+  int transport_mode;  // e-/e+ transport: 0 plain, 1 with multiple scattering, 2 field map with the
+                       // local helix, 3 RK45 stepper
   int pad;
 };
 

@@ -469,7 +469,8 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
 // class is a compile-time constant so each variant carries only its own code.
 // kMode 0: straight lines and the exact uniform-field helix, no multiple
 // scattering; 1: the same propagation with e-/e+ multiple scattering; 2: e-/e+
-// in a field map or with the RK45 stepper (propagate_general), with or without
+// in a field map with the local helix, 3: with the RK45 stepper (map or
+// uniform field), both through propagate_general and with or without
 // multiple scattering.  Each mode is its own kernel, so the simpler ones keep
 // their register budgets.
 template <class R, bool kCharged, int kMode = 0>
@@ -565,7 +566,9 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a
         const double request = msc_on ? msc_geom(bestd, lam1) : bestd;
         double gstep = 0.0;
         if constexpr (kMode == 2)
-          status = propagate_general<R>(P, sp, t.e, t.pos, t.dir, t.vol, request, gstep);
+          status = propagate_general<R, GXT_STEPPER_HELIX>(P, sp, t.e, t.pos, t.dir, t.vol, request, gstep);
+        else if constexpr (kMode == 3)
+          status = propagate_general<R, GXT_STEPPER_RK45>(P, sp, t.e, t.pos, t.dir, t.vol, request, gstep);
         else
           status = propagate<R>(P, sp, t.e, t.pos, t.dir, t.vol, request, gstep);
         step = msc_on ? (status == kPhysics ? bestd : msc_true(gstep, lam1, bestd)) : gstep;
@@ -1031,7 +1034,9 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
   hipLaunchKernelGGL((k_derive<R, 0>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
   if (ev && ev[3]) (void)hipEventRecord(ev[3], s.main);
   if (ev && ev[4]) (void)hipEventRecord(ev[4], s.side);
-  if (a.transport_mode == 2)
+  if (a.transport_mode == 3)
+    hipLaunchKernelGGL((k_transport<R, true, 3>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
+  else if (a.transport_mode == 2)
     hipLaunchKernelGGL((k_transport<R, true, 2>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
   else if (a.transport_mode == 1)
     hipLaunchKernelGGL((k_transport<R, true, 1>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
