@@ -346,3 +346,18 @@ def test_hepevt_events_parity(gpu_lib, tmp_path):
     _compare_events(gpu, cpu)
     for a, b in zip(gprof, cprof):
         np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-9)
+
+
+def test_pool_overflow_is_reported(gpu_lib):
+    """A shower larger than the pool ends the run with GXT_ECAPACITY (every
+    write past a stripe is dropped and flagged, the pool is discarded), and
+    the context is destroyed cleanly."""
+    _flush("pool overflow")
+    cfg = abi.config_pbwo4_block(max_events=1, pool_capacity=1024)
+    p = gpu_lib.make_primaries(cfg.seed, abi.ELECTRON, 10000.0, 10000.0, 0.0, 1)
+    with gpu_lib.Transport(cfg) as t:
+        t.submit(p, 1)
+        with pytest.raises(gpu_lib.GxtError) as e:
+            t.run(max_iterations=10000)
+        assert e.value.code == -4
+        assert "pool" in str(e.value)
