@@ -224,6 +224,46 @@ def serving(g, abi, seconds, rate_jobs_s, seed=4242):
                 rate_jobs_s=rate_jobs_s, max_live=int(st.max_live))
 
 
+def c0_block(g, abi, cpu=True):
+    """The reference's own CPU-runnable case: 1000 gammas of 1 GeV into the
+    20 cm lead block, seed 12345, in full on the GPU and on the host (all
+    threads, then one), FP64."""
+    n = 1000
+    cfg = abi.config_lead_block(max_events=n)
+    prims = g.make_primaries(12345, abi.GAMMA, 1000.0, 1000.0, 0.0, n)
+    import torch
+    with g.Transport(cfg) as t:
+        t.submit(prims, 0)
+        t.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t.submit(prims, 1)
+        t.run()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        st = t.stats()
+        ev = t.event_tallies(1, n)
+    out = dict(workload="1 GeV gamma x %d into the 20x20x20 cm Pb block, FP64" % n, gpu_events_per_s=n / dt,
+               gpu_seconds=dt, edep_mean_mev=float(ev["edep"].mean()))
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_harness as O
+        threads = max(1, min(16, os.cpu_count() or 1))
+        for nt, key in ((threads, "cpu_all_threads"), (1, "cpu_one_core")):
+            o = O.Oracle(cfg)
+            p = O.make_primaries(12345, abi.GAMMA, 1000.0, 1000.0, 0.0, n)
+            t0 = time.perf_counter()
+            o.run(p, nt)
+            dtc = time.perf_counter() - t0
+            steps = int(o.stats().track_steps)
+            cev = o.event_tallies(n)
+            o.close()
+            out[key] = dict(events_per_s=n / dtc, track_steps_per_s=steps / dtc, threads=nt, seconds=dtc)
+        out["edep_mean_rel_diff_gpu_vs_oracle"] = float(ev["edep"].mean() / cev["edep"].mean() - 1.0)
+    log("C0: GPU %.3f s for %d events" % (dt, n))
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -364,6 +404,7 @@ def main():
         rate = 0.6 * gev_per_s / (0.4 * 1.0 + 0.4 * 100.0 + 0.2 * 5000.0)
         log("serving for %.0f s at %.2f jobs/s" % (a.serve_seconds, rate))
         out["serving"] = serving(g, abi, a.serve_seconds, rate)
+    out["c0"] = c0_block(g, abi, cpu=not a.skip_cpu)
     if not a.skip_cpu:
         out["cpu_baseline"] = cpu_baseline(abi)
     print(json.dumps(out))
