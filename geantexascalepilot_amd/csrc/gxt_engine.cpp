@@ -171,6 +171,7 @@ double now_ms(std::chrono::steady_clock::time_point t0) {
 struct Job {
   uint64_t id = 0;
   std::vector<gxt_primary> prims;   // sorted by event
+  gxt_primary *dprims = nullptr;    // device copy of prims (made at submission, freed when the job ends)
   std::vector<int> ev_first;        // prims[ev_first[e] .. ev_first[e+1]) belong to event e
   std::vector<double> ev_gev;       // summed primary energy of each event (GeV)
   std::vector<int> ev_max;          // highest live-track count seen for an event in flight
@@ -215,10 +216,6 @@ struct gxt_ctx {
   int64_t row_stride = 0;
   double *trace_buf = nullptr;
   int64_t trace_cap = 0;
-  gxt_primary *stage = nullptr;
-  int64_t stage_cap = 0;
-  gxt_primary *hstage = nullptr;  // pinned host staging of the primaries of one admission
-  int64_t hstage_cap = 0;
   hipEvent_t snap = nullptr;      // recorded after a round's counter snapshot
   size_t ev_snap = 0;             // profiling entries recorded before that snapshot
   // non-uniform field
@@ -320,11 +317,16 @@ int alloc_events(gxt_ctx *c, int n) {
 }
 
 // Admit events, oldest job first, while their estimated footprint fits the
-// pool budget; inject their primaries with one kernel launch.  Events are
-// admitted whole (all primaries of an event together).
+// pool budget; inject their primaries from the job's device copy, one kernel
+// launch per job.  Events are admitted whole (all primaries of an event
+// together) and in order, so a job's admitted primaries are contiguous.
 int admit(gxt_ctx *c) {
   const double budget = 0.8 * (double)c->cap;
-  std::vector<gxt_primary> buf;
+  struct Range {
+    const gxt_primary *prim;
+    int n, event_base;
+  };
+  std::vector<Range> ranges;
   std::vector<std::pair<int, int>> zero_ranges;
   {
     std::lock_guard<std::mutex> lk(c->mu);
@@ -343,6 +345,7 @@ int admit(gxt_ctx *c) {
       }
       ++active;
       bool full = false;
+      const int e0 = j->next_event;
       while (j->next_event < j->n_events) {
         const int e = j->next_event;
         const double fp = event_peak(c, j, e);
@@ -350,56 +353,37 @@ int admit(gxt_ctx *c) {
           full = true;
           break;
         }
-        for (int k = j->ev_first[e]; k < j->ev_first[e + 1]; ++k) {
-          gxt_primary p = j->prims[k];
-          p.event += j->ev_base;
-          buf.push_back(p);
-        }
         j->ev_state[e] = 1;
         c->committed += fp;
         j->next_event++;
       }
+      if (j->next_event > e0) {
+        const int p0 = j->ev_first[e0], p1 = j->ev_first[j->next_event];
+        ranges.push_back(Range{j->dprims + p0, p1 - p0, j->ev_base});
+      }
       if (full) break;
     }
   }
-  // everything below is queued on the stream behind the work in flight; the
-  // slots of a new job belong to events without tracks, so the look-ahead
-  // iterations still running do not touch them
+  // everything below is queued on the stream behind the work in flight (the
+  // jobs' primaries were copied to the device at submission, earlier on the
+  // same stream); the slots of a new job belong to events without tracks, so
+  // the look-ahead iterations still running do not touch them
   for (auto &r : zero_ranges) {
     HIP_TRY(hipMemsetAsync(c->ev + r.first, 0, r.second * sizeof(DevEvent), c->stream));
     HIP_TRY(hipMemsetAsync(c->ev_live + r.first, 0, r.second * sizeof(int), c->stream));
   }
-  if (buf.empty()) return GXT_OK;
-  const int64_t total = (int64_t)buf.size();
-  if (total > c->stage_cap || total > c->hstage_cap) {
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if (total > c->stage_cap) {
-      (void)hipFree(c->stage);
-      c->stage = nullptr;
-      c->stage_cap = std::max<int64_t>(total, 4096);
-      HIP_TRY(dalloc(&c->stage, c->stage_cap));
-    }
-    if (total > c->hstage_cap) {
-      (void)hipHostFree(c->hstage);
-      c->hstage = nullptr;
-      c->hstage_cap = std::max<int64_t>(total, 4096);
-      HIP_TRY(hipHostMalloc((void **)&c->hstage, c->hstage_cap * sizeof(gxt_primary), hipHostMallocDefault));
-    }
-  }
-  // the previous admission's copy from hstage finished before the last
-  // snapshot, which the host has waited for
-  std::memcpy(c->hstage, buf.data(), buf.size() * sizeof(gxt_primary));
-  HIP_TRY(hipMemcpyAsync(c->stage, c->hstage, buf.size() * sizeof(gxt_primary), hipMemcpyHostToDevice, c->stream));
   const StepArgs a = c->args();
-  hipError_t e;
-  if (c->precision == GXT_FP32) {
-    PoolArgs<float> pa{c->pf[c->parity].p, c->pf[1 - c->parity].p};
-    e = launch_inject<float>(a, pa, c->stage, (int)total, c->stream);
-  } else {
-    PoolArgs<double> pa{c->pd[c->parity].p, c->pd[1 - c->parity].p};
-    e = launch_inject<double>(a, pa, c->stage, (int)total, c->stream);
+  for (const Range &r : ranges) {
+    hipError_t e;
+    if (c->precision == GXT_FP32) {
+      PoolArgs<float> pa{c->pf[c->parity].p, c->pf[1 - c->parity].p};
+      e = launch_inject<float>(a, pa, r.prim, r.n, r.event_base, c->stream);
+    } else {
+      PoolArgs<double> pa{c->pd[c->parity].p, c->pd[1 - c->parity].p};
+      e = launch_inject<double>(a, pa, r.prim, r.n, r.event_base, c->stream);
+    }
+    if (e != hipSuccess) return c->fail(GXT_EHIP, std::string("inject: ") + hipGetErrorString(e));
   }
-  if (e != hipSuccess) return c->fail(GXT_EHIP, std::string("inject: ") + hipGetErrorString(e));
   return GXT_OK;
 }
 
@@ -524,6 +508,9 @@ int collect(gxt_ctx *c, bool &idle) {
       if (it->get() == j) {
         std::unique_ptr<Job> owned = std::move(*it);
         c->jobs.erase(it);
+        // its injections ran before the snapshot that showed it finished
+        if (owned->dprims) HIP_TRY(hipFreeAsync(owned->dprims, c->stream));
+        owned->dprims = nullptr;
         owned->prims.clear();
         owned->prims.shrink_to_fit();
         c->done_unpolled.emplace_back(new Job(*owned));  // completion order for gxt_poll
@@ -614,7 +601,8 @@ void destroy_ctx(gxt_ctx *c) {
   c->dd.release();
   release_field_map(c->fmap);
   if (c->snap) (void)hipEventDestroy(c->snap);
-  (void)hipHostFree(c->hstage);
+  for (auto &jp : c->jobs)
+    if (jp->dprims) (void)hipFree(jp->dprims);
   (void)hipFree(c->queues);
   (void)hipFree(c->newlist);
   (void)hipFree(c->cnt);
@@ -625,7 +613,6 @@ void destroy_ctx(gxt_ctx *c) {
   (void)hipFree(c->rows);
   (void)hipFree(c->spec);
   (void)hipFree(c->trace_buf);
-  (void)hipFree(c->stage);
   if (c->ss.fork) (void)hipEventDestroy(c->ss.fork);
   if (c->ss.join) (void)hipEventDestroy(c->ss.join);
   if (c->ss.side) (void)hipStreamDestroy(c->ss.side);
@@ -847,6 +834,16 @@ int gxt_submit(gxt_ctx *c, const gxt_primary *prim, int n, uint64_t job_id) {
   if (c->finished.count(job_id)) return c->fail(GXT_EINVAL, "job id already used");
   for (const auto &q : c->jobs)
     if (q->id == job_id) return c->fail(GXT_EINVAL, "job id already used");
+  // the primaries go to the device now, on the context's stream, ahead of
+  // any admission of the job (which follows it in stream order); the host
+  // copy stays alive until the job ends
+  const size_t bytes = j->prims.size() * sizeof(gxt_primary);
+  HIP_TRY(hipMallocAsync((void **)&j->dprims, bytes, c->stream));
+  const hipError_t ce = hipMemcpyAsync(j->dprims, j->prims.data(), bytes, hipMemcpyHostToDevice, c->stream);
+  if (ce != hipSuccess) {
+    (void)hipFreeAsync(j->dprims, c->stream);
+    return set_err(GXT_EHIP, std::string("submit copy: ") + hipGetErrorString(ce));
+  }
   c->jobs.push_back(std::move(j));
   return GXT_OK;
 }

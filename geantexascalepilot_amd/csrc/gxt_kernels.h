@@ -72,7 +72,9 @@ template <class R>
 hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const StepStreams &s,
                        hipEvent_t *ev = nullptr);
 template <class R>
-hipError_t launch_inject(const StepArgs &a, const PoolArgs<R> &pa, const gxt_primary *prim, int n, hipStream_t st);
+// Injects n primaries from device memory; event slots are prim[i].event + event_base.
+hipError_t launch_inject(const StepArgs &a, const PoolArgs<R> &pa, const gxt_primary *prim, int n, int event_base,
+                         hipStream_t st);
 
 hipError_t launch_rng_init(const DevParams *P, const uint32_t *seed, const uint64_t *idx, int n, uint32_t *out,
                            hipStream_t st);

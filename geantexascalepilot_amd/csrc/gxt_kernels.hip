@@ -420,7 +420,8 @@ __device__ __forceinline__ void gpil(const DevParams *P, const Nodes<R> &N, cons
 
 // ------------------------------------------------------------------ step loop kernels
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, const gxt_primary *prim, int n) {
+__global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, const gxt_primary *prim, int n,
+                                                   int event_base) {
   const DevParams *P = a.P;
   const Pool<R> &cur = pa.cur;
   const bool trace = P->trace_events > 0;
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
     if (active) {
       const gxt_primary &p = prim[i];
       t.species = p.species;
-      t.event = p.event;
+      t.event = p.event + event_base;
       t.e = (R)p.ekin;
       for (int k = 0; k < 3; ++k) {
         t.pos[k] = p.pos[k];
@@ -1057,11 +1058,12 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
 }
 
 template <class R>
-hipError_t launch_inject(const StepArgs &a, const PoolArgs<R> &pa, const gxt_primary *prim, int n, hipStream_t st) {
+hipError_t launch_inject(const StepArgs &a, const PoolArgs<R> &pa, const gxt_primary *prim, int n, int event_base,
+                         hipStream_t st) {
   if (n <= 0) return hipSuccess;
   int g = grid_for(n);
   if (g > 1024) g = 1024;
-  hipLaunchKernelGGL(k_inject<R>, dim3(g), dim3(kBlock), 0, st, a, pa, prim, n);
+  hipLaunchKernelGGL(k_inject<R>, dim3(g), dim3(kBlock), 0, st, a, pa, prim, n, event_base);
   return hipGetLastError();
 }
 
@@ -1069,9 +1071,9 @@ template hipError_t launch_step<double>(const StepArgs &, const PoolArgs<double>
                                         hipEvent_t *);
 template hipError_t launch_step<float>(const StepArgs &, const PoolArgs<float> &, int, const StepStreams &,
                                        hipEvent_t *);
-template hipError_t launch_inject<double>(const StepArgs &, const PoolArgs<double> &, const gxt_primary *, int,
+template hipError_t launch_inject<double>(const StepArgs &, const PoolArgs<double> &, const gxt_primary *, int, int,
                                           hipStream_t);
-template hipError_t launch_inject<float>(const StepArgs &, const PoolArgs<float> &, const gxt_primary *, int,
+template hipError_t launch_inject<float>(const StepArgs &, const PoolArgs<float> &, const gxt_primary *, int, int,
                                          hipStream_t);
 
 hipError_t launch_rng_init(const DevParams *P, const uint32_t *seed, const uint64_t *idx, int n, uint32_t *out,
