@@ -208,8 +208,10 @@ int gxt_egamma_gun(uint32_t seed, int n_events, int per_event, double gamma_frac
 
 /* ---- batch transport --------------------------------------------------- */
 /* Queue a job of n primaries (replaces ProxyDeviceManager::UploadTrackState,
- * ProxyDeviceManager.cpp:152-204).  Non-blocking; the job is admitted into
- * the shared pool when its estimated footprint fits. */
+ * ProxyDeviceManager.cpp:152-204).  The primaries are copied to device memory
+ * at once, stream-ordered on the context's stream (prim may be reused after
+ * the call); the job is admitted into the shared pool when its estimated
+ * footprint fits. */
 int gxt_submit(gxt_ctx *ctx, const gxt_primary *prim, int n, uint64_t job_id);
 /* Run the step loop until every submitted job has finished (replaces
  * ProxyDeviceManager::DoStep/DoGPIL, ProxyDeviceManager.cpp:206-276, and the
