@@ -322,6 +322,7 @@ def main():
     ap.add_argument("--skip-fp32", action="store_true")
     ap.add_argument("--skip-precision", action="store_true", help="skip the FP32-vs-FP64 calorimeter comparison")
     ap.add_argument("--skip-field", action="store_true", help="skip the field-map and RK45 calorimeter runs")
+    ap.add_argument("--skip-c0", action="store_true", help="skip the 1000 x 1 GeV gamma lead-block run")
     ap.add_argument("--field-only", action="store_true",
                     help="run only the field-map, RK45 and multiple-scattering calorimeter runs and print them")
     ap.add_argument("--pool", type=int, default=2_000_000, help="live-track pool capacity")
@@ -407,7 +408,8 @@ def main():
         rate = 0.6 * gev_per_s / (0.4 * 1.0 + 0.4 * 100.0 + 0.2 * 5000.0)
         log("serving for %.0f s at %.2f jobs/s" % (a.serve_seconds, rate))
         out["serving"] = serving(g, abi, a.serve_seconds, rate)
-    out["c0"] = c0_block(g, abi, cpu=not a.skip_cpu)
+    if not a.skip_c0:
+        out["c0"] = c0_block(g, abi, cpu=not a.skip_cpu)
     if not a.skip_cpu:
         out["cpu_baseline"] = cpu_baseline(abi)
     print(json.dumps(out))
