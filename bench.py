@@ -170,6 +170,78 @@ def field_forms(g, abi, n_events=20000, seed=2468, cpu=False):
     return out
 
 
+def navigation(g, abi, n=1 << 24, reps=20, cpu=False, seed=99):
+    """Volume-tree navigation in the reference's calorimeter (50 layers of
+    10 mm Pb + 10 mm lAr, 1 m square, 152 touchables): n points uniform in the
+    calorimeter with isotropic directions, resident in HBM, located once, then
+    the boundary step (distance, next touchable, safety) of every point, timed
+    with HIP events on the null stream, where the launches go.  Algorithmic
+    bytes per query: position and direction 48 + touchable 4 read, distance
+    and safety 16 + next touchable 4 written = 72 B."""
+    import ctypes as C
+    import numpy as np
+    rs = np.random.default_rng(seed)
+    pos = rs.uniform(-500.0, 500.0, (3, n))
+    d = rs.normal(size=(3, n))
+    d /= np.linalg.norm(d, axis=0)
+    nav = g.Navigator(*abi.reference_calorimeter_tree())
+    D = g.DeviceArray
+    dp, dd = D(pos), D(d)
+    dt, dn = D(np.zeros(n, dtype=np.int32)), D(np.zeros(n, dtype=np.int32))
+    ds, dsaf = D(np.zeros(n)), D(np.zeros(n))
+    hip = g.hip_runtime()
+    ev = [C.c_void_p() for _ in range(4)]
+    for e in ev:
+        assert hip.hipEventCreate(C.byref(e)) == 0
+
+    def locate():
+        g._check(g.lib.gxt_nav_locate(nav.h, dp.ptr, dd.ptr, dt.ptr, n))
+
+    def step():
+        g._check(g.lib.gxt_nav_step(nav.h, dt.ptr, dp.ptr, dd.ptr, ds.ptr, dn.ptr, dsaf.ptr, n))
+
+    def timed(fn, e0, e1):
+        fn()
+        assert hip.hipDeviceSynchronize() == 0
+        assert hip.hipEventRecord(e0, None) == 0
+        for _ in range(reps):
+            fn()
+        assert hip.hipEventRecord(e1, None) == 0
+        assert hip.hipEventSynchronize(e1) == 0
+        ms = C.c_float()
+        assert hip.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
+        return 1e3 * ms.value / reps
+
+    loc_us = timed(locate, ev[0], ev[1])
+    us = timed(step, ev[2], ev[3])
+    nxt, s = dn.numpy(), ds.numpy()
+    achieved = 72.0 * n / (us * 1e-6) / 1e9
+    out = dict(workload="%d points uniform in the reference calorimeter (50 x (10 mm Pb + 10 mm lAr), 1 m "
+                        "square, 152 touchables), isotropic directions, FP64" % n,
+               step_queries_per_s=n / (us * 1e-6), step_launch_us=us, locate_queries_per_s=n / (loc_us * 1e-6),
+               locate_launch_us=loc_us, mean_snext_mm=float(s.mean()), into_world=int((nxt == 0).sum()),
+               roofline=dict(bound="hbm", kernel="k_nav_step", achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s",
+                             frac=achieved / PEAK_HBM_GBS, per_unit="72 B per query"))
+    log("navigation: step %.1f us, locate %.1f us for %d points" % (us, loc_us, n))
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_harness as O
+        m = 1 << 22
+        o = O.Navigator(*abi.reference_calorimeter_tree())
+        ht = dt.numpy()[:m]
+        t0 = time.perf_counter()
+        o.step(ht, pos[:, :m], d[:, :m])
+        dt_s = time.perf_counter() - t0
+        out["cpu_oracle"] = dict(step_queries_per_s=m / dt_s, cores=1,
+                                 sample="first %d points, one thread, %.2f s" % (m, dt_s))
+    for e in ev:
+        hip.hipEventDestroy(e)
+    for a in (dp, dd, dt, dn, ds, dsaf):
+        a.free()
+    nav.close()
+    return out
+
+
 def serving(g, abi, seconds, rate_jobs_s, seed=4242):
     """Poisson stream of jobs (40 % one 1 GeV gamma, 40 % 10 x 10 GeV e-,
     20 % 100 x 50 GeV e-) into the PbWO4 block, FP32 form, one shared pool."""
@@ -323,6 +395,8 @@ def main():
     ap.add_argument("--skip-precision", action="store_true", help="skip the FP32-vs-FP64 calorimeter comparison")
     ap.add_argument("--skip-field", action="store_true", help="skip the field-map and RK45 calorimeter runs")
     ap.add_argument("--skip-c0", action="store_true", help="skip the 1000 x 1 GeV gamma lead-block run")
+    ap.add_argument("--skip-nav", action="store_true", help="skip the volume-tree navigation queries")
+    ap.add_argument("--nav-only", action="store_true", help="run only the volume-tree navigation queries")
     ap.add_argument("--field-only", action="store_true",
                     help="run only the field-map, RK45 and multiple-scattering calorimeter runs and print them")
     ap.add_argument("--pool", type=int, default=2_000_000, help="live-track pool capacity")
@@ -340,6 +414,9 @@ def main():
     from geantexascalepilot_amd import abi
     import geantexascalepilot_amd.gxt as g
 
+    if a.nav_only:
+        print(json.dumps(dict(navigation=navigation(g, abi, cpu=not a.skip_cpu))), flush=True)
+        return
     if a.field_only:
         print(json.dumps(dict(field_forms=field_forms(g, abi, cpu=not a.skip_cpu))), flush=True)
         return
@@ -403,6 +480,8 @@ def main():
         out["precision_form"] = precision_form(g, abi)
     if not a.skip_field:
         out["field_forms"] = field_forms(g, abi, cpu=not a.skip_cpu)
+    if not a.skip_nav:
+        out["navigation"] = navigation(g, abi, cpu=not a.skip_cpu)
     if not a.skip_serving:
         gev_per_s = (out.get("fp32", {}).get("value", value)) * 10.0
         rate = 0.6 * gev_per_s / (0.4 * 1.0 + 0.4 * 100.0 + 0.2 * 5000.0)

@@ -85,6 +85,51 @@ class SampleIO(C.Structure):
                 ("sec_ekin", C.POINTER(C.c_double)), ("sec_dir", C.POINTER(C.c_double))]
 
 
+class BoxVolume(C.Structure):
+    _fields_ = [("half", C.c_double * 3), ("material", C.c_int32), ("pad", C.c_int32)]
+
+
+class Placement(C.Structure):
+    _fields_ = [("mother", C.c_int32), ("daughter", C.c_int32), ("translation", C.c_double * 3)]
+
+
+def box_tree(volumes, placements):
+    """ctypes arrays from [(half_xyz, material)] and [(mother, daughter, translation_xyz)]."""
+    vol = (BoxVolume * len(volumes))()
+    for v, (half, mat) in zip(vol, volumes):
+        v.half[:] = [float(x) for x in half]
+        v.material = mat
+    pl = (Placement * max(len(placements), 1))()
+    for q, (m, d, t) in zip(pl, placements):
+        q.mother, q.daughter = m, d
+        q.translation[:] = [float(x) for x in t]
+    return vol, pl
+
+
+def reference_calorimeter_tree(n_layers=50, thickness=(10.0, 10.0), materials=(MAT_PB, MAT_LAR), size_yz=1000.0):
+    """The reference's calorimeter as a volume tree
+    (proxy/test/BasicCpuTransport/ProxyDetectorConstruction.cpp:150-164, 225-325):
+    world (1.2 x the calorimeter, vacuum) > calorimeter > n_layers layers along x
+    > one box per absorber.  Returns (volumes, placements, world index)."""
+    layer_x = float(sum(thickness))
+    calo_x = n_layers * layer_x
+    volumes = [((0.6 * calo_x, 0.6 * size_yz, 0.6 * size_yz), -1),
+               ((0.5 * layer_x, 0.5 * size_yz, 0.5 * size_yz), -1),
+               ((0.5 * calo_x, 0.5 * size_yz, 0.5 * size_yz), -1)]
+    placements = []
+    start = -0.5 * layer_x
+    for i, (t, m) in enumerate(zip(thickness, materials)):
+        volumes.append(((0.5 * t, 0.5 * size_yz, 0.5 * size_yz), m))
+        placements.append((1, 3 + i, (start + 0.5 * t, 0.0, 0.0)))
+        start += t
+    start = -0.5 * calo_x
+    for i in range(n_layers):
+        placements.append((2, 1, (start + 0.5 * layer_x, 0.0, 0.0)))
+        start += layer_x
+    placements.append((0, 2, (0.0, 0.0, 0.0)))
+    return volumes, placements, 0
+
+
 def ptr(a, ctype=C.c_double):
     return a.ctypes.data_as(C.POINTER(ctype))
 

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgxt.so")
-SOURCES = ["gxt_kernels.hip", "gxt_engine.cpp", "gxt_data.cpp", "gxt_events.cpp", "gxt_field.cpp"]
+SOURCES = ["gxt_kernels.hip", "gxt_engine.cpp", "gxt_data.cpp", "gxt_events.cpp", "gxt_field.cpp", "gxt_nav.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-parameter",

@@ -921,6 +921,124 @@ __device__ __forceinline__ double safety(const DevGeometry &g, int v, const doub
   return s < 0 ? 0.0 : s;
 }
 
+// ------------------------------------------------------- volume-tree navigator
+// Boxes placed by translations inside each other (VecGeom's UnplacedBox and
+// PlaceDaughter as the reference builds its calorimeter,
+// proxy/test/BasicCpuTransport/ProxyDetectorConstruction.cpp:225-325), with
+// VecGeom's surface tolerance (NavigationInterface.hpp:66).  A point within
+// kNavTol of a face belongs to the box only if its direction enters through
+// that face.
+constexpr double kNavTol = 1e-9;
+constexpr int kNavMaxDepth = 32;
+
+__device__ __forceinline__ bool nav_contains(const NavNode &b, const double p[3], const double v[3]) {
+  for (int k = 0; k < 3; ++k) {
+    const double x = p[k] - b.c[k];
+    const double a = fabs(x);
+    if (a > b.h[k] + kNavTol) return false;
+    if (a >= b.h[k] - kNavTol && x * v[k] >= 0.0) return false;
+  }
+  return true;
+}
+
+// Deepest touchable below t (inclusive) containing the point; t contains it.
+__device__ __forceinline__ int nav_descend(const NavNode *node, int t, const double p[3], const double v[3]) {
+  for (int depth = 0; depth < kNavMaxDepth; ++depth) {
+    const NavNode &m = node[t];
+    int in = -1;
+    for (int d = m.first; d < m.first + m.count; ++d) {
+      if (nav_contains(node[d], p, v)) {
+        in = d;
+        break;
+      }
+    }
+    if (in < 0) break;
+    t = in;
+  }
+  return t;
+}
+
+// GlobalLocator::LocateGlobalPoint: -1 outside the world (touchable 0).
+__device__ __forceinline__ int nav_locate(const NavNode *node, const double p[3], const double v[3]) {
+  if (!nav_contains(node[0], p, v)) return -1;
+  return nav_descend(node, 0, p, v);
+}
+
+// Starting from touchable t, the touchable holding the point: up through the
+// parents until one contains it, then down (the relocation after a boundary
+// crossing).
+__device__ __forceinline__ int nav_relocate(const NavNode *node, int t, const double p[3], const double v[3]) {
+  for (int depth = 0; depth <= kNavMaxDepth && t >= 0; ++depth) {
+    if (nav_contains(node[t], p, v)) return nav_descend(node, t, p, v);
+    t = node[t].parent;
+  }
+  return -1;
+}
+
+// Distance along v from p to the box b from outside (the slab method):
+// kHuge unless the ray passes more than kNavTol through b.
+__device__ __forceinline__ double nav_dist_in(const NavNode &b, const double p[3], const double v[3]) {
+  double tmin = -kHuge, tmax = kHuge;
+  for (int k = 0; k < 3; ++k) {
+    const double x = p[k] - b.c[k];
+    if (v[k] == 0.0) {
+      if (fabs(x) >= b.h[k] - kNavTol) return kHuge;
+      continue;
+    }
+    const double t1 = (-b.h[k] - x) / v[k], t2 = (b.h[k] - x) / v[k];
+    tmin = fmax(tmin, fmin(t1, t2));
+    tmax = fmin(tmax, fmax(t1, t2));
+  }
+  const double t0 = fmax(tmin, 0.0);
+  return tmax > t0 + kNavTol ? t0 : kHuge;
+}
+
+// Distance along v from p inside the box b to its surface.
+__device__ __forceinline__ double nav_dist_out(const NavNode &b, const double p[3], const double v[3]) {
+  double s = kHuge;
+  for (int k = 0; k < 3; ++k) {
+    const double x = p[k] - b.c[k];
+    if (v[k] > 0.0)
+      s = fmin(s, (b.h[k] - x) / v[k]);
+    else if (v[k] < 0.0)
+      s = fmin(s, (-b.h[k] - x) / v[k]);
+  }
+  return s < 0.0 ? 0.0 : s;
+}
+
+// The boundary step from p in touchable t (FindNextBoundaryAndStep,
+// NavigationInterface.hpp:64-98): distance to leaving t or entering one of its
+// daughters, whichever is first (ties leave t, then take the lowest
+// daughter), and the touchable holding p + s v.  Also the isotropic safety.
+__device__ __forceinline__ int nav_step(const NavNode *node, int t, const double p[3], const double v[3],
+                                        double &snext, double &safety) {
+  const NavNode &m = node[t];
+  double s = nav_dist_out(m, p, v);
+  int enter = -1;
+  double sm = kHuge;
+  for (int k = 0; k < 3; ++k) sm = fmin(sm, m.h[k] - fabs(p[k] - m.c[k]));
+  double dmin = kHuge;
+  for (int d = m.first; d < m.first + m.count; ++d) {
+    const NavNode &b = node[d];
+    const double sd = nav_dist_in(b, p, v);
+    if (sd < s) {
+      s = sd;
+      enter = d;
+    }
+    double q = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double e = fabs(p[k] - b.c[k]) - b.h[k];
+      if (e > 0.0) q += e * e;
+    }
+    dmin = fmin(dmin, q);
+  }
+  if (dmin < kHuge) sm = fmin(sm, sqrt(dmin));
+  safety = sm < 0.0 ? 0.0 : sm;
+  snext = s;
+  const double q[3] = {p[0] + s * v[0], p[1] + s * v[1], p[2] + s * v[2]};
+  return nav_relocate(node, enter >= 0 ? enter : t, q, v);
+}
+
 // ConstFieldHelixStepper::DoStep (magneticfield/ConstFieldHelixStepper.hpp:164-239)
 // in mm/MeV/tesla; the direction is renormalised (FieldPropagationHandler.cpp:323-330).
 // The arithmetic is in R; the position update is accumulated in FP64.

@@ -96,6 +96,18 @@ struct DevField {
   int general;             // e-/e+ take the general propagation (a map, or RK45)
 };
 
+// One placed box of a volume tree (a touchable), in global coordinates: boxes
+// are placed by translations only.  Touchables are numbered breadth-first, so
+// the daughters of a touchable are [first, first + count).
+struct alignas(64) NavNode {
+  double c[3];     // centre (mm)
+  double h[3];     // half-lengths (mm)
+  int32_t parent;  // -1 for the world
+  int32_t first;
+  int32_t count;
+  int32_t volume;  // index into the caller's volume array
+};
+
 // Everything a kernel needs besides the pool: lives in one device buffer.
 struct DevParams {
   DevMaterial mat[kMaxSlots];

@@ -139,6 +139,102 @@ class FieldMap:
             pass
 
 
+def hip_runtime():
+    """The HIP runtime libgxt.so runs on (already loaded; found by its soname)."""
+    h = C.CDLL("libamdhip64.so.7")
+    h.hipGetErrorString.restype = C.c_char_p
+    return h
+
+
+class DeviceArray:
+    """A device copy of a numpy array, allocated with the HIP runtime libgxt.so
+    uses, so that it can be passed to the entry points that take device memory
+    (ptr)."""
+
+    def __init__(self, a):
+        self.hip = hip_runtime()
+        self.a = np.ascontiguousarray(a)
+        self.ptr = C.c_void_p()
+        self._ok(self.hip.hipMalloc(C.byref(self.ptr), C.c_size_t(max(self.a.nbytes, 1))))
+        self._ok(self.hip.hipMemcpy(self.ptr, self.a.ctypes.data_as(C.c_void_p), C.c_size_t(self.a.nbytes), 1))
+
+    def _ok(self, rc):
+        if rc != 0:
+            raise GxtError(-2, "HIP: " + self.hip.hipGetErrorString(rc).decode())
+
+    def numpy(self):
+        """Copies the device contents back (after the device is synchronised)."""
+        self._ok(self.hip.hipDeviceSynchronize())
+        out = np.empty_like(self.a)
+        self._ok(self.hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(self.a.nbytes), 2))
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Navigator:
+    """A volume tree of placed boxes resident on the current HIP device.
+
+    volumes: [(half_xyz, material)]; placements: [(mother, daughter,
+    translation_xyz)] (abi.reference_calorimeter_tree builds the reference's
+    calorimeter).  Touchables are numbered breadth-first from the world (0)."""
+
+    def __init__(self, volumes, placements, world=0):
+        vol, pl = abi.box_tree(volumes, placements)
+        self.h = C.c_void_p()
+        _check(lib.gxt_nav_create(vol, len(volumes), pl, len(placements), world, C.byref(self.h)))
+
+    def info(self):
+        n = C.c_int()
+        _check(lib.gxt_nav_info(self.h, C.byref(n), None, None, None))
+        vol = np.zeros(n.value, dtype=np.int32)
+        par = np.zeros(n.value, dtype=np.int32)
+        cen = np.zeros((n.value, 3))
+        _check(lib.gxt_nav_info(self.h, None, abi.ptr(vol, C.c_int32), abi.ptr(par, C.c_int32), abi.ptr(cen)))
+        return dict(volume=vol, parent=par, centre=cen)
+
+    def locate(self, pos, dirs):
+        """Deepest touchable holding each point (-1 outside); pos, dirs: [3, n]."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        dirs = np.ascontiguousarray(dirs, dtype=np.float64)
+        n = pos.shape[1]
+        out = np.zeros(n, dtype=np.int32)
+        _check(lib.gxt_nav_locate(self.h, abi.ptr(pos), abi.ptr(dirs), abi.ptr(out, C.c_int32), n))
+        return out
+
+    def step(self, touchable, pos, dirs):
+        """(snext, next touchable, safety) of each point inside touchable[i]."""
+        t = np.ascontiguousarray(touchable, dtype=np.int32)
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        dirs = np.ascontiguousarray(dirs, dtype=np.float64)
+        n = len(t)
+        s, saf = np.zeros(n), np.zeros(n)
+        nxt = np.zeros(n, dtype=np.int32)
+        _check(lib.gxt_nav_step(self.h, abi.ptr(t, C.c_int32), abi.ptr(pos), abi.ptr(dirs), abi.ptr(s),
+                                abi.ptr(nxt, C.c_int32), abi.ptr(saf), n))
+        return s, nxt, saf
+
+    def close(self):
+        if self.h:
+            _check(lib.gxt_nav_free(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def field_step(field_map, field, charge, mom, step, pos, dirs, stepper=abi.STEPPER_HELIX, precision=abi.FP64,
                origin=(0.0, 0.0, 0.0)):
     """One propagation sub-step per track in a map (field_map) or a uniform field; returns (pos, dir, err)."""

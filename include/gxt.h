@@ -360,6 +360,56 @@ int gxt_field_step(const gxt_field_map *map, const double field[3], const double
                    int precision, const double *charge, const double *momentum, const double *step,
                    double *pos, double *dir, double *err, int n);
 
+/* ---- volume-tree navigation ---------------------------------------------- */
+/* A geometry of box volumes placed inside each other by translations, as the
+ * reference builds its calorimeter with VecGeom (UnplacedBox, LogicalVolume,
+ * PlaceDaughter; proxy/test/BasicCpuTransport/ProxyDetectorConstruction.cpp:
+ * 225-325), navigated on the device.  Replaces, for box geometries, the
+ * navigation calls of geometry/NavigationInterface.hpp:42-98 (FindNextBoundary,
+ * IsSameLocation) and VecGeom's locate.  A volume's daughters must lie inside
+ * it and must not overlap each other (checked when the navigator is built).
+ * Touchables (placed instances) are numbered breadth-first from the world (0).
+ * The transport loop itself uses the analytic slab navigator. */
+typedef struct gxt_box_volume {
+  double half[3];      /* half-lengths (mm)                         */
+  int32_t material;    /* GXT_MAT_* or -1 (vacuum)                  */
+  int32_t pad;
+} gxt_box_volume;
+typedef struct gxt_placement {
+  int32_t mother;      /* index into the volume array               */
+  int32_t daughter;    /* index into the volume array               */
+  double translation[3]; /* daughter centre in the mother frame (mm) */
+} gxt_placement;
+typedef struct gxt_navigator gxt_navigator;
+/* world: the volume placed at the origin; at most 65535 touchables, 32
+ * levels.  GXT_EINVAL for half-lengths below 1e-6 mm, a daughter outside its
+ * mother, daughters that overlap by more than 1e-9 mm, a placement cycle or
+ * out-of-range indices. */
+int gxt_nav_create(const gxt_box_volume *volumes, int n_volumes, const gxt_placement *placements, int n_placements,
+                   int world, gxt_navigator **out);
+int gxt_nav_free(gxt_navigator *nav);
+/* Touchables, and for touchable t: its volume, parent touchable and global
+ * centre (3 doubles); any output may be NULL. */
+int gxt_nav_info(const gxt_navigator *nav, int *n_touchables, int32_t *volume, int32_t *parent, double *centre);
+/* Queries take n points (pos, dir: [3n] SoA, dir unit vectors).  Arrays in
+ * host memory are checked (finite, unit, touchables in range: GXT_EINVAL),
+ * copied and the call returns with the results.  Arrays that are all device
+ * memory of the navigator's device are used in place: the kernel is launched
+ * on the null stream and the call returns without waiting; a touchable out of
+ * range then gives next = -2 and NaN distances.  Mixed arrays: GXT_EINVAL.
+ *
+ * Locate: the deepest touchable containing each point (-1 outside the world).
+ * A point within 1e-9 mm of a face belongs to the box only if its direction
+ * enters through that face. */
+int gxt_nav_locate(const gxt_navigator *nav, const double *pos, const double *dir, int32_t *touchable, int n);
+/* Step, for points inside touchable[i]: the distance along dir to the next
+ * boundary (leaving the touchable or entering a daughter; a daughter the ray
+ * passes through by no more than 1e-9 mm is not entered), the touchable
+ * holding the boundary point with that direction (-1: outside the world), and
+ * the safety (distance to the nearest boundary in any direction). */
+int gxt_nav_step(const gxt_navigator *nav, const int32_t *touchable, const double *pos, const double *dir,
+                 double *snext, int32_t *next, double *safety, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2368,3 +2368,221 @@ int gxref_msc_screening(const double *tau, int n, double *u) {
   for (int i = 0; i < n; ++i) u[i] = msc_u(tau[i]);
   return GXT_OK;
 }
+
+// ------------------------------------------------------- volume-tree navigator
+// VecGeom's navigation over placed boxes (the geometry library the reference
+// links, an un-vendored submodule at source/VecGeom; its call sites are
+// geometry/NavigationInterface.hpp:42-98 and the locate of
+// proxy/test/test_BasicCpuTransport.cpp:572), restated for boxes placed by
+// translations (proxy/test/BasicCpuTransport/ProxyDetectorConstruction.cpp:
+// 225-325 builds the reference calorimeter from exactly these):
+//  - locate: descend from the world into the first daughter holding the point;
+//  - next boundary: the least of the distance out of the current box and the
+//    distances into each daughter (UnplacedBox's slab method), then relocate
+//    the boundary point (up through the mothers, then down);
+//  - safety: the isotropic distance to the current box's surface or the
+//    nearest daughter.
+// The surface tolerance is VecGeom's kTolerance, 1e-9 mm
+// (NavigationInterface.hpp:66); a point within it of a face belongs to the box
+// only if its direction enters through that face.
+struct gxref_nav {
+  struct Touchable {
+    double c[3], h[3];
+    int parent, volume;
+    std::vector<int> daughters;
+  };
+  std::vector<Touchable> t;
+};
+
+namespace gxref {
+namespace nav {
+
+constexpr double kTol = 1e-9;
+
+static bool inside(const gxref_nav::Touchable &b, const double p[3], const double v[3]) {
+  for (int k = 0; k < 3; ++k) {
+    const double x = p[k] - b.c[k];
+    if (std::fabs(x) > b.h[k] + kTol) return false;                  // outside
+    if (std::fabs(x) >= b.h[k] - kTol && !(x * v[k] < 0.0)) return false;  // on the face, not entering
+  }
+  return true;
+}
+
+static int down(const gxref_nav &g, int cur, const double p[3], const double v[3]) {
+  for (int d : g.t[cur].daughters)
+    if (inside(g.t[d], p, v)) return down(g, d, p, v);
+  return cur;
+}
+
+static int relocate(const gxref_nav &g, int cur, const double p[3], const double v[3]) {
+  while (cur >= 0 && !inside(g.t[cur], p, v)) cur = g.t[cur].parent;
+  return cur < 0 ? -1 : down(g, cur, p, v);
+}
+
+// UnplacedBox DistanceToIn: entry into the slab intersection, if the ray
+// runs more than the tolerance inside the box.
+static double to_in(const gxref_nav::Touchable &b, const double p[3], const double v[3]) {
+  double enter = -kHuge, leave = kHuge;
+  for (int k = 0; k < 3; ++k) {
+    const double x = p[k] - b.c[k];
+    if (v[k] == 0.0) {
+      if (std::fabs(x) >= b.h[k] - kTol) return kHuge;  // parallel, outside the slab or on its face
+      continue;
+    }
+    const double a = (-b.h[k] - x) / v[k], c = (b.h[k] - x) / v[k];
+    enter = std::max(enter, std::min(a, c));
+    leave = std::min(leave, std::max(a, c));
+  }
+  enter = std::max(enter, 0.0);
+  return leave > enter + kTol ? enter : kHuge;
+}
+
+// UnplacedBox DistanceToOut from inside.
+static double to_out(const gxref_nav::Touchable &b, const double p[3], const double v[3]) {
+  double s = kHuge;
+  for (int k = 0; k < 3; ++k) {
+    if (v[k] == 0.0) continue;
+    const double x = p[k] - b.c[k];
+    s = std::min(s, ((v[k] > 0.0 ? b.h[k] : -b.h[k]) - x) / v[k]);
+  }
+  return std::max(s, 0.0);
+}
+
+static double safety(const gxref_nav &g, int cur, const double p[3]) {
+  const gxref_nav::Touchable &m = g.t[cur];
+  double s = kHuge;
+  for (int k = 0; k < 3; ++k) s = std::min(s, m.h[k] - std::fabs(p[k] - m.c[k]));
+  double q2 = kHuge;
+  for (int d : m.daughters) {
+    double q = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double e = std::fabs(p[k] - g.t[d].c[k]) - g.t[d].h[k];
+      if (e > 0.0) q += e * e;
+    }
+    q2 = std::min(q2, q);
+  }
+  if (q2 < kHuge) s = std::min(s, std::sqrt(q2));
+  return std::max(s, 0.0);
+}
+
+}  // namespace nav
+}  // namespace gxref
+
+extern "C" {
+
+int gxref_nav_create(const gxt_box_volume *vol, int nvol, const gxt_placement *pl, int npl, int world,
+                     gxref_nav **out) {
+  using gxref::fail;
+  if (!vol || nvol <= 0 || npl < 0 || (npl > 0 && !pl) || world < 0 || world >= nvol || !out)
+    return fail(GXT_EINVAL, "bad navigator arguments");
+  for (int i = 0; i < nvol; ++i)
+    for (int k = 0; k < 3; ++k)
+      if (!(vol[i].half[k] >= 1e-6) || !std::isfinite(vol[i].half[k]) || vol[i].material < -1 ||
+          vol[i].material > GXT_MAT_PBWO4)
+        return fail(GXT_EINVAL, "bad volume");
+  for (int i = 0; i < npl; ++i) {
+    const gxt_placement &p = pl[i];
+    if (p.mother < 0 || p.mother >= nvol || p.daughter < 0 || p.daughter >= nvol) return fail(GXT_EINVAL, "bad index");
+    for (int k = 0; k < 3; ++k)
+      if (!std::isfinite(p.translation[k]) ||
+          std::fabs(p.translation[k]) + vol[p.daughter].half[k] > vol[p.mother].half[k] + gxref::nav::kTol)
+        return fail(GXT_EINVAL, "daughter outside its mother");
+  }
+  // overlaps: every pair of daughters of one mother
+  for (int i = 0; i < npl; ++i)
+    for (int j = i + 1; j < npl; ++j) {
+      if (pl[i].mother != pl[j].mother) continue;
+      int k = 0;
+      while (k < 3 && std::fabs(pl[i].translation[k] - pl[j].translation[k]) <
+                          vol[pl[i].daughter].half[k] + vol[pl[j].daughter].half[k] - gxref::nav::kTol)
+        ++k;
+      if (k == 3) return fail(GXT_EINVAL, "overlapping daughters");
+    }
+  gxref_nav *g = new gxref_nav();
+  std::vector<int> level;
+  gxref_nav::Touchable w;
+  for (int k = 0; k < 3; ++k) {
+    w.c[k] = 0.0;
+    w.h[k] = vol[world].half[k];
+  }
+  w.parent = -1;
+  w.volume = world;
+  g->t.push_back(w);
+  level.push_back(0);
+  // breadth-first: touchables of one level in order of their mothers, the
+  // daughters of one touchable in placement order
+  for (size_t cur = 0; cur < g->t.size(); ++cur) {
+    for (int i = 0; i < npl; ++i) {
+      if (pl[i].mother != g->t[cur].volume) continue;
+      if (level[cur] >= 31 || g->t.size() >= 65535) {
+        delete g;
+        return fail(GXT_EINVAL, "volume tree too deep or too large");
+      }
+      gxref_nav::Touchable d;
+      for (int k = 0; k < 3; ++k) {
+        d.c[k] = g->t[cur].c[k] + pl[i].translation[k];
+        d.h[k] = vol[pl[i].daughter].half[k];
+      }
+      d.parent = (int)cur;
+      d.volume = pl[i].daughter;
+      g->t[cur].daughters.push_back((int)g->t.size());
+      g->t.push_back(d);
+      level.push_back(level[cur] + 1);
+    }
+  }
+  *out = g;
+  return GXT_OK;
+}
+
+int gxref_nav_free(gxref_nav *g) {
+  delete g;
+  return GXT_OK;
+}
+
+int gxref_nav_info(const gxref_nav *g, int *n, int32_t *volume, int32_t *parent, double *centre) {
+  if (!g) return gxref::fail(GXT_EINVAL, "null navigator");
+  if (n) *n = (int)g->t.size();
+  for (size_t i = 0; i < g->t.size(); ++i) {
+    if (volume) volume[i] = g->t[i].volume;
+    if (parent) parent[i] = g->t[i].parent;
+    if (centre)
+      for (int k = 0; k < 3; ++k) centre[3 * i + k] = g->t[i].c[k];
+  }
+  return GXT_OK;
+}
+
+int gxref_nav_locate(const gxref_nav *g, const double *pos, const double *dir, int32_t *out, int n) {
+  if (!g || !pos || !dir || !out || n <= 0) return gxref::fail(GXT_EINVAL, "bad locate arguments");
+  for (int i = 0; i < n; ++i) {
+    const double p[3] = {pos[i], pos[n + i], pos[2 * n + i]}, v[3] = {dir[i], dir[n + i], dir[2 * n + i]};
+    out[i] = gxref::nav::inside(g->t[0], p, v) ? gxref::nav::down(*g, 0, p, v) : -1;
+  }
+  return GXT_OK;
+}
+
+int gxref_nav_step(const gxref_nav *g, const int32_t *cur, const double *pos, const double *dir, double *snext,
+                   int32_t *next, double *safety, int n) {
+  if (!g || !cur || !pos || !dir || !snext || !next || !safety || n <= 0)
+    return gxref::fail(GXT_EINVAL, "bad step arguments");
+  for (int i = 0; i < n; ++i) {
+    if (cur[i] < 0 || cur[i] >= (int)g->t.size()) return gxref::fail(GXT_EINVAL, "touchable out of range");
+    const double p[3] = {pos[i], pos[n + i], pos[2 * n + i]}, v[3] = {dir[i], dir[n + i], dir[2 * n + i]};
+    const gxref_nav::Touchable &m = g->t[cur[i]];
+    double s = gxref::nav::to_out(m, p, v);
+    int target = cur[i];
+    for (int d : m.daughters) {
+      const double sd = gxref::nav::to_in(g->t[d], p, v);
+      if (sd < s) {
+        s = sd;
+        target = d;
+      }
+    }
+    const double q[3] = {p[0] + s * v[0], p[1] + s * v[1], p[2] + s * v[2]};
+    snext[i] = s;
+    next[i] = gxref::nav::relocate(*g, target, q, v);
+    safety[i] = gxref::nav::safety(*g, cur[i], p);
+  }
+  return GXT_OK;
+}
+
+}  // extern "C"

@@ -79,6 +79,15 @@ int gxref_field_step(const gxref_field_map *map, const double field[3], const do
                      double *err, int n);
 int gxref_rotate_to_lab(double *u, double *v, double *w, const double *u1, const double *u2,
                         const double *u3, int n);
+/* Volume-tree navigation, same contract as gxt_nav_* (no device). */
+typedef struct gxref_nav gxref_nav;
+int gxref_nav_create(const gxt_box_volume *volumes, int n_volumes, const gxt_placement *placements, int n_placements,
+                     int world, gxref_nav **out);
+int gxref_nav_free(gxref_nav *nav);
+int gxref_nav_info(const gxref_nav *nav, int *n_touchables, int32_t *volume, int32_t *parent, double *centre);
+int gxref_nav_locate(const gxref_nav *nav, const double *pos, const double *dir, int32_t *touchable, int n);
+int gxref_nav_step(const gxref_nav *nav, const int32_t *touchable, const double *pos, const double *dir,
+                   double *snext, int32_t *next, double *safety, int n);
 /* Sandia photo-absorption macroscopic cross-section (1/mm). */
 int gxref_sandia_xs(gxref_ctx *ctx, int material_slot, const double *ekin, double *out, int n);
 

@@ -249,6 +249,48 @@ class FieldMap:
             self.h = C.c_void_p()
 
 
+class Navigator:
+    """Oracle copy of a volume tree of placed boxes (host memory)."""
+
+    def __init__(self, volumes, placements, world=0):
+        vol, pl = abi.box_tree(volumes, placements)
+        self.h = C.c_void_p()
+        check(lib().gxref_nav_create(vol, len(volumes), pl, len(placements), world, C.byref(self.h)))
+
+    def info(self):
+        n = C.c_int()
+        check(lib().gxref_nav_info(self.h, C.byref(n), None, None, None))
+        vol = np.zeros(n.value, dtype=np.int32)
+        par = np.zeros(n.value, dtype=np.int32)
+        cen = np.zeros((n.value, 3))
+        check(lib().gxref_nav_info(self.h, None, abi.ptr(vol, C.c_int32), abi.ptr(par, C.c_int32), abi.ptr(cen)))
+        return dict(volume=vol, parent=par, centre=cen)
+
+    def locate(self, pos, dirs):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        dirs = np.ascontiguousarray(dirs, dtype=np.float64)
+        n = pos.shape[1]
+        out = np.zeros(n, dtype=np.int32)
+        check(lib().gxref_nav_locate(self.h, abi.ptr(pos), abi.ptr(dirs), abi.ptr(out, C.c_int32), n))
+        return out
+
+    def step(self, touchable, pos, dirs):
+        t = np.ascontiguousarray(touchable, dtype=np.int32)
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        dirs = np.ascontiguousarray(dirs, dtype=np.float64)
+        n = len(t)
+        s, saf = np.zeros(n), np.zeros(n)
+        nxt = np.zeros(n, dtype=np.int32)
+        check(lib().gxref_nav_step(self.h, abi.ptr(t, C.c_int32), abi.ptr(pos), abi.ptr(dirs), abi.ptr(s),
+                                   abi.ptr(nxt, C.c_int32), abi.ptr(saf), n))
+        return s, nxt, saf
+
+    def __del__(self):
+        if self.h and _lib is not None:
+            _lib.gxref_nav_free(self.h)
+            self.h = C.c_void_p()
+
+
 def field_step(field_map, field, charge, mom, step, pos, dirs, stepper=abi.STEPPER_HELIX, origin=(0.0, 0.0, 0.0)):
     charge = np.ascontiguousarray(charge, dtype=np.float64)
     mom = np.ascontiguousarray(mom, dtype=np.float64)

@@ -40,6 +40,8 @@ STRUCTS = {
     "gxt_job_result": abi.JobResult,
     "gxt_sample_io": abi.SampleIO,
     "gxt_kernel_stats": abi.KernelStats,
+    "gxt_box_volume": abi.BoxVolume,
+    "gxt_placement": abi.Placement,
 }
 
 

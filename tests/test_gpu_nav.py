@@ -1,0 +1,170 @@
+"""GPU parity of the volume-tree navigator: libgxt.so against the CPU oracle,
+through the C ABI.
+
+Locate, boundary distance, next touchable and safety are bit-exact: both
+sides do the same correctly rounded FP64 operations (differences, quotients,
+min/max, one square root).  Cases: the reference's calorimeter (152
+touchables) over the whole world, points on every layer face with directions
+that enter, leave and run along it, random nested trees, and rays walked
+boundary to boundary through the calorimeter.
+"""
+import numpy as np
+import pytest
+
+from geantexascalepilot_amd import abi
+import oracle_harness as O
+from test_gpu_parity import _flush
+from test_nav import _unit, random_tree
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(gpu_lib, vols, pls, world=0):
+    return gpu_lib.Navigator(vols, pls, world), O.Navigator(vols, pls, world)
+
+
+def _same(a, b):
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+@pytest.fixture(scope="module")
+def calo(gpu_lib):
+    g, c = _pair(gpu_lib, *abi.reference_calorimeter_tree())
+    yield g, c
+    g.close()
+
+
+def test_info_matches(calo):
+    _flush("nav info: reference calorimeter")
+    g, c = calo
+    gi, ci = g.info(), c.info()
+    for k in gi:
+        np.testing.assert_array_equal(gi[k], ci[k])
+
+
+def test_calorimeter_bit_exact(calo):
+    _flush("nav locate/step: reference calorimeter, 200k points")
+    g, c = calo
+    rs = np.random.default_rng(21)
+    n = 200000
+    pos = rs.uniform(-1.0, 1.0, (3, n)) * np.array([[600.0], [600.0], [600.0]])
+    d = _unit(rs, n)
+    t = g.locate(pos, d)
+    np.testing.assert_array_equal(t, c.locate(pos, d))
+    ins = t >= 0
+    _same(g.step(t[ins], pos[:, ins], d[:, ins]), c.step(t[ins], pos[:, ins], d[:, ins]))
+
+
+def test_surface_points_bit_exact(calo):
+    _flush("nav surface points: every layer face")
+    g, c = calo
+    rs = np.random.default_rng(22)
+    faces = -500.0 + 10.0 * np.arange(101)
+    m = 200
+    x = np.repeat(faces, m)
+    n = len(x)
+    pos = np.stack([x, rs.uniform(-500, 500, n), rs.uniform(-500, 500, n)])
+    d = _unit(rs, n)
+    # a third run along the face (no x component), some along an axis
+    d[0, ::3] = 0.0
+    d[:, ::3] /= np.linalg.norm(d[:, ::3], axis=0)
+    d[:, 1::7] = np.array([[1.0], [0.0], [0.0]])
+    d[:, 2::7] = np.array([[-1.0], [0.0], [0.0]])
+    t = g.locate(pos, d)
+    np.testing.assert_array_equal(t, c.locate(pos, d))
+    ins = t >= 0
+    _same(g.step(t[ins], pos[:, ins], d[:, ins]), c.step(t[ins], pos[:, ins], d[:, ins]))
+
+
+@pytest.mark.parametrize("seed", [3, 4, 5, 6, 7, 8])
+def test_random_trees_bit_exact(gpu_lib, seed):
+    _flush("nav random tree seed %d" % seed)
+    rs = np.random.default_rng(seed)
+    vols, pls = random_tree(rs)
+    g, c = _pair(gpu_lib, vols, pls)
+    try:
+        gi, ci = g.info(), c.info()
+        for k in gi:
+            np.testing.assert_array_equal(gi[k], ci[k])
+        half = np.array(vols[0][0])[:, None]
+        n = 50000
+        pos = rs.uniform(-1.1, 1.1, (3, n)) * half
+        d = _unit(rs, n)
+        t = g.locate(pos, d)
+        np.testing.assert_array_equal(t, c.locate(pos, d))
+        ins = t >= 0
+        _same(g.step(t[ins], pos[:, ins], d[:, ins]), c.step(t[ins], pos[:, ins], d[:, ins]))
+    finally:
+        g.close()
+
+
+def test_walk_bit_exact(calo):
+    """4096 rays from the world, stepped boundary to boundary until they leave it."""
+    _flush("nav walk: 4096 rays through the reference calorimeter")
+    g, c = calo
+    rs = np.random.default_rng(23)
+    n = 4096
+    pos = np.stack([np.full(n, -600.0), rs.uniform(-300, 300, n), rs.uniform(-300, 300, n)])
+    d = _unit(rs, n)
+    d[0] = np.abs(d[0]) + 0.2
+    d /= np.linalg.norm(d, axis=0)
+    t = g.locate(pos, d)
+    np.testing.assert_array_equal(t, c.locate(pos, d))
+    crossings = 0
+    for _ in range(400):
+        live = np.nonzero(t >= 0)[0]
+        if len(live) == 0:
+            break
+        gs = g.step(t[live], pos[:, live], d[:, live])
+        _same(gs, c.step(t[live], pos[:, live], d[:, live]))
+        s, nxt, _ = gs
+        pos[:, live] += s * d[:, live]
+        t[live] = nxt
+        crossings += len(live)
+    assert (t == -1).all()
+    assert crossings > 100 * n // 4
+
+
+def test_device_arrays_match_host_arrays(calo, gpu_lib):
+    """Arrays already in HBM are used in place (stream-ordered on the null
+    stream); a touchable out of range gives next = -2 and NaN distances."""
+    import ctypes as C
+    _flush("nav device-resident arrays")
+    g, c = calo
+    rs = np.random.default_rng(24)
+    n = 100000
+    pos = rs.uniform(-1.0, 1.0, (3, n)) * 600.0
+    d = _unit(rs, n)
+    t = c.locate(pos, d)
+    t[::1000] = 152
+    t[1::1000] = -7
+    D = gpu_lib.DeviceArray
+    dp, dd, dl = D(pos), D(d), D(np.zeros(n, dtype=np.int32))
+    lib = gpu_lib.lib
+    gpu_lib._check(lib.gxt_nav_locate(g.h, dp.ptr, dd.ptr, dl.ptr, n))
+    np.testing.assert_array_equal(dl.numpy(), c.locate(pos, d))
+    dt, ds, dn, dsaf = D(t), D(np.zeros(n)), D(np.zeros(n, dtype=np.int32)), D(np.zeros(n))
+    gpu_lib._check(lib.gxt_nav_step(g.h, dt.ptr, dp.ptr, dd.ptr, ds.ptr, dn.ptr, dsaf.ptr, n))
+    bad = (t < 0) | (t >= 152)
+    ok = ~bad
+    want = c.step(t[ok], pos[:, ok], d[:, ok])
+    got = (ds.numpy(), dn.numpy(), dsaf.numpy())
+    for x, y in zip(got, want):
+        np.testing.assert_array_equal(x[ok], y)
+    assert (got[1][bad] == -2).all() and np.isnan(got[0][bad]).all() and np.isnan(got[2][bad]).all()
+    # host and device arrays mixed
+    hs = np.zeros(n)
+    assert lib.gxt_nav_step(g.h, dt.ptr, dp.ptr, dd.ptr, abi.ptr(hs), dn.ptr, dsaf.ptr, n) == -1
+    for a in (dp, dd, dl, dt, ds, dn, dsaf):
+        a.free()
+
+
+def test_bad_touchable_rejected(calo, gpu_lib):
+    g, _ = calo
+    pos = np.zeros((3, 1))
+    d = np.array([[1.0], [0.0], [0.0]])
+    with pytest.raises(gpu_lib.GxtError, match="touchable"):
+        g.step(np.array([152]), pos, d)
+    with pytest.raises(gpu_lib.GxtError, match="unit direction"):
+        g.locate(pos, np.array([[1.0], [1.0], [0.0]]))
