@@ -930,6 +930,7 @@ __device__ __forceinline__ double safety(const DevGeometry &g, int v, const doub
 // that face.
 constexpr double kNavTol = 1e-9;
 constexpr int kNavMaxDepth = 32;
+constexpr int kNavGridMin = 8;  // daughters from which a volume has a daughter index
 
 __device__ __forceinline__ bool nav_contains(const NavNode &b, const double p[3], const double v[3]) {
   for (int k = 0; k < 3; ++k) {
@@ -942,14 +943,29 @@ __device__ __forceinline__ bool nav_contains(const NavNode &b, const double p[3]
 }
 
 // Deepest touchable below t (inclusive) containing the point; t contains it.
-__device__ __forceinline__ int nav_descend(const NavNode *node, int t, const double p[3], const double v[3]) {
+// The first daughter (lowest index) holding the point is taken: with a
+// daughter index only the candidates of the point's bin are tested, which
+// include every daughter that can hold it, in the same order.
+__device__ __forceinline__ int nav_descend(const DevNav &g, int t, const double p[3], const double v[3]) {
   for (int depth = 0; depth < kNavMaxDepth; ++depth) {
-    const NavNode &m = node[t];
+    const NavNode &m = g.node[t];
     int in = -1;
-    for (int d = m.first; d < m.first + m.count; ++d) {
-      if (nav_contains(node[d], p, v)) {
-        in = d;
-        break;
+    if (m.count >= kNavGridMin && g.grid[m.volume].nb > 0) {
+      const NavGrid &gr = g.grid[m.volume];
+      const int b = nav_bin(p[gr.ax] - m.c[gr.ax], m.h[gr.ax], gr.inv, gr.nb);
+      for (int k = g.idx[gr.off + b]; k < g.idx[gr.off + b + 1]; ++k) {
+        const int d = m.first + g.idx[k];
+        if (nav_contains(g.node[d], p, v)) {
+          in = d;
+          break;
+        }
+      }
+    } else {
+      for (int d = m.first; d < m.first + m.count; ++d) {
+        if (nav_contains(g.node[d], p, v)) {
+          in = d;
+          break;
+        }
       }
     }
     if (in < 0) break;
@@ -959,18 +975,18 @@ __device__ __forceinline__ int nav_descend(const NavNode *node, int t, const dou
 }
 
 // GlobalLocator::LocateGlobalPoint: -1 outside the world (touchable 0).
-__device__ __forceinline__ int nav_locate(const NavNode *node, const double p[3], const double v[3]) {
-  if (!nav_contains(node[0], p, v)) return -1;
-  return nav_descend(node, 0, p, v);
+__device__ __forceinline__ int nav_locate(const DevNav &g, const double p[3], const double v[3]) {
+  if (!nav_contains(g.node[0], p, v)) return -1;
+  return nav_descend(g, 0, p, v);
 }
 
 // Starting from touchable t, the touchable holding the point: up through the
 // parents until one contains it, then down (the relocation after a boundary
 // crossing).
-__device__ __forceinline__ int nav_relocate(const NavNode *node, int t, const double p[3], const double v[3]) {
+__device__ __forceinline__ int nav_relocate(const DevNav &g, int t, const double p[3], const double v[3]) {
   for (int depth = 0; depth <= kNavMaxDepth && t >= 0; ++depth) {
-    if (nav_contains(node[t], p, v)) return nav_descend(node, t, p, v);
-    t = node[t].parent;
+    if (nav_contains(g.node[t], p, v)) return nav_descend(g, t, p, v);
+    t = g.node[t].parent;
   }
   return -1;
 }
@@ -1010,8 +1026,9 @@ __device__ __forceinline__ double nav_dist_out(const NavNode &b, const double p[
 // NavigationInterface.hpp:64-98): distance to leaving t or entering one of its
 // daughters, whichever is first (ties leave t, then take the lowest
 // daughter), and the touchable holding p + s v.  Also the isotropic safety.
-__device__ __forceinline__ int nav_step(const NavNode *node, int t, const double p[3], const double v[3],
+__device__ __forceinline__ int nav_step(const DevNav &g, int t, const double p[3], const double v[3],
                                         double &snext, double &safety) {
+  const NavNode *node = g.node;
   const NavNode &m = node[t];
   double s = nav_dist_out(m, p, v);
   int enter = -1;
@@ -1036,7 +1053,7 @@ __device__ __forceinline__ int nav_step(const NavNode *node, int t, const double
   safety = sm < 0.0 ? 0.0 : sm;
   snext = s;
   const double q[3] = {p[0] + s * v[0], p[1] + s * v[1], p[2] + s * v[2]};
-  return nav_relocate(node, enter >= 0 ? enter : t, q, v);
+  return nav_relocate(g, enter >= 0 ? enter : t, q, v);
 }
 
 // ConstFieldHelixStepper::DoStep (magneticfield/ConstFieldHelixStepper.hpp:164-239)

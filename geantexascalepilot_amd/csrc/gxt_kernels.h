@@ -90,11 +90,11 @@ hipError_t launch_field_value(const DevField &F, const double *pos, double *b, i
 hipError_t launch_field_step(const DevField &F, int precision, const double *charge, const double *mom,
                              const double *step, double *pos, double *dir, double *err, int n, hipStream_t st);
 
-// Volume-tree queries over n points (pos, dir: [3n] SoA); node holds ntouch
-// touchables (the step kernel checks touchable[i] against it).
-hipError_t launch_nav_locate(const NavNode *node, const double *pos, const double *dir, int32_t *out, int n,
+// Volume-tree queries over n points (pos, dir: [3n] SoA) in the tree g (the
+// step kernel checks touchable[i] against g.ntouch).
+hipError_t launch_nav_locate(const DevNav &g, const double *pos, const double *dir, int32_t *out, int n,
                              hipStream_t st);
-hipError_t launch_nav_step(const NavNode *node, int ntouch, const int32_t *touchable, const double *pos,
-                           const double *dir, double *snext, int32_t *next, double *safety, int n, hipStream_t st);
+hipError_t launch_nav_step(const DevNav &g, const int32_t *touchable, const double *pos, const double *dir,
+                           double *snext, int32_t *next, double *safety, int n, hipStream_t st);
 
 }  // namespace gxt

@@ -992,21 +992,21 @@ __global__ void k_field_value(DevField F, const double *pos, double *b, int n) {
   for (int k = 0; k < 3; ++k) b[k * n + i] = B[k];
 }
 
-__global__ void k_nav_locate(const NavNode *node, const double *pos, const double *dir, int32_t *out, int n) {
+__global__ void k_nav_locate(DevNav g, const double *pos, const double *dir, int32_t *out, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
   const double v[3] = {dir[i], dir[n + i], dir[2 * n + i]};
-  out[i] = nav_locate(node, p, v);
+  out[i] = nav_locate(g, p, v);
 }
 
-// A touchable outside [0, ntouch) gives next = -2 and NaN distances.
-__global__ void k_nav_step(const NavNode *node, int ntouch, const int32_t *touchable, const double *pos,
-                           const double *dir, double *snext, int32_t *next, double *safety, int n) {
+// A touchable outside [0, g.ntouch) gives next = -2 and NaN distances.
+__global__ void k_nav_step(DevNav g, const int32_t *touchable, const double *pos, const double *dir, double *snext,
+                           int32_t *next, double *safety, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int t = touchable[i];
-  if (t < 0 || t >= ntouch) {
+  if (t < 0 || t >= g.ntouch) {
     next[i] = -2;
     snext[i] = __builtin_nan("");
     safety[i] = __builtin_nan("");
@@ -1015,7 +1015,7 @@ __global__ void k_nav_step(const NavNode *node, int ntouch, const int32_t *touch
   const double p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
   const double v[3] = {dir[i], dir[n + i], dir[2 * n + i]};
   double s, saf;
-  next[i] = nav_step(node, t, p, v, s, saf);
+  next[i] = nav_step(g, t, p, v, s, saf);
   snext[i] = s;
   safety[i] = saf;
 }
@@ -1157,16 +1157,16 @@ hipError_t launch_field_value(const DevField &F, const double *pos, double *b, i
   return hipGetLastError();
 }
 
-hipError_t launch_nav_locate(const NavNode *node, const double *pos, const double *dir, int32_t *out, int n,
+hipError_t launch_nav_locate(const DevNav &g, const double *pos, const double *dir, int32_t *out, int n,
                              hipStream_t st) {
-  hipLaunchKernelGGL(k_nav_locate, dim3(grid_for(n)), dim3(kBlock), 0, st, node, pos, dir, out, n);
+  hipLaunchKernelGGL(k_nav_locate, dim3(grid_for(n)), dim3(kBlock), 0, st, g, pos, dir, out, n);
   return hipGetLastError();
 }
 
-hipError_t launch_nav_step(const NavNode *node, int ntouch, const int32_t *touchable, const double *pos,
-                           const double *dir, double *snext, int32_t *next, double *safety, int n, hipStream_t st) {
-  hipLaunchKernelGGL(k_nav_step, dim3(grid_for(n)), dim3(kBlock), 0, st, node, ntouch, touchable, pos, dir, snext,
-                     next, safety, n);
+hipError_t launch_nav_step(const DevNav &g, const int32_t *touchable, const double *pos, const double *dir,
+                           double *snext, int32_t *next, double *safety, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_nav_step, dim3(grid_for(n)), dim3(kBlock), 0, st, g, touchable, pos, dir, snext, next, safety,
+                     n);
   return hipGetLastError();
 }
 

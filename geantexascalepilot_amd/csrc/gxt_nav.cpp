@@ -17,7 +17,21 @@
 struct gxt_navigator {
   int device = -1;
   std::vector<gxt::NavNode> node;  // touchables, breadth-first from the world
-  gxt::NavNode *dev = nullptr;
+  std::vector<gxt::NavGrid> grid;  // daughter index per volume
+  std::vector<int32_t> idx;
+  gxt::NavNode *dev = nullptr;     // one allocation: nodes, grids, index
+  gxt::DevNav view() const {
+    gxt::DevNav g;
+    g.node = dev;
+    g.grid = reinterpret_cast<const gxt::NavGrid *>(dev + node.size());
+    g.idx = reinterpret_cast<const int32_t *>(g.grid + grid.size());
+    g.ntouch = (int)node.size();
+    g.pad = 0;
+    return g;
+  }
+  size_t bytes() const {
+    return node.size() * sizeof(gxt::NavNode) + grid.size() * sizeof(gxt::NavGrid) + idx.size() * sizeof(int32_t);
+  }
 };
 
 using namespace gxt;
@@ -74,7 +88,7 @@ int check_overlaps(const gxt_box_volume *vol, const std::vector<const gxt_placem
 }
 
 int build_tree(const gxt_box_volume *vol, int nvol, const gxt_placement *pl, int npl, int world,
-               std::vector<NavNode> &node) {
+               std::vector<NavNode> &node, std::vector<std::vector<const gxt_placement *>> &daughters) {
   for (int i = 0; i < nvol; ++i) {
     const gxt_box_volume &v = vol[i];
     if (!finite3(v.half) || !(v.half[0] >= kMinHalf && v.half[1] >= kMinHalf && v.half[2] >= kMinHalf))
@@ -82,7 +96,7 @@ int build_tree(const gxt_box_volume *vol, int nvol, const gxt_placement *pl, int
     if (v.material < -1 || v.material > GXT_MAT_PBWO4)
       return set_error(GXT_EINVAL, "volume " + std::to_string(i) + ": material must be a GXT_MAT_* or -1");
   }
-  std::vector<std::vector<const gxt_placement *>> daughters(nvol);
+  daughters.assign(nvol, {});
   for (int i = 0; i < npl; ++i) {
     const gxt_placement &p = pl[i];
     if (p.mother < 0 || p.mother >= nvol || p.daughter < 0 || p.daughter >= nvol || !finite3(p.translation))
@@ -129,6 +143,75 @@ int build_tree(const gxt_box_volume *vol, int nvol, const gxt_placement *pl, int
     node[t].count = (int)daughters[v].size();
   }
   return GXT_OK;
+}
+
+constexpr int kGridMin = 8;       // as the device's kNavGridMin
+constexpr int kGridMaxBins = 4096;
+
+// Axis along which the centres of a volume's daughters spread most (-1: they
+// do not spread).
+int spread_axis(const std::vector<const gxt_placement *> &pl) {
+  int ax = -1;
+  double best = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    double lo = pl[0]->translation[k], hi = lo;
+    for (const gxt_placement *q : pl) {
+      lo = std::min(lo, q->translation[k]);
+      hi = std::max(hi, q->translation[k]);
+    }
+    if (hi - lo > best) {
+      best = hi - lo;
+      ax = k;
+    }
+  }
+  return ax;
+}
+
+// Daughter index (the device's nav_descend tests only the candidates of a
+// point's bin) for each volume with at least kGridMin daughters.  A
+// daughter's extent along the axis, widened by a margin far above the
+// rounding of the device's coordinate differences, is placed in bins with the
+// device's own nav_bin; a volume whose index would hold more than 16 entries
+// per daughter keeps the plain scan.
+void build_index(const gxt_box_volume *vol, int nvol, const std::vector<std::vector<const gxt_placement *>> &dau,
+                 double world_extent, std::vector<NavGrid> &grid, std::vector<int32_t> &idx) {
+  grid.assign(nvol, NavGrid{0.0, 0, 0, 0, 0});
+  idx.clear();
+  const double margin = 1e-6 + 1e-12 * world_extent;
+  for (int v = 0; v < nvol; ++v) {
+    const auto &pl = dau[v];
+    const int n = (int)pl.size();
+    if (n < kGridMin) continue;
+    const int ax = spread_axis(pl);
+    if (ax < 0) continue;
+    // bins about as wide as the mean daughter extent along the axis
+    const double h = vol[v].half[ax];
+    double mean = 0.0;
+    for (const gxt_placement *q : pl) mean += 2.0 * vol[q->daughter].half[ax] / n;
+    const double want = std::ceil(2.0 * h / mean);
+    const int nb = (int)std::max(1.0, std::min(want, (double)std::min(kGridMaxBins, 2 * n)));
+    const double inv = (double)nb / (2.0 * h);
+    std::vector<int> b0(n), b1(n);
+    size_t total = 0;
+    for (int j = 0; j < n; ++j) {
+      const double t = pl[j]->translation[ax], hd = vol[pl[j]->daughter].half[ax];
+      b0[j] = nav_bin(t - hd - margin, h, inv, nb);
+      b1[j] = nav_bin(t + hd + margin, h, inv, nb);
+      total += (size_t)(b1[j] - b0[j] + 1);
+    }
+    if (total > 16 * (size_t)n) continue;
+    std::vector<std::vector<int32_t>> bins(nb);
+    for (int j = 0; j < n; ++j)
+      for (int b = b0[j]; b <= b1[j]; ++b) bins[b].push_back(j);
+    const size_t base = idx.size();
+    idx.resize(base + nb + 1);
+    for (int b = 0; b < nb; ++b) {
+      idx[base + b] = (int32_t)idx.size();
+      idx.insert(idx.end(), bins[b].begin(), bins[b].end());
+    }
+    idx[base + nb] = (int32_t)idx.size();
+    grid[v] = NavGrid{inv, ax, nb, (int32_t)base, 0};
+  }
 }
 
 template <class T>
@@ -189,12 +272,22 @@ int gxt_nav_create(const gxt_box_volume *volumes, int n_volumes, const gxt_place
       world >= n_volumes || !out)
     return set_error(GXT_EINVAL, "bad navigator arguments");
   gxt_navigator *nav = new gxt_navigator();
-  int rc = build_tree(volumes, n_volumes, placements, n_placements, world, nav->node);
+  std::vector<std::vector<const gxt_placement *>> daughters;
+  int rc = build_tree(volumes, n_volumes, placements, n_placements, world, nav->node, daughters);
   if (rc == GXT_OK) {
+    const double *wh = volumes[world].half;
+    build_index(volumes, n_volumes, daughters, std::max(wh[0], std::max(wh[1], wh[2])), nav->grid, nav->idx);
+    // nodes, grids and index in one allocation (the grids start 64 B-aligned after the nodes)
+    std::vector<char> blob(nav->bytes());
+    char *q = blob.data();
+    std::memcpy(q, nav->node.data(), nav->node.size() * sizeof(NavNode));
+    q += nav->node.size() * sizeof(NavNode);
+    std::memcpy(q, nav->grid.data(), nav->grid.size() * sizeof(NavGrid));
+    q += nav->grid.size() * sizeof(NavGrid);
+    if (!nav->idx.empty()) std::memcpy(q, nav->idx.data(), nav->idx.size() * sizeof(int32_t));
     hipError_t e = hipGetDevice(&nav->device);
-    if (e == hipSuccess) e = hipMalloc(&nav->dev, nav->node.size() * sizeof(NavNode));
-    if (e == hipSuccess)
-      e = hipMemcpy(nav->dev, nav->node.data(), nav->node.size() * sizeof(NavNode), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&nav->dev, blob.size());
+    if (e == hipSuccess) e = hipMemcpy(nav->dev, blob.data(), blob.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) rc = hip_fail(e, "navigator upload");
   }
   if (rc) {
@@ -234,7 +327,7 @@ int gxt_nav_locate(const gxt_navigator *nav, const double *pos, const double *di
   const int res = residence(arrays, 3, nav->device);
   if (res < 0) return set_error(GXT_EINVAL, "locate arrays must be all host or all device memory");
   if (res == 1) {
-    const hipError_t e = launch_nav_locate(nav->dev, pos, dir, touchable, n, 0);
+    const hipError_t e = launch_nav_locate(nav->view(), pos, dir, touchable, n, 0);
     return e == hipSuccess ? GXT_OK : hip_fail(e, "gxt_nav_locate");
   }
   if (const int rc = check_points(pos, dir, n)) return rc;
@@ -246,7 +339,7 @@ int gxt_nav_locate(const gxt_navigator *nav, const double *pos, const double *di
   if (e == hipSuccess) e = dt.alloc(n);
   if (e == hipSuccess) e = hipMemcpy(dp.p, pos, n3, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dd.p, dir, n3, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_nav_locate(nav->dev, dp.p, dd.p, dt.p, n, 0);
+  if (e == hipSuccess) e = launch_nav_locate(nav->view(), dp.p, dd.p, dt.p, n, 0);
   if (e == hipSuccess) e = hipMemcpy(touchable, dt.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost);
   return e == hipSuccess ? GXT_OK : hip_fail(e, "gxt_nav_locate");
 }
@@ -261,7 +354,7 @@ int gxt_nav_step(const gxt_navigator *nav, const int32_t *touchable, const doubl
   const int res = residence(arrays, 6, nav->device);
   if (res < 0) return set_error(GXT_EINVAL, "step arrays must be all host or all device memory");
   if (res == 1) {
-    const hipError_t e = launch_nav_step(nav->dev, nt, touchable, pos, dir, snext, next, safety, n, 0);
+    const hipError_t e = launch_nav_step(nav->view(), touchable, pos, dir, snext, next, safety, n, 0);
     return e == hipSuccess ? GXT_OK : hip_fail(e, "gxt_nav_step");
   }
   for (int i = 0; i < n; ++i)
@@ -280,7 +373,7 @@ int gxt_nav_step(const gxt_navigator *nav, const int32_t *touchable, const doubl
   if (e == hipSuccess) e = hipMemcpy(dp.p, pos, n3, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dd.p, dir, n3, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dt.p, touchable, ni, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_nav_step(nav->dev, nt, dt.p, dp.p, dd.p, ds.p, dn.p, dsaf.p, n, 0);
+  if (e == hipSuccess) e = launch_nav_step(nav->view(), dt.p, dp.p, dd.p, ds.p, dn.p, dsaf.p, n, 0);
   if (e == hipSuccess) e = hipMemcpy(snext, ds.p, n1, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(next, dn.p, ni, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(safety, dsaf.p, n1, hipMemcpyDeviceToHost);

@@ -108,6 +108,35 @@ struct alignas(64) NavNode {
   int32_t volume;  // index into the caller's volume array
 };
 
+// Daughter index of a volume with many daughters: nb bins of its extent
+// along axis ax (mother frame, bin = (x + h) inv); the daughters that may hold
+// a point of bin b are ordinals idx[idx[off + b] .. idx[off + b + 1]), in
+// ascending order.  nb = 0: no index (the daughters are scanned).
+struct NavGrid {
+  double inv;
+  int32_t ax, nb, off, pad;
+};
+
+// Bin of mother-frame coordinate x in a daughter index (clamped; NaN -> 0).
+// The same function places the daughters' extents when the index is built:
+// it is monotone in x, so a point inside a daughter's extent falls in one of
+// the daughter's bins.
+__host__ __device__ inline int nav_bin(double x, double h, double inv, int nb) {
+  const double u = (x + h) * inv;
+  if (!(u >= 0.0)) return 0;
+  if (u >= (double)nb) return nb - 1;
+  return (int)u;
+}
+
+// A volume tree on the device.
+struct DevNav {
+  const NavNode *node;
+  const NavGrid *grid;   // per volume
+  const int32_t *idx;
+  int ntouch;
+  int pad;
+};
+
 // Everything a kernel needs besides the pool: lives in one device buffer.
 struct DevParams {
   DevMaterial mat[kMaxSlots];

@@ -14,7 +14,7 @@ import pytest
 from geantexascalepilot_amd import abi
 import oracle_harness as O
 from test_gpu_parity import _flush
-from test_nav import _unit, random_tree
+from test_nav import _unit, many_daughters_tree, random_tree
 
 pytestmark = pytest.mark.gpu
 
@@ -91,6 +91,43 @@ def test_random_trees_bit_exact(gpu_lib, seed):
         n = 50000
         pos = rs.uniform(-1.1, 1.1, (3, n)) * half
         d = _unit(rs, n)
+        t = g.locate(pos, d)
+        np.testing.assert_array_equal(t, c.locate(pos, d))
+        ins = t >= 0
+        _same(g.step(t[ins], pos[:, ins], d[:, ins]), c.step(t[ins], pos[:, ins], d[:, ins]))
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("seed,layers", [(31, False), (32, True), (33, False)])
+def test_many_daughters_bit_exact(gpu_lib, seed, layers):
+    """Volumes with hundreds of daughters take the device's daughter index;
+    the oracle scans every daughter.  Points include every daughter's faces."""
+    _flush("nav many daughters seed %d layers %d" % (seed, layers))
+    rs = np.random.default_rng(seed)
+    vols, pls = many_daughters_tree(rs, n=400, layers=layers)
+    g, c = _pair(gpu_lib, vols, pls)
+    try:
+        info = c.info()
+        assert len(info["volume"]) > 300
+        half = np.array([vols[v][0] for v in info["volume"]])
+        n = 60000
+        pos = rs.uniform(-1.02, 1.02, (3, n)) * 100.0
+        # points on faces: a random face of a random touchable
+        m = n // 2
+        tt = rs.integers(1, len(half), m)
+        ax = rs.integers(0, 3, m)
+        sign = rs.choice([-1.0, 1.0], m)
+        pos[:, :m] = info["centre"][tt].T + rs.uniform(-1, 1, (3, m)) * half[tt].T
+        pos[ax, np.arange(m)] = info["centre"][tt, ax] + sign * half[tt, ax]
+        d = _unit(rs, n)
+        # along the face normal (in and out), and tangent to the face
+        normal = np.arange(0, m, 5)
+        d[:, normal] = 0.0
+        d[ax[normal], normal] = rs.choice([-1.0, 1.0], len(normal))
+        tangent = np.arange(1, m, 5)
+        d[ax[tangent], tangent] = 0.0
+        d[:, tangent] /= np.linalg.norm(d[:, tangent], axis=0)
         t = g.locate(pos, d)
         np.testing.assert_array_equal(t, c.locate(pos, d))
         ins = t >= 0

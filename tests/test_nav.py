@@ -135,6 +135,42 @@ def random_tree(rs, nvol=7, max_daughters=4):
     return vols, pls
 
 
+def many_daughters_tree(rs, n=300, layers=False):
+    """One mother holding up to n small boxes placed without overlap by
+    rejection (layers: slabs stacked along y, each holding boxes), so that
+    volumes have far more daughters than the device's index threshold."""
+    world = np.array([100.0, 100.0, 100.0])
+    vols = [(world, -1)]
+    pls = []
+    if layers:
+        vols.append(((100.0, 4.0, 100.0), -1))
+        for i in range(25):
+            pls.append((0, 1, (0.0, -96.0 + 8.0 * i, 0.0)))
+        mother, mhalf = 1, np.array([100.0, 4.0, 100.0])
+    else:
+        mother, mhalf = 0, world
+    boxes = []
+    for _ in range(n):
+        h = rs.uniform(0.5, 6.0, 3)
+        h = np.minimum(h, mhalf)
+        lim = mhalf - h
+        for _try in range(20):
+            t = rs.uniform(-lim, lim)
+            if rs.random() < 0.3:  # snap to a neighbour's face
+                if boxes:
+                    t2, h2 = boxes[int(rs.integers(len(boxes)))]
+                    k = int(rs.integers(3))
+                    t[k] = t2[k] + h2[k] + h[k]
+                    if abs(t[k]) > lim[k]:
+                        continue
+            if all((np.abs(t - t2) >= h + h2).any() for t2, h2 in boxes):
+                boxes.append((t, h))
+                vols.append((h, 0))
+                pls.append((mother, len(vols) - 1, t))
+                break
+    return vols, pls
+
+
 class BruteTree:
     """Touchables as plain boxes: the deepest box holding a point, by testing
     every box."""
@@ -154,10 +190,14 @@ class BruteTree:
         return np.where(lvl.max(axis=1) >= 0, best, -1)
 
 
-@pytest.mark.parametrize("seed", [3, 4, 5, 6])
-def test_random_tree_against_brute_force(seed):
+@pytest.mark.parametrize("seed,shape", [(3, "nested"), (4, "nested"), (5, "nested"), (6, "nested"),
+                                        (7, "many"), (8, "layers")])
+def test_random_tree_against_brute_force(seed, shape):
     rs = np.random.default_rng(seed)
-    vols, pls = random_tree(rs)
+    if shape == "nested":
+        vols, pls = random_tree(rs)
+    else:
+        vols, pls = many_daughters_tree(rs, n=16 if shape == "layers" else 120, layers=shape == "layers")
     nav = O.Navigator(vols, pls, 0)
     info = nav.info()
     half = np.array([vols[v][0] for v in info["volume"]])
