@@ -1056,6 +1056,27 @@ __device__ __forceinline__ int nav_step(const DevNav &g, int t, const double p[3
   return nav_relocate(g, enter >= 0 ? enter : t, q, v);
 }
 
+// NavigationInterface::FindNextBoundary (NavigationInterface.hpp:64-98) on
+// top of nav_step: the boundary distance limited to the physics step pstep,
+// the safety shortcut, the boundary flag and the 2 kNavTol push.
+__device__ __forceinline__ int nav_find_next_boundary(const DevNav &g, int t, const double p[3], const double v[3],
+                                                      double pstep, double &snext, double &safety, int &boundary) {
+  double d;
+  int next = nav_step(g, t, p, v, d, safety);
+  if (!(d < pstep)) {
+    d = pstep;
+    next = t;
+  }
+  if (safety > pstep) {
+    snext = pstep;
+    boundary = 0;
+    return t;
+  }
+  boundary = d < pstep ? 1 : 0;
+  snext = fmax(2.0 * kNavTol, d + 2.0 * kNavTol);
+  return next;
+}
+
 // ConstFieldHelixStepper::DoStep (magneticfield/ConstFieldHelixStepper.hpp:164-239)
 // in mm/MeV/tesla; the direction is renormalised (FieldPropagationHandler.cpp:323-330).
 // The arithmetic is in R; the position update is accumulated in FP64.

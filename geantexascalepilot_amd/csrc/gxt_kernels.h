@@ -96,5 +96,8 @@ hipError_t launch_nav_locate(const DevNav &g, const double *pos, const double *d
                              hipStream_t st);
 hipError_t launch_nav_step(const DevNav &g, const int32_t *touchable, const double *pos, const double *dir,
                            double *snext, int32_t *next, double *safety, int n, hipStream_t st);
+hipError_t launch_nav_find_next_boundary(const DevNav &g, const int32_t *touchable, const double *pos,
+                                        const double *dir, const double *pstep, double *snext, int32_t *next,
+                                        double *safety, int32_t *boundary, int n, hipStream_t st);
 
 }  // namespace gxt

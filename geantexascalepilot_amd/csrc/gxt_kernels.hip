@@ -1020,6 +1020,32 @@ __global__ void k_nav_step(DevNav g, const int32_t *touchable, const double *pos
   safety[i] = saf;
 }
 
+// A touchable outside [0, g.ntouch) or a pstep that is negative or not
+// finite gives next = -2 and NaN distances.
+__global__ void k_nav_find_next_boundary(DevNav g, const int32_t *touchable, const double *pos, const double *dir,
+                                         const double *pstep, double *snext, int32_t *next, double *safety,
+                                         int32_t *boundary, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int t = touchable[i];
+  const double ps = pstep[i];
+  if (t < 0 || t >= g.ntouch || !(ps >= 0.0) || !(ps <= 1e300)) {
+    next[i] = -2;
+    snext[i] = __builtin_nan("");
+    safety[i] = __builtin_nan("");
+    boundary[i] = 0;
+    return;
+  }
+  const double p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
+  const double v[3] = {dir[i], dir[n + i], dir[2 * n + i]};
+  double s, saf;
+  int b;
+  next[i] = nav_find_next_boundary(g, t, p, v, ps, s, saf, b);
+  snext[i] = s;
+  safety[i] = saf;
+  boundary[i] = b;
+}
+
 template <class R>
 __global__ void k_field_step(DevField F, const double *charge, const double *mom, const double *step, double *pos,
                              double *dir, double *err, int n) {
@@ -1167,6 +1193,14 @@ hipError_t launch_nav_step(const DevNav &g, const int32_t *touchable, const doub
                            double *snext, int32_t *next, double *safety, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_nav_step, dim3(grid_for(n)), dim3(kBlock), 0, st, g, touchable, pos, dir, snext, next, safety,
                      n);
+  return hipGetLastError();
+}
+
+hipError_t launch_nav_find_next_boundary(const DevNav &g, const int32_t *touchable, const double *pos,
+                                        const double *dir, const double *pstep, double *snext, int32_t *next,
+                                        double *safety, int32_t *boundary, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_nav_find_next_boundary, dim3(grid_for(n)), dim3(kBlock), 0, st, g, touchable, pos, dir, pstep,
+                     snext, next, safety, boundary, n);
   return hipGetLastError();
 }
 

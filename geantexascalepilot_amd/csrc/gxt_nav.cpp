@@ -381,3 +381,49 @@ int gxt_nav_step(const gxt_navigator *nav, const int32_t *touchable, const doubl
 }
 
 }  // extern "C"
+
+extern "C" int gxt_nav_find_next_boundary(const gxt_navigator *nav, const int32_t *touchable, const double *pos,
+                                          const double *dir, const double *pstep, double *snext, int32_t *next,
+                                          double *safety, int32_t *boundary, int n) {
+  if (!nav || !nav->dev || !touchable || !pos || !dir || !pstep || !snext || !next || !safety || !boundary || n <= 0)
+    return set_error(GXT_EINVAL, "bad find-next-boundary arguments");
+  if (const int rc = check_device(nav)) return rc;
+  const int nt = (int)nav->node.size();
+  const void *arrays[8] = {touchable, pos, dir, pstep, snext, next, safety, boundary};
+  const int res = residence(arrays, 8, nav->device);
+  if (res < 0) return set_error(GXT_EINVAL, "find-next-boundary arrays must be all host or all device memory");
+  if (res == 1) {
+    const hipError_t e =
+        launch_nav_find_next_boundary(nav->view(), touchable, pos, dir, pstep, snext, next, safety, boundary, n, 0);
+    return e == hipSuccess ? GXT_OK : hip_fail(e, "gxt_nav_find_next_boundary");
+  }
+  for (int i = 0; i < n; ++i) {
+    if (touchable[i] < 0 || touchable[i] >= nt)
+      return set_error(GXT_EINVAL, "point " + std::to_string(i) + ": touchable out of range");
+    if (!(pstep[i] >= 0.0) || !(pstep[i] <= 1e300))
+      return set_error(GXT_EINVAL, "point " + std::to_string(i) + ": pstep must be finite and >= 0");
+  }
+  if (const int rc = check_points(pos, dir, n)) return rc;
+  const size_t n1 = (size_t)n * sizeof(double), n3 = 3 * n1, ni = (size_t)n * sizeof(int32_t);
+  DevBuf<double> dp, dd, dps, ds, dsaf;
+  DevBuf<int32_t> dt, dn, db;
+  hipError_t e = dp.alloc(3 * (size_t)n);
+  if (e == hipSuccess) e = dd.alloc(3 * (size_t)n);
+  if (e == hipSuccess) e = dps.alloc(n);
+  if (e == hipSuccess) e = ds.alloc(n);
+  if (e == hipSuccess) e = dsaf.alloc(n);
+  if (e == hipSuccess) e = dt.alloc(n);
+  if (e == hipSuccess) e = dn.alloc(n);
+  if (e == hipSuccess) e = db.alloc(n);
+  if (e == hipSuccess) e = hipMemcpy(dp.p, pos, n3, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dd.p, dir, n3, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dps.p, pstep, n1, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dt.p, touchable, ni, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_nav_find_next_boundary(nav->view(), dt.p, dp.p, dd.p, dps.p, ds.p, dn.p, dsaf.p, db.p, n, 0);
+  if (e == hipSuccess) e = hipMemcpy(snext, ds.p, n1, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(next, dn.p, ni, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(safety, dsaf.p, n1, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(boundary, db.p, ni, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? GXT_OK : hip_fail(e, "gxt_nav_find_next_boundary");
+}

@@ -223,6 +223,20 @@ class Navigator:
                                 abi.ptr(nxt, C.c_int32), abi.ptr(saf), n))
         return s, nxt, saf
 
+    def find_next_boundary(self, touchable, pos, dirs, pstep):
+        """FindNextBoundary: (snext, next touchable, safety, boundary flag) for proposed steps pstep."""
+        t = np.ascontiguousarray(touchable, dtype=np.int32)
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        dirs = np.ascontiguousarray(dirs, dtype=np.float64)
+        ps = np.ascontiguousarray(pstep, dtype=np.float64)
+        n = len(t)
+        s, saf = np.zeros(n), np.zeros(n)
+        nxt = np.zeros(n, dtype=np.int32)
+        b = np.zeros(n, dtype=np.int32)
+        _check(lib.gxt_nav_find_next_boundary(self.h, abi.ptr(t, C.c_int32), abi.ptr(pos), abi.ptr(dirs), abi.ptr(ps), abi.ptr(s),
+                 abi.ptr(nxt, C.c_int32), abi.ptr(saf), abi.ptr(b, C.c_int32), n))
+        return s, nxt, saf, b
+
     def close(self):
         if self.h:
             _check(lib.gxt_nav_free(self.h))

@@ -409,6 +409,20 @@ int gxt_nav_locate(const gxt_navigator *nav, const double *pos, const double *di
  * the safety (distance to the nearest boundary in any direction). */
 int gxt_nav_step(const gxt_navigator *nav, const int32_t *touchable, const double *pos, const double *dir,
                  double *snext, int32_t *next, double *safety, int n);
+/* NavigationInterface::FindNextBoundary (geometry/NavigationInterface.hpp:
+ * 64-98) for n tracks with proposed physics steps pstep[i] >= 0: with d the
+ * boundary distance of gxt_nav_step limited to pstep,
+ *   safety > pstep:  snext = pstep, boundary = 0;
+ *   otherwise:       boundary = (d < pstep), snext = max(2e-9, d + 2e-9)
+ *                    (the push past the surface);
+ * next is the touchable beyond the boundary (boundary = 1) or touchable[i];
+ * safety is the isotropic safety, >= 0 (the reference leaves the track's
+ * previous safety in place in the first case).  Same array rules as
+ * gxt_nav_step; a negative or non-finite pstep is GXT_EINVAL for host
+ * arrays and gives next = -2, NaN distances for device arrays. */
+int gxt_nav_find_next_boundary(const gxt_navigator *nav, const int32_t *touchable, const double *pos,
+                               const double *dir, const double *pstep, double *snext, int32_t *next, double *safety,
+                               int32_t *boundary, int n);
 
 #ifdef __cplusplus
 }

@@ -2586,3 +2586,32 @@ int gxref_nav_step(const gxref_nav *g, const int32_t *cur, const double *pos, co
 }
 
 }  // extern "C"
+
+// NavigationInterface::FindNextBoundary (geometry/NavigationInterface.hpp:64-98):
+// the boundary distance within the physics step, the safety shortcut, the
+// boundary flag and the push of 2 x 1e-9 mm past the surface.
+extern "C" int gxref_nav_find_next_boundary(const gxref_nav *g, const int32_t *cur, const double *pos,
+                                            const double *dir, const double *pstep, double *snext, int32_t *next,
+                                            double *safety, int32_t *boundary, int n) {
+  if (!pstep || !boundary) return gxref::fail(GXT_EINVAL, "bad find-next-boundary arguments");
+  const int rc = gxref_nav_step(g, cur, pos, dir, snext, next, safety, n);
+  if (rc) return rc;
+  const double tol = gxref::nav::kTol;
+  for (int i = 0; i < n; ++i) {
+    const double ps = pstep[i];
+    double s = snext[i];
+    if (!(s < ps)) {  // no boundary within the proposed step
+      s = ps;
+      next[i] = cur[i];
+    }
+    if (safety[i] > ps) {
+      snext[i] = ps;
+      boundary[i] = 0;
+      next[i] = cur[i];
+      continue;
+    }
+    boundary[i] = s < ps;
+    snext[i] = std::max(2.0 * tol, s + 2.0 * tol);
+  }
+  return GXT_OK;
+}

@@ -88,6 +88,9 @@ int gxref_nav_info(const gxref_nav *nav, int *n_touchables, int32_t *volume, int
 int gxref_nav_locate(const gxref_nav *nav, const double *pos, const double *dir, int32_t *touchable, int n);
 int gxref_nav_step(const gxref_nav *nav, const int32_t *touchable, const double *pos, const double *dir,
                    double *snext, int32_t *next, double *safety, int n);
+int gxref_nav_find_next_boundary(const gxref_nav *nav, const int32_t *touchable, const double *pos, const double *dir,
+                                 const double *pstep, double *snext, int32_t *next, double *safety, int32_t *boundary,
+                                 int n);
 /* Sandia photo-absorption macroscopic cross-section (1/mm). */
 int gxref_sandia_xs(gxref_ctx *ctx, int material_slot, const double *ekin, double *out, int n);
 

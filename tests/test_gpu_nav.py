@@ -197,6 +197,38 @@ def test_device_arrays_match_host_arrays(calo, gpu_lib):
         a.free()
 
 
+def test_find_next_boundary_bit_exact(calo, gpu_lib):
+    """FindNextBoundary over random points and proposed steps that fall below
+    the safety, between safety and boundary, and beyond the boundary; host
+    arrays and device arrays."""
+    _flush("nav find_next_boundary")
+    g, c = calo
+    rs = np.random.default_rng(25)
+    n = 100000
+    pos = rs.uniform(-1.0, 1.0, (3, n)) * 600.0
+    d = _unit(rs, n)
+    t = c.locate(pos, d)
+    ins = t >= 0
+    t, pos, d = t[ins], pos[:, ins], d[:, ins]
+    n = len(t)
+    s0, _, saf0 = c.step(t, pos, d)
+    u = rs.uniform(0.0, 1.0, n)
+    pstep = np.select([u < 0.25, u < 0.5, u < 0.75, u < 0.8], [saf0 * rs.uniform(0, 1, n), s0,
+                      saf0 + (s0 - saf0) * rs.uniform(0, 1, n), np.zeros(n)], s0 * 3 + 1)
+    want = c.find_next_boundary(t, pos, d, pstep)
+    _same(g.find_next_boundary(t, pos, d, pstep), want)
+    assert 0.1 < want[3].mean() < 0.5
+    D = gpu_lib.DeviceArray
+    arrs = [D(t), D(pos), D(d), D(pstep), D(np.zeros(n)), D(np.zeros(n, dtype=np.int32)), D(np.zeros(n)),
+            D(np.zeros(n, dtype=np.int32))]
+    gpu_lib._check(gpu_lib.lib.gxt_nav_find_next_boundary(g.h, *[a.ptr for a in arrs], n))
+    _same([arrs[4].numpy(), arrs[5].numpy(), arrs[6].numpy(), arrs[7].numpy()], want)
+    for a in arrs:
+        a.free()
+    with pytest.raises(gpu_lib.GxtError, match="pstep"):
+        g.find_next_boundary(t[:1], pos[:, :1], d[:, :1], np.array([-1.0]))
+
+
 def test_bad_touchable_rejected(calo, gpu_lib):
     g, _ = calo
     pos = np.zeros((3, 1))

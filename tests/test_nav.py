@@ -295,3 +295,29 @@ def test_overlap_sweep_matches_pairwise_check():
         assert (a == -1) == (b == -1)
         verdicts.append(a == -1)
     assert 0.2 < np.mean(verdicts) < 0.9
+
+
+def test_find_next_boundary_cases():
+    """NavigationInterface::FindNextBoundary (NavigationInterface.hpp:64-98)
+    on the reference calorimeter: a point 5 mm into the first Pb slab moving
+    +x has boundary distance 5 and safety 5; a point on the Pb/lAr face moving
+    +x is in lAr with distance 10 and safety 0."""
+    nav = O.Navigator(*abi.reference_calorimeter_tree())
+    tol2 = 2e-9
+    pos = np.array([[-495.0] * 4 + [-490.0] * 2, [0.0] * 6, [0.0] * 6])
+    d = np.array([[1.0] * 6, [0.0] * 6, [0.0] * 6])
+    t = nav.locate(pos, d)
+    np.testing.assert_array_equal(t, [52, 52, 52, 52, 53, 53])
+    pstep = np.array([1.0, 5.0, 7.0, 1e20, 0.0, 3.0])
+    s, nxt, saf, b = nav.find_next_boundary(t, pos, d, pstep)
+    # safety 5 > pstep 1: the physics step, no boundary
+    assert (s[0], nxt[0], b[0]) == (1.0, 52, 0)
+    # safety 5 <= pstep 5 = distance: no boundary, pushed past the step
+    assert (s[1], nxt[1], b[1]) == (5.0 + tol2, 52, 0)
+    # boundary within the step: pushed past the face, into the lAr slab
+    assert (s[2], nxt[2], b[2]) == (5.0 + tol2, 53, 1)
+    assert (s[3], nxt[3], b[3]) == (5.0 + tol2, 53, 1)
+    # on the face (safety 0): pstep 0 gives the minimum push; pstep 3 < 10 no boundary
+    assert (s[4], nxt[4], b[4]) == (tol2, 53, 0)
+    assert (s[5], nxt[5], b[5]) == (3.0 + tol2, 53, 0)
+    np.testing.assert_array_equal(saf, [5.0, 5.0, 5.0, 5.0, 0.0, 0.0])
