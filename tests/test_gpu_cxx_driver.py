@@ -1,0 +1,66 @@
+"""The C ABI from a plain C++ program (tests/cxx/gxt_driver.cpp, built by
+__graft_entry__.build()): the transport of 10 GeV e- in the PbWO4 block gives
+the same events as the Python binding of the same configuration (counts
+exact; energies to 1e-12, the order of the device's FP64 atomic sums varies
+from run to run), and FindNextBoundary on the reference calorimeter tree
+equals the oracle's bit for bit."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from geantexascalepilot_amd import abi
+import oracle_harness as O
+from test_gpu_parity import _flush
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "cxx", "gxt_driver")
+
+
+def _driver(*args):
+    assert os.path.exists(DRIVER), "tests/cxx/gxt_driver is not built (run __graft_entry__.build())"
+    r = subprocess.run([DRIVER, abi.DATA_DIR] + list(args), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout.split("\n")
+
+
+def test_cxx_transport_matches_python_binding(gpu_lib):
+    n = 24
+    _flush("C++ driver: %d x 10 GeV e- in PbWO4" % n)
+    rows = [line.split() for line in _driver("block", str(n)) if line.startswith("event")]
+    assert len(rows) == n
+    cfg = abi.config_pbwo4_block(max_events=n)
+    prims = gpu_lib.make_primaries(777, abi.ELECTRON, 10000.0, 10000.0, 0.0, n)
+    with gpu_lib.Transport(cfg) as t:
+        t.submit(prims, 1)
+        t.run()
+        ev = t.event_tallies(1, n)
+    for i, r in enumerate(rows):
+        assert int(r[1]) == i
+        np.testing.assert_allclose([float(x) for x in r[2:5]], [ev["edep"][i], ev["visible"][i], ev["leak"][i]],
+                                   rtol=1e-12, atol=1e-9)
+        assert int(r[5]) == ev["n_steps"][i]
+        assert [int(x) for x in r[6:9]] == list(ev["n_secondaries"][i])
+
+
+def test_cxx_find_next_boundary_matches_oracle():
+    _flush("C++ driver: navigation")
+    out = _driver("nav")
+    assert out[0] == "touchables 152"
+    rows = [line.split() for line in out if line.startswith("track")]
+    n = len(rows)
+    assert n == 6
+    # the driver's tracks
+    pos = np.array([[-495.0, -495.0, -490.0, 0.0, 13.7, 550.0], [0.0, 0.0, 0.0, 499.0, -20.0, 0.0],
+                    [0.0, 0.0, 0.0, 0.0, 7.5, 0.0]])
+    d = np.array([[1.0, 1.0, 1.0, 0.6, -0.48, -1.0], [0.0, 0.0, 0.0, 0.8, 0.6, 0.0], [0.0, 0.0, 0.0, 0.0, 0.64, 0.0]])
+    pstep = np.array([1.0, 7.0, 0.0, 1e20, 3.0, 1e20])
+    c = O.Navigator(*abi.reference_calorimeter_tree())
+    t = c.locate(pos, d)
+    s, nxt, saf, b = c.find_next_boundary(t, pos, d, pstep)
+    for i, r in enumerate(rows):
+        assert int(r[1]) == i and int(r[2]) == t[i]
+        assert float(r[3]) == s[i] and int(r[4]) == nxt[i] and float(r[5]) == saf[i] and int(r[6]) == b[i]
