@@ -28,12 +28,12 @@ PEAK_HBM_GBS = 8000.0
 ME = 0.510998910
 
 
-def profiled_traffic(kernel):
+def profiled_traffic(kernel, summary="hbm_traffic_c1_fp64.md"):
     """HBM bytes per launch of `kernel` as measured by the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a copy
+    FETCH_SIZE / WRITE_SIZE summary of its workload (calibrated against a copy
     kernel of known size); None when the summary does not list the kernel."""
     import re
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_c1_fp64.md")
+    path = os.path.join(ROOT, "profiles", summary)
     try:
         text = open(path).read()
     except OSError:
@@ -221,7 +221,9 @@ def navigation(g, abi, n=1 << 24, reps=20, cpu=False, seed=99):
                step_queries_per_s=n / (us * 1e-6), step_launch_us=us, locate_queries_per_s=n / (loc_us * 1e-6),
                locate_launch_us=loc_us, mean_snext_mm=float(s.mean()), into_world=int((nxt == 0).sum()),
                roofline=dict(bound="hbm", kernel="k_nav_step", achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s",
-                             frac=achieved / PEAK_HBM_GBS, per_unit="72 B per query"))
+                             frac=achieved / PEAK_HBM_GBS, per_unit="72 B per query",
+                             traffic=profiled_traffic("k_nav_step", "hbm_traffic_nav_reference_calorimeter.md")
+                             if n == 1 << 24 else None))
     log("navigation: step %.1f us, locate %.1f us for %d points" % (us, loc_us, n))
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -414,12 +416,6 @@ def main():
     from geantexascalepilot_amd import abi
     import geantexascalepilot_amd.gxt as g
 
-    if a.nav_only:
-        print(json.dumps(dict(navigation=navigation(g, abi, cpu=not a.skip_cpu))), flush=True)
-        return
-    if a.field_only:
-        print(json.dumps(dict(field_forms=field_forms(g, abi, cpu=not a.skip_cpu))), flush=True)
-        return
     if a.copy_probe > 0:
         src = torch.ones(a.copy_probe << 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -428,6 +424,12 @@ def main():
         torch.cuda.synchronize()
         log("copy probe: %d MiB read, %d MiB written" % (a.copy_probe, a.copy_probe))
         del src, dst
+    if a.nav_only:
+        print(json.dumps(dict(navigation=navigation(g, abi, cpu=not a.skip_cpu))), flush=True)
+        return
+    if a.field_only:
+        print(json.dumps(dict(field_forms=field_forms(g, abi, cpu=not a.skip_cpu))), flush=True)
+        return
     n = a.events
     r64 = run_batch(g, abi, abi.FP64, n, a.steps, a.warmup, pool=a.pool)
     value = a.steps * n / r64["seconds"]
