@@ -1121,7 +1121,8 @@ int gxt_macro_xs(gxt_ctx *c, int slot, int species, int process, int precision, 
 }
 
 int gxt_sample(gxt_ctx *c, int model, int slot, int precision, gxt_sample_io *io) {
-  if (!c || !io || io->n <= 0 || model < 0 || model > GXT_MODEL_ANNIHIL || slot < 0 || slot >= nslots(c))
+  if (!c || !io || io->n <= 0 || io->n > kMaxSoaPoints || model < 0 || model > GXT_MODEL_ANNIHIL || slot < 0 ||
+      slot >= nslots(c))
     return set_err(GXT_EINVAL, "bad sample arguments");
   const int n = io->n;
   SampleArgs s;
@@ -1169,7 +1170,8 @@ int gxt_sample(gxt_ctx *c, int model, int slot, int precision, gxt_sample_io *io
 
 int gxt_helix_step(const double field[3], int precision, const double *charge, const double *mom, const double *step,
                    double *pos, double *dir, int n) {
-  if (!field || !charge || !mom || !step || !pos || !dir || n <= 0) return set_err(GXT_EINVAL, "bad helix arguments");
+  if (!field || !charge || !mom || !step || !pos || !dir || n <= 0 || n > kMaxSoaPoints)
+    return set_err(GXT_EINVAL, "bad helix arguments (1 <= n <= 2^29)");
   const double bm = std::sqrt(field[0] * field[0] + field[1] * field[1] + field[2] * field[2]);
   if (!(bm > 0)) return set_err(GXT_EINVAL, "zero field");
   double *dq = nullptr, *dm = nullptr, *ds = nullptr, *dp = nullptr, *dd = nullptr;

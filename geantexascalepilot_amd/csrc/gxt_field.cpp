@@ -203,7 +203,8 @@ int gxt_field_map_info(const gxt_field_map *m, int *nr, int *nz, double *r0, dou
 }
 
 int gxt_field_map_value(const gxt_field_map *m, const double *pos, double *b, int n) {
-  if (!m || !m->dev || !pos || !b || n <= 0) return set_error(GXT_EINVAL, "bad field-map value arguments");
+  if (!m || !m->dev || !pos || !b || n <= 0 || n > kMaxSoaPoints)
+    return set_error(GXT_EINVAL, "bad field-map value arguments (1 <= n <= 2^29)");
   if (const int rc = check_device(m)) return rc;
   const DevField f = dev_field(m, nullptr, nullptr, GXT_STEPPER_HELIX);
   DevBuf<double> dp, db;
@@ -218,7 +219,7 @@ int gxt_field_map_value(const gxt_field_map *m, const double *pos, double *b, in
 int gxt_field_step(const gxt_field_map *m, const double field[3], const double origin[3], int stepper, int precision,
                    const double *charge, const double *mom, const double *step, double *pos, double *dir, double *err,
                    int n) {
-  if ((!m && !field) || !charge || !mom || !step || !pos || !dir || n <= 0)
+  if ((!m && !field) || !charge || !mom || !step || !pos || !dir || n <= 0 || n > kMaxSoaPoints)
     return set_error(GXT_EINVAL, "bad field-step arguments");
   if (m && !m->dev) return set_error(GXT_EINVAL, "field map not on a device");
   if (m)

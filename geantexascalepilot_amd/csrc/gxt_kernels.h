@@ -91,12 +91,15 @@ hipError_t launch_field_step(const DevField &F, int precision, const double *cha
                              const double *step, double *pos, double *dir, double *err, int n, hipStream_t st);
 
 // Volume-tree queries over n points (pos, dir: [3n] SoA) in the tree g (the
-// step kernel checks touchable[i] against g.ntouch).
-hipError_t launch_nav_locate(const DevNav &g, const double *pos, const double *dir, int32_t *out, int n,
-                             hipStream_t st);
-hipError_t launch_nav_step(const DevNav &g, const int32_t *touchable, const double *pos, const double *dir,
-                           double *snext, int32_t *next, double *safety, int n, hipStream_t st);
-hipError_t launch_nav_find_next_boundary(const DevNav &g, const int32_t *touchable, const double *pos,
+// step kernels check touchable[i] against g.ntouch).  lds_bytes > 0: the
+// tree is one allocation of lds_bytes (a multiple of 16, at most 48 KiB)
+// starting at g.node, which each workgroup copies into LDS; 0: it is read in
+// place.
+hipError_t launch_nav_locate(const DevNav &g, int lds_bytes, const double *pos, const double *dir, int32_t *out,
+                             int n, hipStream_t st);
+hipError_t launch_nav_step(const DevNav &g, int lds_bytes, const int32_t *touchable, const double *pos,
+                           const double *dir, double *snext, int32_t *next, double *safety, int n, hipStream_t st);
+hipError_t launch_nav_find_next_boundary(const DevNav &g, int lds_bytes, const int32_t *touchable, const double *pos,
                                         const double *dir, const double *pstep, double *snext, int32_t *next,
                                         double *safety, int32_t *boundary, int n, hipStream_t st);
 

@@ -992,58 +992,89 @@ __global__ void k_field_value(DevField F, const double *pos, double *b, int n) {
   for (int k = 0; k < 3; ++k) b[k * n + i] = B[k];
 }
 
-__global__ void k_nav_locate(DevNav g, const double *pos, const double *dir, int32_t *out, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
-  const double v[3] = {dir[i], dir[n + i], dir[2 * n + i]};
-  out[i] = nav_locate(g, p, v);
+// Volume-tree queries.  With kLds the workgroup first copies the whole tree
+// (nodes, grids and daughter index: lds_bytes, a multiple of 16, read from
+// one padded allocation) into LDS, where the node lookups of all its lanes are
+// served; the grid then strides over the queries.  Without it the lookups go
+// to the tree in device memory.
+extern __shared__ uint4 nav_lds[];
+
+template <bool kLds>
+__device__ __forceinline__ DevNav nav_view(const DevNav &g, int lds_bytes) {
+  if (!kLds) return g;
+  const uint4 *src = reinterpret_cast<const uint4 *>(g.node);
+  for (int k = threadIdx.x; k < lds_bytes / 16; k += blockDim.x) nav_lds[k] = src[k];
+  __syncthreads();
+  const char *base = reinterpret_cast<const char *>(g.node);
+  const char *lds = reinterpret_cast<const char *>(nav_lds);
+  DevNav l = g;
+  l.node = reinterpret_cast<const NavNode *>(lds);
+  l.grid = reinterpret_cast<const NavGrid *>(lds + (reinterpret_cast<const char *>(g.grid) - base));
+  l.idx = reinterpret_cast<const int32_t *>(lds + (reinterpret_cast<const char *>(g.idx) - base));
+  return l;
+}
+
+template <bool kLds>
+__global__ __launch_bounds__(kBlock) void k_nav_locate(DevNav g0, int lds_bytes, const double *pos, const double *dir,
+                                                       int32_t *out, int n) {
+  const DevNav g = nav_view<kLds>(g0, lds_bytes);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
+    const double v[3] = {dir[i], dir[n + i], dir[2 * n + i]};
+    out[i] = nav_locate(g, p, v);
+  }
 }
 
 // A touchable outside [0, g.ntouch) gives next = -2 and NaN distances.
-__global__ void k_nav_step(DevNav g, const int32_t *touchable, const double *pos, const double *dir, double *snext,
-                           int32_t *next, double *safety, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int t = touchable[i];
-  if (t < 0 || t >= g.ntouch) {
-    next[i] = -2;
-    snext[i] = __builtin_nan("");
-    safety[i] = __builtin_nan("");
-    return;
+template <bool kLds>
+__global__ __launch_bounds__(kBlock) void k_nav_step(DevNav g0, int lds_bytes, const int32_t *touchable,
+                                                     const double *pos, const double *dir, double *snext,
+                                                     int32_t *next, double *safety, int n) {
+  const DevNav g = nav_view<kLds>(g0, lds_bytes);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int t = touchable[i];
+    if (t < 0 || t >= g.ntouch) {
+      next[i] = -2;
+      snext[i] = __builtin_nan("");
+      safety[i] = __builtin_nan("");
+      continue;
+    }
+    const double p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
+    const double v[3] = {dir[i], dir[n + i], dir[2 * n + i]};
+    double s, saf;
+    next[i] = nav_step(g, t, p, v, s, saf);
+    snext[i] = s;
+    safety[i] = saf;
   }
-  const double p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
-  const double v[3] = {dir[i], dir[n + i], dir[2 * n + i]};
-  double s, saf;
-  next[i] = nav_step(g, t, p, v, s, saf);
-  snext[i] = s;
-  safety[i] = saf;
 }
 
 // A touchable outside [0, g.ntouch) or a pstep that is negative or not
 // finite gives next = -2 and NaN distances.
-__global__ void k_nav_find_next_boundary(DevNav g, const int32_t *touchable, const double *pos, const double *dir,
-                                         const double *pstep, double *snext, int32_t *next, double *safety,
-                                         int32_t *boundary, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int t = touchable[i];
-  const double ps = pstep[i];
-  if (t < 0 || t >= g.ntouch || !(ps >= 0.0) || !(ps <= 1e300)) {
-    next[i] = -2;
-    snext[i] = __builtin_nan("");
-    safety[i] = __builtin_nan("");
-    boundary[i] = 0;
-    return;
+template <bool kLds>
+__global__ __launch_bounds__(kBlock) void k_nav_find_next_boundary(DevNav g0, int lds_bytes, const int32_t *touchable,
+                                                                   const double *pos, const double *dir,
+                                                                   const double *pstep, double *snext, int32_t *next,
+                                                                   double *safety, int32_t *boundary, int n) {
+  const DevNav g = nav_view<kLds>(g0, lds_bytes);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int t = touchable[i];
+    const double ps = pstep[i];
+    if (t < 0 || t >= g.ntouch || !(ps >= 0.0) || !(ps <= 1e300)) {
+      next[i] = -2;
+      snext[i] = __builtin_nan("");
+      safety[i] = __builtin_nan("");
+      boundary[i] = 0;
+      continue;
+    }
+    const double p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
+    const double v[3] = {dir[i], dir[n + i], dir[2 * n + i]};
+    double s, saf;
+    int b;
+    next[i] = nav_find_next_boundary(g, t, p, v, ps, s, saf, b);
+    snext[i] = s;
+    safety[i] = saf;
+    boundary[i] = b;
   }
-  const double p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
-  const double v[3] = {dir[i], dir[n + i], dir[2 * n + i]};
-  double s, saf;
-  int b;
-  next[i] = nav_find_next_boundary(g, t, p, v, ps, s, saf, b);
-  snext[i] = s;
-  safety[i] = saf;
-  boundary[i] = b;
 }
 
 template <class R>
@@ -1183,24 +1214,59 @@ hipError_t launch_field_value(const DevField &F, const double *pos, double *b, i
   return hipGetLastError();
 }
 
-hipError_t launch_nav_locate(const DevNav &g, const double *pos, const double *dir, int32_t *out, int n,
-                             hipStream_t st) {
-  hipLaunchKernelGGL(k_nav_locate, dim3(grid_for(n)), dim3(kBlock), 0, st, g, pos, dir, out, n);
+// LDS form (trees of at most kNavLdsMax bytes): as many workgroups as the
+// CUs hold at once, striding over the queries (one copy of the tree each and
+// no partial second round: for the step, 6 per CU when resident, 8 per CU
+// ran 10 % slower, 12 to 32 per CU tied with 6); otherwise one thread per
+// query.
+template <class K>
+static dim3 nav_grid(K kernel, int n, int lds_bytes) {
+  const int g = grid_for(n);
+  if (lds_bytes == 0) return dim3(g);
+  int dev = 0, ncu = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                              hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, lds_bytes) != hipSuccess || per < 1 ||
+      ncu < 1) {
+    (void)hipGetLastError();
+    per = 4;
+    ncu = 256;
+  }
+  const int r = per * ncu;
+  return dim3(g < r ? g : r);
+}
+
+hipError_t launch_nav_locate(const DevNav &g, int lds_bytes, const double *pos, const double *dir, int32_t *out,
+                             int n, hipStream_t st) {
+  if (lds_bytes > 0)
+    hipLaunchKernelGGL(k_nav_locate<true>, nav_grid(k_nav_locate<true>, n, lds_bytes), dim3(kBlock), lds_bytes, st, g,
+                       lds_bytes, pos, dir, out, n);
+  else
+    hipLaunchKernelGGL(k_nav_locate<false>, dim3(grid_for(n)), dim3(kBlock), 0, st, g, 0, pos, dir, out, n);
   return hipGetLastError();
 }
 
-hipError_t launch_nav_step(const DevNav &g, const int32_t *touchable, const double *pos, const double *dir,
-                           double *snext, int32_t *next, double *safety, int n, hipStream_t st) {
-  hipLaunchKernelGGL(k_nav_step, dim3(grid_for(n)), dim3(kBlock), 0, st, g, touchable, pos, dir, snext, next, safety,
-                     n);
+hipError_t launch_nav_step(const DevNav &g, int lds_bytes, const int32_t *touchable, const double *pos,
+                           const double *dir, double *snext, int32_t *next, double *safety, int n, hipStream_t st) {
+  if (lds_bytes > 0)
+    hipLaunchKernelGGL(k_nav_step<true>, nav_grid(k_nav_step<true>, n, lds_bytes), dim3(kBlock), lds_bytes, st, g,
+                       lds_bytes, touchable, pos, dir, snext, next, safety, n);
+  else
+    hipLaunchKernelGGL(k_nav_step<false>, dim3(grid_for(n)), dim3(kBlock), 0, st, g, 0, touchable, pos, dir, snext, next,
+                       safety, n);
   return hipGetLastError();
 }
 
-hipError_t launch_nav_find_next_boundary(const DevNav &g, const int32_t *touchable, const double *pos,
+hipError_t launch_nav_find_next_boundary(const DevNav &g, int lds_bytes, const int32_t *touchable, const double *pos,
                                         const double *dir, const double *pstep, double *snext, int32_t *next,
                                         double *safety, int32_t *boundary, int n, hipStream_t st) {
-  hipLaunchKernelGGL(k_nav_find_next_boundary, dim3(grid_for(n)), dim3(kBlock), 0, st, g, touchable, pos, dir, pstep,
-                     snext, next, safety, boundary, n);
+  if (lds_bytes > 0)
+    hipLaunchKernelGGL(k_nav_find_next_boundary<true>, nav_grid(k_nav_find_next_boundary<true>, n, lds_bytes),
+                       dim3(kBlock), lds_bytes, st, g, lds_bytes, touchable, pos, dir, pstep, snext, next, safety,
+                       boundary, n);
+  else
+    hipLaunchKernelGGL(k_nav_find_next_boundary<false>, dim3(grid_for(n)), dim3(kBlock), 0, st, g, 0, touchable, pos,
+                       dir, pstep, snext, next, safety, boundary, n);
   return hipGetLastError();
 }
 

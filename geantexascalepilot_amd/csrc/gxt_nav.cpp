@@ -32,6 +32,9 @@ struct gxt_navigator {
   size_t bytes() const {
     return node.size() * sizeof(gxt::NavNode) + grid.size() * sizeof(gxt::NavGrid) + idx.size() * sizeof(int32_t);
   }
+  // the allocation: whole 16-byte words, which the LDS form copies
+  size_t bytes_padded() const { return (bytes() + 15) & ~(size_t)15; }
+  int lds_bytes() const { return bytes_padded() <= (size_t)gxt::kNavLdsMax ? (int)bytes_padded() : 0; }
 };
 
 using namespace gxt;
@@ -278,7 +281,7 @@ int gxt_nav_create(const gxt_box_volume *volumes, int n_volumes, const gxt_place
     const double *wh = volumes[world].half;
     build_index(volumes, n_volumes, daughters, std::max(wh[0], std::max(wh[1], wh[2])), nav->grid, nav->idx);
     // nodes, grids and index in one allocation (the grids start 64 B-aligned after the nodes)
-    std::vector<char> blob(nav->bytes());
+    std::vector<char> blob(nav->bytes_padded(), 0);
     char *q = blob.data();
     std::memcpy(q, nav->node.data(), nav->node.size() * sizeof(NavNode));
     q += nav->node.size() * sizeof(NavNode);
@@ -321,13 +324,14 @@ int gxt_nav_info(const gxt_navigator *nav, int *n_touchables, int32_t *volume, i
 }
 
 int gxt_nav_locate(const gxt_navigator *nav, const double *pos, const double *dir, int32_t *touchable, int n) {
-  if (!nav || !nav->dev || !pos || !dir || !touchable || n <= 0) return set_error(GXT_EINVAL, "bad locate arguments");
+  if (!nav || !nav->dev || !pos || !dir || !touchable || n <= 0 || n > kMaxSoaPoints)
+    return set_error(GXT_EINVAL, "bad locate arguments (1 <= n <= 2^29)");
   if (const int rc = check_device(nav)) return rc;
   const void *arrays[3] = {pos, dir, touchable};
   const int res = residence(arrays, 3, nav->device);
   if (res < 0) return set_error(GXT_EINVAL, "locate arrays must be all host or all device memory");
   if (res == 1) {
-    const hipError_t e = launch_nav_locate(nav->view(), pos, dir, touchable, n, 0);
+    const hipError_t e = launch_nav_locate(nav->view(), nav->lds_bytes(), pos, dir, touchable, n, 0);
     return e == hipSuccess ? GXT_OK : hip_fail(e, "gxt_nav_locate");
   }
   if (const int rc = check_points(pos, dir, n)) return rc;
@@ -339,14 +343,14 @@ int gxt_nav_locate(const gxt_navigator *nav, const double *pos, const double *di
   if (e == hipSuccess) e = dt.alloc(n);
   if (e == hipSuccess) e = hipMemcpy(dp.p, pos, n3, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dd.p, dir, n3, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_nav_locate(nav->view(), dp.p, dd.p, dt.p, n, 0);
+  if (e == hipSuccess) e = launch_nav_locate(nav->view(), nav->lds_bytes(), dp.p, dd.p, dt.p, n, 0);
   if (e == hipSuccess) e = hipMemcpy(touchable, dt.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost);
   return e == hipSuccess ? GXT_OK : hip_fail(e, "gxt_nav_locate");
 }
 
 int gxt_nav_step(const gxt_navigator *nav, const int32_t *touchable, const double *pos, const double *dir,
                  double *snext, int32_t *next, double *safety, int n) {
-  if (!nav || !nav->dev || !touchable || !pos || !dir || !snext || !next || !safety || n <= 0)
+  if (!nav || !nav->dev || !touchable || !pos || !dir || !snext || !next || !safety || n <= 0 || n > kMaxSoaPoints)
     return set_error(GXT_EINVAL, "bad step arguments");
   if (const int rc = check_device(nav)) return rc;
   const int nt = (int)nav->node.size();
@@ -354,7 +358,7 @@ int gxt_nav_step(const gxt_navigator *nav, const int32_t *touchable, const doubl
   const int res = residence(arrays, 6, nav->device);
   if (res < 0) return set_error(GXT_EINVAL, "step arrays must be all host or all device memory");
   if (res == 1) {
-    const hipError_t e = launch_nav_step(nav->view(), touchable, pos, dir, snext, next, safety, n, 0);
+    const hipError_t e = launch_nav_step(nav->view(), nav->lds_bytes(), touchable, pos, dir, snext, next, safety, n, 0);
     return e == hipSuccess ? GXT_OK : hip_fail(e, "gxt_nav_step");
   }
   for (int i = 0; i < n; ++i)
@@ -373,7 +377,7 @@ int gxt_nav_step(const gxt_navigator *nav, const int32_t *touchable, const doubl
   if (e == hipSuccess) e = hipMemcpy(dp.p, pos, n3, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dd.p, dir, n3, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dt.p, touchable, ni, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_nav_step(nav->view(), dt.p, dp.p, dd.p, ds.p, dn.p, dsaf.p, n, 0);
+  if (e == hipSuccess) e = launch_nav_step(nav->view(), nav->lds_bytes(), dt.p, dp.p, dd.p, ds.p, dn.p, dsaf.p, n, 0);
   if (e == hipSuccess) e = hipMemcpy(snext, ds.p, n1, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(next, dn.p, ni, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(safety, dsaf.p, n1, hipMemcpyDeviceToHost);
@@ -385,7 +389,8 @@ int gxt_nav_step(const gxt_navigator *nav, const int32_t *touchable, const doubl
 extern "C" int gxt_nav_find_next_boundary(const gxt_navigator *nav, const int32_t *touchable, const double *pos,
                                           const double *dir, const double *pstep, double *snext, int32_t *next,
                                           double *safety, int32_t *boundary, int n) {
-  if (!nav || !nav->dev || !touchable || !pos || !dir || !pstep || !snext || !next || !safety || !boundary || n <= 0)
+  if (!nav || !nav->dev || !touchable || !pos || !dir || !pstep || !snext || !next || !safety || !boundary || n <= 0 ||
+      n > kMaxSoaPoints)
     return set_error(GXT_EINVAL, "bad find-next-boundary arguments");
   if (const int rc = check_device(nav)) return rc;
   const int nt = (int)nav->node.size();
@@ -394,7 +399,7 @@ extern "C" int gxt_nav_find_next_boundary(const gxt_navigator *nav, const int32_
   if (res < 0) return set_error(GXT_EINVAL, "find-next-boundary arrays must be all host or all device memory");
   if (res == 1) {
     const hipError_t e =
-        launch_nav_find_next_boundary(nav->view(), touchable, pos, dir, pstep, snext, next, safety, boundary, n, 0);
+        launch_nav_find_next_boundary(nav->view(), nav->lds_bytes(), touchable, pos, dir, pstep, snext, next, safety, boundary, n, 0);
     return e == hipSuccess ? GXT_OK : hip_fail(e, "gxt_nav_find_next_boundary");
   }
   for (int i = 0; i < n; ++i) {
@@ -420,7 +425,7 @@ extern "C" int gxt_nav_find_next_boundary(const gxt_navigator *nav, const int32_
   if (e == hipSuccess) e = hipMemcpy(dps.p, pstep, n1, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dt.p, touchable, ni, hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = launch_nav_find_next_boundary(nav->view(), dt.p, dp.p, dd.p, dps.p, ds.p, dn.p, dsaf.p, db.p, n, 0);
+    e = launch_nav_find_next_boundary(nav->view(), nav->lds_bytes(), dt.p, dp.p, dd.p, dps.p, ds.p, dn.p, dsaf.p, db.p, n, 0);
   if (e == hipSuccess) e = hipMemcpy(snext, ds.p, n1, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(next, dn.p, ni, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(safety, dsaf.p, n1, hipMemcpyDeviceToHost);

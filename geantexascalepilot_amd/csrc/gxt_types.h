@@ -128,6 +128,13 @@ __host__ __device__ inline int nav_bin(double x, double h, double inv, int nb) {
   return (int)u;
 }
 
+// Most points per call of the entry points that take [3n] structure-of-arrays
+// (their kernels index them with 32-bit k n + i).
+constexpr int kMaxSoaPoints = 1 << 29;
+
+// Largest tree (nodes, grids and index) the navigation kernels copy into LDS.
+constexpr int kNavLdsMax = 48 * 1024;
+
 // A volume tree on the device.
 struct DevNav {
   const NavNode *node;

@@ -11,6 +11,8 @@
  *
  * Units are CLHEP: mm, MeV, ns; magnetic field in tesla.
  * Species codes: 0 = gamma, 1 = electron, 2 = positron.
+ * Entry points that take n points as [3n] structure-of-arrays (x[0..n),
+ * y[0..n), z[0..n)) accept 1 <= n <= 2^29 per call.
  */
 #ifndef GXT_H
 #define GXT_H
