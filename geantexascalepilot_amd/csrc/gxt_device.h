@@ -33,9 +33,66 @@ __device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float m_sqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double m_pow(double x, double y) { return pow(x, y); }
 __device__ __forceinline__ float m_pow(float x, float y) { return powf(x, y); }
-__device__ __forceinline__ double m_sin(double x) { return sin(x); }
+// FP64 sine and cosine, computed the same way by the CPU oracle so that both
+// give the same bits (a helix turns by sin/cos every step, and a last-ulp
+// difference between two libms grows into a different shower).  Cody and
+// Waite's additive reduction x = k pi/2 + r, with k = rint(x 2/pi) and pi/2
+// taken off in three parts (the first two hold 33 bits, so k P1 and k P2 are
+// exact for |k| < 2^20); then the Taylor series of sin to r^15 and of cos to
+// r^16 on |r| <= pi/4 (truncation below 1e-16 relative), and the quadrant
+// k mod 4.  Every operation is a correctly rounded IEEE one.
+constexpr double kTwoOverPi = 0x1.45f306dc9c883p-1;  // 2/pi
+constexpr double kPio2P1 = 0x1.921fb544p+0;            // pi/2 to 33 bits
+constexpr double kPio2P2 = 0x1.0b4611a6p-34;           // the next 33 bits
+constexpr double kPio2P3 = 0x1.3198a2e037073p-69;      // the rest, to 53 bits
+__device__ __forceinline__ void sincos_cw(double x, double &s, double &c) {
+  if (!(fabs(x) <= 1e15)) {
+    s = x - x;
+    c = x - x;
+    return;
+  }
+  const double k = rint(x * kTwoOverPi);
+  const double r = ((x - k * kPio2P1) - k * kPio2P2) - k * kPio2P3;
+  const double z = r * r;
+  const double ps =
+      -1.0 / 6.0 +
+      z * (1.0 / 120.0 +
+           z * (-1.0 / 5040.0 +
+                z * (1.0 / 362880.0 + z * (-1.0 / 39916800.0 + z * (1.0 / 6227020800.0 + z * (-1.0 / 1307674368000.0))))));
+  const double pc =
+      1.0 / 24.0 +
+      z * (-1.0 / 720.0 +
+           z * (1.0 / 40320.0 +
+                z * (-1.0 / 3628800.0 + z * (1.0 / 479001600.0 + z * (-1.0 / 87178291200.0 + z * (1.0 / 20922789888000.0))))));
+  const double sr = r + r * z * ps;
+  const double cr = (1.0 - 0.5 * z) + z * z * pc;
+  double q = fmod(k, 4.0);
+  if (q < 0.0) q += 4.0;
+  if (q == 0.0) {
+    s = sr;
+    c = cr;
+  } else if (q == 1.0) {
+    s = cr;
+    c = -sr;
+  } else if (q == 2.0) {
+    s = -sr;
+    c = -cr;
+  } else {
+    s = -cr;
+    c = sr;
+  }
+}
+__device__ __forceinline__ double m_sin(double x) {
+  double s, c;
+  sincos_cw(x, s, c);
+  return s;
+}
 __device__ __forceinline__ float m_sin(float x) { return sinf(x); }
-__device__ __forceinline__ double m_cos(double x) { return cos(x); }
+__device__ __forceinline__ double m_cos(double x) {
+  double s, c;
+  sincos_cw(x, s, c);
+  return c;
+}
 __device__ __forceinline__ float m_cos(float x) { return cosf(x); }
 __device__ __forceinline__ double m_fabs(double x) { return fabs(x); }
 __device__ __forceinline__ float m_fabs(float x) { return fabsf(x); }
@@ -1263,7 +1320,7 @@ __device__ __forceinline__ void msc_deflect(double tau, R dir[3], Rng &rng) {
   const double cost = 1.0 - x;
   const double sint = sqrt(fmax(x * (2.0 - x), 0.0));
   const double phi = kTwoPi * r2;
-  R uu = (R)(sint * cos(phi)), v = (R)(sint * sin(phi)), w = (R)cost;
+  R uu = (R)(sint * m_cos(phi)), v = (R)(sint * m_sin(phi)), w = (R)cost;
   rotate_to_lab<R>(uu, v, w, dir[0], dir[1], dir[2]);
   dir[0] = uu;
   dir[1] = v;

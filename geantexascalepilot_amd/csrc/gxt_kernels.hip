@@ -397,7 +397,7 @@ __device__ __forceinline__ void emit_at_rest(const StepArgs &a, const Pool<R> &n
   const double cost = 2.0 * t.rng.uniform() - 1.0;
   const double sint = sqrt((1.0 - cost) * (1.0 + cost));
   const double phi = kTwoPi * t.rng.uniform();
-  const R d0 = (R)(sint * cos(phi)), d1 = (R)(sint * sin(phi)), d2 = (R)cost;
+  const R d0 = (R)(sint * m_cos(phi)), d1 = (R)(sint * m_sin(phi)), d2 = (R)cost;
   const uint64_t u1 = child_substream(t.rng);
   emit_new(a, nxt, idx, nk, t, 0, (R)kMe, d0, d1, d2, u1, wh, trace);
   const uint64_t u2 = child_substream(t.rng);

@@ -606,6 +606,66 @@ static double safety(const Geometry &g, int v, const double p[3]) {
   return s < 0 ? 0.0 : s;
 }
 
+// FP64 sine and cosine with the same operations as the device build, so
+// that both give the same bits: Cody and Waite's additive reduction
+// x = k pi/2 + r (k = rint(x 2/pi); pi/2 in three parts, the first two of 33
+// bits so that k P1 and k P2 are exact for |k| < 2^20), the Taylor series of
+// sin to r^15 and of cos to r^16 on |r| <= pi/4, and the quadrant k mod 4.
+// The split of pi/2 and 2/pi are the correctly rounded values (checked
+// against a 60-digit pi in the tests).
+constexpr double kTwoOverPi = 0x1.45f306dc9c883p-1;
+constexpr double kPio2P1 = 0x1.921fb544p+0;
+constexpr double kPio2P2 = 0x1.0b4611a6p-34;
+constexpr double kPio2P3 = 0x1.3198a2e037073p-69;
+static inline void sincos_cw(double x, double &s, double &c) {
+  if (!(std::fabs(x) <= 1e15)) {
+    s = x - x;
+    c = x - x;
+    return;
+  }
+  const double k = std::rint(x * kTwoOverPi);
+  const double r = ((x - k * kPio2P1) - k * kPio2P2) - k * kPio2P3;
+  const double z = r * r;
+  // 1/n! as exact quotients of exact integers
+  const double ps =
+      -1.0 / 6.0 +
+      z * (1.0 / 120.0 +
+           z * (-1.0 / 5040.0 +
+                z * (1.0 / 362880.0 + z * (-1.0 / 39916800.0 + z * (1.0 / 6227020800.0 + z * (-1.0 / 1307674368000.0))))));
+  const double pc =
+      1.0 / 24.0 +
+      z * (-1.0 / 720.0 +
+           z * (1.0 / 40320.0 +
+                z * (-1.0 / 3628800.0 + z * (1.0 / 479001600.0 + z * (-1.0 / 87178291200.0 + z * (1.0 / 20922789888000.0))))));
+  const double sr = r + r * z * ps;
+  const double cr = (1.0 - 0.5 * z) + z * z * pc;
+  double q = std::fmod(k, 4.0);
+  if (q < 0.0) q += 4.0;
+  if (q == 0.0) {
+    s = sr;
+    c = cr;
+  } else if (q == 1.0) {
+    s = cr;
+    c = -sr;
+  } else if (q == 2.0) {
+    s = -sr;
+    c = -cr;
+  } else {
+    s = -cr;
+    c = sr;
+  }
+}
+static inline double sin_cw(double x) {
+  double s, c;
+  sincos_cw(x, s, c);
+  return s;
+}
+static inline double cos_cw(double x) {
+  double s, c;
+  sincos_cw(x, s, c);
+  return c;
+}
+
 // Math::RotateToLabFrame (core/math_wrappers.hpp:194-212).
 static inline void rotate_to_lab(double &u, double &v, double &w, double u1, double u2, double u3) {
   double up = u1 * u1 + u2 * u2;
@@ -647,7 +707,7 @@ static void helix_step(const double B[3], double bmag, double charge, double mom
   const double cvb[3] = {vx[1] * ub[2] - vx[2] * ub[1], vx[2] * ub[0] - vx[0] * ub[2],
                          vx[0] * ub[1] - vx[1] * ub[0]};
   const double phi = -step * charge * bmag * kB2C / mom;
-  const double sphi = std::sin(phi), cphi = std::cos(phi);
+  const double sphi = sin_cw(phi), cphi = cos_cw(phi);
   for (int i = 0; i < 3; ++i) {
     pos[i] = pos[i] + R * (cphi - 1.0) * cvb[i] - R * sphi * vx[i] + step * uvb * ub[i];
     dir[i] = uvb * ub[i] + cphi * sinvb * vx[i] + sphi * sinvb * cvb[i];
@@ -958,7 +1018,7 @@ static void sample_compton(const Material &, double e0, const double d0[3], uint
   const double sinT = sint2 < 0.0 ? 0.0 : std::sqrt(sint2);
   const double phi = kTwoPi * mrg_uniform(s);
   const double e1 = eps * e0;
-  double gd[3] = {sinT * std::cos(phi), sinT * std::sin(phi), 1.0 - onecost};
+  double gd[3] = {sinT * cos_cw(phi), sinT * sin_cw(phi), 1.0 - onecost};
   rotate_to_lab(gd[0], gd[1], gd[2], d0[0], d0[1], d0[2]);
   const double te = e0 - e1;
   if (e1 > lowE) {
@@ -1047,10 +1107,10 @@ static void sample_pair(const Material &m, double e0, const double d0[3], uint32
   const double aa0 = -std::log(r1 * r2);
   const double u = (0.25 > mrg_uniform(s)) ? aa0 / 0.625 : aa0 / 1.875;
   const double phi = kTwoPi * mrg_uniform(s);
-  const double cosp = std::cos(phi), sinp = std::sin(phi);
+  const double cosp = cos_cw(phi), sinp = sin_cw(phi);
   const double the = u * kMe / eTot, thp = u * kMe / pTot;
-  const double ste = std::sin(the), cte = std::cos(the);
-  const double stp = std::sin(thp), ctp = std::cos(thp);
+  const double ste = sin_cw(the), cte = cos_cw(the);
+  const double stp = sin_cw(thp), ctp = cos_cw(thp);
   Secondary &el = o.sec[0];
   el.species = 1;
   el.ekin = eTot - kMe;
@@ -1113,8 +1173,8 @@ static void sample_photo(const Material &m, double e0, const double d0[3], uint3
   Secondary &e = o.sec[0];
   e.species = 1;
   e.ekin = te;
-  e.dir[0] = sint * std::cos(phi);
-  e.dir[1] = sint * std::sin(phi);
+  e.dir[0] = sint * cos_cw(phi);
+  e.dir[1] = sint * sin_cw(phi);
   e.dir[2] = cost;
   rotate_to_lab(e.dir[0], e.dir[1], e.dir[2], d0[0], d0[1], d0[2]);
   o.nsec = 1;
@@ -1175,8 +1235,8 @@ static void sample_brems(const Material &m, double t0, const double d0[3], uint3
   Secondary &g = o.sec[0];
   g.species = 0;
   g.ekin = k;
-  g.dir[0] = sint * std::cos(phi);
-  g.dir[1] = sint * std::sin(phi);
+  g.dir[0] = sint * cos_cw(phi);
+  g.dir[1] = sint * sin_cw(phi);
   g.dir[2] = cost;
   rotate_to_lab(g.dir[0], g.dir[1], g.dir[2], d0[0], d0[1], d0[2]);
   o.nsec = 1;
@@ -1247,8 +1307,8 @@ static void sample_ioni(const Material &m, bool positron, double t0, const doubl
   Secondary &d = o.sec[0];
   d.species = 1;
   d.ekin = dT;
-  d.dir[0] = sint * std::cos(phi);
-  d.dir[1] = sint * std::sin(phi);
+  d.dir[0] = sint * cos_cw(phi);
+  d.dir[1] = sint * sin_cw(phi);
   d.dir[2] = cost;
   rotate_to_lab(d.dir[0], d.dir[1], d.dir[2], d0[0], d0[1], d0[2]);
   o.nsec = 1;
@@ -1290,8 +1350,8 @@ static void sample_annihil(const Material &, double t0, const double d0[3], uint
   Secondary &g1 = o.sec[0];
   g1.species = 0;
   g1.ekin = eps * etot;
-  g1.dir[0] = sint * std::cos(phi);
-  g1.dir[1] = sint * std::sin(phi);
+  g1.dir[0] = sint * cos_cw(phi);
+  g1.dir[1] = sint * sin_cw(phi);
   g1.dir[2] = cost;
   rotate_to_lab(g1.dir[0], g1.dir[1], g1.dir[2], d0[0], d0[1], d0[2]);
   Secondary &g2 = o.sec[1];
@@ -1416,7 +1476,7 @@ static void msc_deflect(double tau, double dir[3], uint32_t rng[6]) {
   const double cost = 1.0 - x;
   const double sint = std::sqrt(std::max(x * (2.0 - x), 0.0));
   const double phi = kTwoPi * r2;
-  double uu = sint * std::cos(phi), v = sint * std::sin(phi), w = cost;
+  double uu = sint * cos_cw(phi), v = sint * sin_cw(phi), w = cost;
   rotate_to_lab(uu, v, w, dir[0], dir[1], dir[2]);
   dir[0] = uu;
   dir[1] = v;
@@ -1514,7 +1574,7 @@ static void annihilate_at_rest(const gxref_ctx &c, Worker &w, EventTally &ev, st
   const double cost = 2.0 * mrg_uniform(s) - 1.0;
   const double sint = std::sqrt((1.0 - cost) * (1.0 + cost));
   const double phi = kTwoPi * mrg_uniform(s);
-  const double d1[3] = {sint * std::cos(phi), sint * std::sin(phi), cost};
+  const double d1[3] = {sint * cos_cw(phi), sint * sin_cw(phi), cost};
   const double d2[3] = {-d1[0], -d1[1], -d1[2]};
   ev.nannihil++;
   push_secondary(c, w, ev, stack, t, s, 0, kMe, d1);
@@ -2613,5 +2673,11 @@ extern "C" int gxref_nav_find_next_boundary(const gxref_nav *g, const int32_t *c
     boundary[i] = s < ps;
     snext[i] = std::max(2.0 * tol, s + 2.0 * tol);
   }
+  return GXT_OK;
+}
+
+extern "C" int gxref_sincos(const double *x, double *s, double *c, int n) {
+  if (!x || !s || !c || n <= 0) return gxref::fail(GXT_EINVAL, "bad sincos arguments");
+  for (int i = 0; i < n; ++i) gxref::sincos_cw(x[i], s[i], c[i]);
   return GXT_OK;
 }

@@ -7,15 +7,16 @@ does not depend on size: every event balances energy (deposit + leakage +
 counter equals the sum over events; no track hits the step limit.  Against
 the oracle, run alone on the first 256 primaries of the batch:
   - with RK45 (in the map) at least 98 % of the events are equal in every
-    count, and those agree in every energy to 1e-7.  Its propagation is the
-    same correctly rounded FP64 operations on both sides; the samplers'
-    sin/cos/log still differ in the last ulp between the device's and the
-    host's libm, which a shower of 10^5 steps occasionally amplifies into a
-    different decision, and otherwise into step lengths that drift by more
-    than 1e-9;
-  - with the exact helix (with or without multiple scattering) most of these
-    high-energy showers in 4 T diverge that way (about 60 % of them), since
-    the helix itself calls sin/cos every step;
+    count, and those agree in every energy to 1e-7.  Propagation is the
+    same correctly rounded FP64 operations on both sides (sin/cos included,
+    the shared sincos_cw), but the samplers' log/exp/pow come from the
+    device's and the host's libm and differ in the last ulp, which a shower
+    of 10^5 steps occasionally amplifies into a different decision, and
+    otherwise into step lengths that drift by more than 1e-9;
+  - in the uniform 4 T field across the slabs, low-energy e-/e+ curl through
+    the thin layers, and most of these high-energy showers (about 60 %)
+    diverge that way, with the helix or RK45, with or without multiple
+    scattering;
   - for every form the per-event deposit, visible deposit, charged length
     and step count, each divided by the primary's energy, agree as
     distributions (two-sample KS, p > 0.01).

@@ -140,8 +140,8 @@ def test_helix_fp64_and_fp32(gpu_lib):
     d /= np.linalg.norm(d, axis=0)
     cp, cd = O.helix_step(field, charge, mom, step, pos, d)
     gp, gd = gpu_lib.helix_step(field, charge, mom, step, pos, d, abi.FP64)
-    np.testing.assert_allclose(gp, cp, rtol=1e-12, atol=1e-10)
-    np.testing.assert_allclose(gd, cd, rtol=1e-12, atol=1e-12)
+    # FP64: the same correctly rounded operations and the same sin/cos on both sides
+    assert np.array_equal(gp, cp) and np.array_equal(gd, cd)
     fp, fd = gpu_lib.helix_step(field, charge, mom, step, pos, d, abi.FP32)
     np.testing.assert_allclose(fp, cp, rtol=0, atol=1e-4 * (1 + step.max()))
     np.testing.assert_allclose(fd, cd, rtol=0, atol=2e-4)
