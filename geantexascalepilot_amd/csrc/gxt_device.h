@@ -25,13 +25,102 @@ constexpr double kAvogadro = 6.02214179e+23;
 constexpr double kClassicRe = 2.8179402894e-12;
 
 // ------------------------------------------------------------------ math
-__device__ __forceinline__ double m_log(double x) { return log(x); }
+// FP64 log and exp, and what is built on them, computed the same way by the
+// CPU oracle so that both give the same bits (samplers, table bins and
+// interaction lengths call them every step; a last-ulp difference between
+// two libms grows into a different shower in a strong field).
+//  - log: x = 2^e m with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh f,
+//    f = (m - 1)/(m + 1), by its series to f^21 (|f| <= 0.172, truncation
+//    below 1e-18), plus e ln 2 with ln 2 in two parts (the first of 29 bits,
+//    so e ln2_hi is exact);
+//  - exp: Cody and Waite's reduction x = k ln 2 + r, |r| <= ln2/2, the
+//    Taylor series of e^r to r^13, scaled by 2^k (ldexp, exact);
+//  - log1p: log(u) - ((u - 1) - x)/u with u = 1 + x (Kahan's correction);
+//  - expm1: the Taylor series to x^14 for |x| < ln2/2, else exp(x) - 1;
+//  - pow(x, y) = exp(y log x) (x > 0), log10 x = log x / ln 10.
+// Every operation is a correctly rounded IEEE one.
+constexpr double kLn2Hi = 0x1.62e42ffp-1;
+constexpr double kLn2Lo = -0x1.718432a1b0e26p-35;
+constexpr double kInvLn2 = 0x1.71547652b82fep+0;
+constexpr double kInvLn10 = 0x1.bcb7b1526e50ep-2;
+constexpr double kSqrtHalf = 0x1.6a09e667f3bcdp-1;
+__device__ __forceinline__ double log_cw(double x) {
+  if (!(x > 0.0)) return x == 0.0 ? -HUGE_VAL : (x - x) / (x - x);
+  if (x == HUGE_VAL) return x;
+  int e;
+  double m = frexp(x, &e);
+  if (m < kSqrtHalf) {
+    m *= 2.0;
+    e -= 1;
+  }
+  const double f = (m - 1.0) / (m + 1.0);
+  const double w = f * f;
+  const double t =
+      w * (2.0 / 3.0 +
+           w * (2.0 / 5.0 +
+                w * (2.0 / 7.0 +
+                     w * (2.0 / 9.0 +
+                          w * (2.0 / 11.0 +
+                               w * (2.0 / 13.0 + w * (2.0 / 15.0 + w * (2.0 / 17.0 + w * (2.0 / 19.0 + w * (2.0 / 21.0))))))))));
+  const double lm = 2.0 * f + f * t;
+  const double de = (double)e;
+  return de * kLn2Hi + (de * kLn2Lo + lm);
+}
+__device__ __forceinline__ double exp_cw(double x) {
+  if (x != x) return x;
+  if (x > 709.782712893384) return HUGE_VAL;
+  if (x < -745.2) return 0.0;
+  const double k = rint(x * kInvLn2);
+  const double r = (x - k * kLn2Hi) - k * kLn2Lo;
+  const double p =
+      1.0 +
+      r * (1.0 +
+           r * (1.0 / 2.0 +
+                r * (1.0 / 6.0 +
+                     r * (1.0 / 24.0 +
+                          r * (1.0 / 120.0 +
+                               r * (1.0 / 720.0 +
+                                    r * (1.0 / 5040.0 +
+                                         r * (1.0 / 40320.0 +
+                                              r * (1.0 / 362880.0 +
+                                                   r * (1.0 / 3628800.0 +
+                                                        r * (1.0 / 39916800.0 +
+                                                             r * (1.0 / 479001600.0 + r * (1.0 / 6227020800.0)))))))))))));
+  return ldexp(p, (int)k);
+}
+__device__ __forceinline__ double log1p_cw(double x) {
+  const double u = 1.0 + x;
+  if (u == 1.0) return x;
+  if (!(u > 0.0) || u == HUGE_VAL) return log_cw(u);
+  return log_cw(u) - ((u - 1.0) - x) / u;
+}
+__device__ __forceinline__ double expm1_cw(double x) {
+  if (!(fabs(x) < 0.34657359027997264)) return exp_cw(x) - 1.0;
+  return x * (1.0 +
+              x * (1.0 / 2.0 +
+                   x * (1.0 / 6.0 +
+                        x * (1.0 / 24.0 +
+                             x * (1.0 / 120.0 +
+                                  x * (1.0 / 720.0 +
+                                       x * (1.0 / 5040.0 +
+                                            x * (1.0 / 40320.0 +
+                                                 x * (1.0 / 362880.0 +
+                                                      x * (1.0 / 3628800.0 +
+                                                           x * (1.0 / 39916800.0 +
+                                                                x * (1.0 / 479001600.0 +
+                                                                     x * (1.0 / 6227020800.0 +
+                                                                          x * (1.0 / 87178291200.0))))))))))))));
+}
+__device__ __forceinline__ double pow_cw(double x, double y) { return exp_cw(y * log_cw(x)); }
+__device__ __forceinline__ double log10_cw(double x) { return log_cw(x) * kInvLn10; }
+
+__device__ __forceinline__ double m_log(double x) { return log_cw(x); }
 __device__ __forceinline__ float m_log(float x) { return logf(x); }
-__device__ __forceinline__ double m_exp(double x) { return exp(x); }
+__device__ __forceinline__ double m_exp(double x) { return exp_cw(x); }
 __device__ __forceinline__ float m_exp(float x) { return expf(x); }
 __device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float m_sqrt(float x) { return sqrtf(x); }
-__device__ __forceinline__ double m_pow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ double m_pow(double x, double y) { return pow_cw(x, y); }
 __device__ __forceinline__ float m_pow(float x, float y) { return powf(x, y); }
 // FP64 sine and cosine, computed the same way by the CPU oracle so that both
 // give the same bits (a helix turns by sin/cos every step, and a last-ulp
@@ -96,7 +185,7 @@ __device__ __forceinline__ double m_cos(double x) {
 __device__ __forceinline__ float m_cos(float x) { return cosf(x); }
 __device__ __forceinline__ double m_fabs(double x) { return fabs(x); }
 __device__ __forceinline__ float m_fabs(float x) { return fabsf(x); }
-__device__ __forceinline__ double m_log10(double x) { return log10(x); }
+__device__ __forceinline__ double m_log10(double x) { return log10_cw(x); }
 
 template <class R> __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
 template <class R> __device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
@@ -1263,13 +1352,13 @@ __device__ __forceinline__ double msc_lambda1(const Nodes<R> &N, const DevMateri
 }
 
 // (expm1 and log1p keep full precision for short steps)
-__device__ __forceinline__ double msc_geom(double t, double lam) { return -lam * expm1(-t / lam); }
+__device__ __forceinline__ double msc_geom(double t, double lam) { return -lam * expm1_cw(-t / lam); }
 
 // true path of a geometric path z, at most tmax
 __device__ __forceinline__ double msc_true(double z, double lam, double tmax) {
   const double x = z / lam;
   if (x >= 1.0) return tmax;
-  return fmin(-lam * log1p(-x), tmax);
+  return fmin(-lam * log1p_cw(-x), tmax);
 }
 
 // Screening of the deflection x = 1 - cos(theta) ~ 1/(x + a)^2 on [0, 2],
@@ -1282,16 +1371,16 @@ __device__ __forceinline__ double msc_true(double z, double lam, double tmax) {
 // ((j + 1)(j + 2)) (from the series of ln(1 + u)) is solved for u < 0.17 by
 // twelve Newton steps from u = 3q with 20 terms.
 __device__ __attribute__((noinline)) double msc_u(double tau) {
-  const double q = exp(-tau);
+  const double q = exp_cw(-tau);
   if (q >= 0.05) {
-    const double mean = -expm1(-tau);
-    double a = mean < 0.5 ? mean / fmax(log(2.0 / mean) - 1.0, 0.5) : 2.0 / (3.0 * q);
+    const double mean = -expm1_cw(-tau);
+    double a = mean < 0.5 ? mean / fmax(log_cw(2.0 / mean) - 1.0, 0.5) : 2.0 / (3.0 * q);
     for (int k = 0; k < 12; ++k) {
-      const double L = log1p(2.0 / a);
+      const double L = log1p_cw(2.0 / a);
       const double f = 0.5 * a * (a + 2.0) * L - a - mean;
       const double df = a * ((a + 1.0) * L - 2.0);
       const double du = fmin(fmax(-f / df, -2.0), 2.0);
-      a *= exp(du);
+      a *= exp_cw(du);
     }
     return 2.0 / a;
   }

@@ -47,6 +47,95 @@ static int fail(int code, const std::string &msg) {
   return code;
 }
 
+// FP64 log and exp, and what is built on them, computed the same way by the
+// CPU oracle so that both give the same bits (samplers, table bins and
+// interaction lengths call them every step; a last-ulp difference between
+// two libms grows into a different shower in a strong field).
+//  - log: x = 2^e m with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh f,
+//    f = (m - 1)/(m + 1), by its series to f^21 (|f| <= 0.172, truncation
+//    below 1e-18), plus e ln 2 with ln 2 in two parts (the first of 29 bits,
+//    so e ln2_hi is exact);
+//  - exp: Cody and Waite's reduction x = k ln 2 + r, |r| <= ln2/2, the
+//    Taylor series of e^r to r^13, scaled by 2^k (ldexp, exact);
+//  - log1p: log(u) - ((u - 1) - x)/u with u = 1 + x (Kahan's correction);
+//  - expm1: the Taylor series to x^14 for |x| < ln2/2, else exp(x) - 1;
+//  - pow(x, y) = exp(y log x) (x > 0), log10 x = log x / ln 10.
+// Every operation is a correctly rounded IEEE one.
+constexpr double kLn2Hi = 0x1.62e42ffp-1;
+constexpr double kLn2Lo = -0x1.718432a1b0e26p-35;
+constexpr double kInvLn2 = 0x1.71547652b82fep+0;
+constexpr double kInvLn10 = 0x1.bcb7b1526e50ep-2;
+constexpr double kSqrtHalf = 0x1.6a09e667f3bcdp-1;
+static inline double log_cw(double x) {
+  if (!(x > 0.0)) return x == 0.0 ? -HUGE_VAL : (x - x) / (x - x);
+  if (x == HUGE_VAL) return x;
+  int e;
+  double m = std::frexp(x, &e);
+  if (m < kSqrtHalf) {
+    m *= 2.0;
+    e -= 1;
+  }
+  const double f = (m - 1.0) / (m + 1.0);
+  const double w = f * f;
+  const double t =
+      w * (2.0 / 3.0 +
+           w * (2.0 / 5.0 +
+                w * (2.0 / 7.0 +
+                     w * (2.0 / 9.0 +
+                          w * (2.0 / 11.0 +
+                               w * (2.0 / 13.0 + w * (2.0 / 15.0 + w * (2.0 / 17.0 + w * (2.0 / 19.0 + w * (2.0 / 21.0))))))))));
+  const double lm = 2.0 * f + f * t;
+  const double de = (double)e;
+  return de * kLn2Hi + (de * kLn2Lo + lm);
+}
+static inline double exp_cw(double x) {
+  if (x != x) return x;
+  if (x > 709.782712893384) return HUGE_VAL;
+  if (x < -745.2) return 0.0;
+  const double k = std::rint(x * kInvLn2);
+  const double r = (x - k * kLn2Hi) - k * kLn2Lo;
+  const double p =
+      1.0 +
+      r * (1.0 +
+           r * (1.0 / 2.0 +
+                r * (1.0 / 6.0 +
+                     r * (1.0 / 24.0 +
+                          r * (1.0 / 120.0 +
+                               r * (1.0 / 720.0 +
+                                    r * (1.0 / 5040.0 +
+                                         r * (1.0 / 40320.0 +
+                                              r * (1.0 / 362880.0 +
+                                                   r * (1.0 / 3628800.0 +
+                                                        r * (1.0 / 39916800.0 +
+                                                             r * (1.0 / 479001600.0 + r * (1.0 / 6227020800.0)))))))))))));
+  return std::ldexp(p, (int)k);
+}
+static inline double log1p_cw(double x) {
+  const double u = 1.0 + x;
+  if (u == 1.0) return x;
+  if (!(u > 0.0) || u == HUGE_VAL) return log_cw(u);
+  return log_cw(u) - ((u - 1.0) - x) / u;
+}
+static inline double expm1_cw(double x) {
+  if (!(std::fabs(x) < 0.34657359027997264)) return exp_cw(x) - 1.0;
+  return x * (1.0 +
+              x * (1.0 / 2.0 +
+                   x * (1.0 / 6.0 +
+                        x * (1.0 / 24.0 +
+                             x * (1.0 / 120.0 +
+                                  x * (1.0 / 720.0 +
+                                       x * (1.0 / 5040.0 +
+                                            x * (1.0 / 40320.0 +
+                                                 x * (1.0 / 362880.0 +
+                                                      x * (1.0 / 3628800.0 +
+                                                           x * (1.0 / 39916800.0 +
+                                                                x * (1.0 / 479001600.0 +
+                                                                     x * (1.0 / 6227020800.0 +
+                                                                          x * (1.0 / 87178291200.0))))))))))))));
+}
+static inline double pow_cw(double x, double y) { return exp_cw(y * log_cw(x)); }
+static inline double log10_cw(double x) { return log_cw(x) * kInvLn10; }
+
 // ---------------------------------------------------------------- MRG32k3a
 // L'Ecuyer's combined MRG (random/MRG.hpp:35-45, random/src/MRG32k3a.cpp:123-163).
 // The reference carries the residues in FP64 and reduces with floor(p/m)
@@ -226,7 +315,7 @@ static double pvec_value(const PVec &p, double e) {
   if (N == 0) return 0.0;
   if (e <= p.x[0]) return p.y[0];
   if (e >= p.x[N - 1]) return p.y[N - 1];
-  int bin = (int)(std::log(e) / p.dl - p.base);
+  int bin = (int)(log_cw(e) / p.dl - p.base);
   if (bin > N - 2) bin = N - 2;
   if (bin < 0) bin = 0;
   if (bin > 0 && e < p.x[bin]) --bin;
@@ -508,7 +597,7 @@ static double heitler_xs(double ekin) {
   const double bg2 = tau * (tau + 2.0);
   const double bg = std::sqrt(bg2);
   return kPi * kClassicRe * kClassicRe *
-         ((gamma2 + 4.0 * gam + 1.0) * std::log(gam + bg) - (gam + 3.0) * bg) / (bg2 * (gam + 1.0));
+         ((gamma2 + 4.0 * gam + 1.0) * log_cw(gam + bg) - (gam + 3.0) * bg) / (bg2 * (gam + 1.0));
 }
 
 // ---------------------------------------------------------------- geometry
@@ -1000,12 +1089,12 @@ static void sample_compton(const Material &, double e0, const double d0[3], uint
   const double E0_m = e0 / kMe;
   const double eps0 = 1.0 / (1.0 + 2.0 * E0_m);
   const double eps0sq = eps0 * eps0;
-  const double alpha1 = -std::log(eps0);
+  const double alpha1 = -log_cw(eps0);
   const double alpha2 = 0.5 * (1.0 - eps0sq);
   double eps, epssq, onecost, sint2, greject;
   do {
     if (alpha1 / (alpha1 + alpha2) > mrg_uniform(s)) {
-      eps = std::exp(-alpha1 * mrg_uniform(s));
+      eps = exp_cw(-alpha1 * mrg_uniform(s));
       epssq = eps * eps;
     } else {
       epssq = eps0sq + (1.0 - eps0sq) * mrg_uniform(s);
@@ -1041,10 +1130,10 @@ static void sample_compton(const Material &, double e0, const double d0[3], uint
 
 // Bethe-Heitler screening functions (ProxyBetheHeitler.hpp:184-210).
 static inline double screen1(double x) {
-  return x > 1.0 ? 42.24 - 8.368 * std::log(x + 0.952) : 42.392 - x * (7.796 - 1.961 * x);
+  return x > 1.0 ? 42.24 - 8.368 * log_cw(x + 0.952) : 42.392 - x * (7.796 - 1.961 * x);
 }
 static inline double screen2(double x) {
-  return x > 1.0 ? 42.24 - 8.368 * std::log(x + 0.952) : 41.405 - x * (5.828 - 0.8945 * x);
+  return x > 1.0 ? 42.24 - 8.368 * log_cw(x + 0.952) : 41.405 - x * (5.828 - 0.8945 * x);
 }
 // ProxyEmModel::ComputeCoulombFactor (proxy/ProxyEmModel.hpp:114-128).
 static inline double coulomb_factor(double Z) {
@@ -1073,12 +1162,12 @@ static void sample_pair(const Material &m, double e0, const double d0[3], uint32
   if (e0 < 2.0) {
     eps = eps0 + (0.5 - eps0) * mrg_uniform(s);
   } else {
-    const double logZ3 = std::log(1.0 * Z) / 3.0;
+    const double logZ3 = log_cw(1.0 * Z) / 3.0;
     double FZ = 8.0 * logZ3;
     if (e0 > 50.0) FZ += 8.0 * coulomb_factor(Z);
-    const double Z3 = std::pow(1.0 * Z, 1.0 / 3.0);
+    const double Z3 = pow_cw(1.0 * Z, 1.0 / 3.0);
     const double screenfac = 136.0 * eps0 / Z3;
-    const double screenmax = std::exp((42.24 - FZ) / 8.368) - 0.952;
+    const double screenmax = exp_cw((42.24 - FZ) / 8.368) - 0.952;
     const double screenmin = std::min(4.0 * screenfac, screenmax);
     const double eps1 = 0.5 - 0.5 * std::sqrt(1.0 - screenmin / screenmax);
     const double epsmin = std::max(eps0, eps1);
@@ -1090,7 +1179,7 @@ static void sample_pair(const Material &m, double e0, const double d0[3], uint32
     double greject;
     do {
       if (NormF1 / (NormF1 + NormF2) > mrg_uniform(s)) {
-        eps = 0.5 - epsrange * std::pow(mrg_uniform(s), 0.333333);
+        eps = 0.5 - epsrange * pow_cw(mrg_uniform(s), 0.333333);
         const double sv = screenfac / (eps * (1.0 - eps));
         greject = (screen1(sv) - FZ) / F10;
       } else {
@@ -1104,7 +1193,7 @@ static void sample_pair(const Material &m, double e0, const double d0[3], uint32
   const double pTot = e0 - eTot;
   const double r1 = mrg_uniform(s);
   const double r2 = mrg_uniform(s);
-  const double aa0 = -std::log(r1 * r2);
+  const double aa0 = -log_cw(r1 * r2);
   const double u = (0.25 > mrg_uniform(s)) ? aa0 / 0.625 : aa0 / 1.875;
   const double phi = kTwoPi * mrg_uniform(s);
   const double cosp = cos_cw(phi), sinp = sin_cw(phi);
@@ -1201,21 +1290,21 @@ static void sample_brems(const Material &m, double t0, const double d0[3], uint3
   const SB2D &sb = m.el[ie].sb;
   const double etot = t0 + kMe;
   const double dcorr = m.migdal * etot * etot;
-  const double xmin = std::log(cut * cut + dcorr);
-  const double xmax = std::log(emax * emax + dcorr);
-  const double y = std::log(t0);
+  const double xmin = log_cw(cut * cut + dcorr);
+  const double xmax = log_cw(emax * emax + dcorr);
+  const double y = log_cw(t0);
   const double x0 = cut / t0;
   double vmax = sb_value(sb, x0, y) * 1.02;
   const double epeak = 300.0, elow = 10.0e-3;
   if (x0 < 0.97 && (t0 > epeak || t0 < elow)) {
-    const double ylim = std::min(sb_value(sb, 0.97, 4.0 * std::log(10.0)), 1.1 * sb_value(sb, 0.97, y));
+    const double ylim = std::min(sb_value(sb, 0.97, 4.0 * log_cw(10.0)), 1.1 * sb_value(sb, 0.97, y));
     if (ylim > vmax) vmax = ylim;
   }
   if (x0 < 0.05) vmax *= 1.2;
   double k, v;
   do {
     const double r = mrg_uniform(s);
-    double x = std::exp(xmin + r * (xmax - xmin)) - dcorr;
+    double x = exp_cw(xmin + r * (xmax - xmin)) - dcorr;
     if (x < 0.0) x = 0.0;
     k = std::sqrt(x);
     v = sb_value(sb, k / t0, y);
@@ -1225,7 +1314,7 @@ static void sample_brems(const Material &m, double t0, const double d0[3], uint3
   const double signc = c < 0.0 ? -1.0 : 1.0;
   const double a = std::fabs(c);
   const double delta = 0.5 * (std::sqrt(4.0 + a * a) + a);
-  const double cofA = -signc * std::pow(delta, 1.0 / 3.0);
+  const double cofA = -signc * pow_cw(delta, 1.0 / 3.0);
   double cost = cofA - 1.0 / cofA;
   const double tau = t0 / kMe;
   const double beta = std::sqrt(tau * (tau + 2.0)) / (tau + 1.0);
@@ -1338,7 +1427,7 @@ static void sample_annihil(const Material &, double t0, const double d0[3], uint
   const double epsqot = epsmax / epsmin;
   double eps, greject;
   do {
-    eps = epsmin * std::exp(std::log(epsqot) * mrg_uniform(s));
+    eps = epsmin * exp_cw(log_cw(epsqot) * mrg_uniform(s));
     greject = 1.0 - eps + (2.0 * gam * eps - 1.0) / (eps * tau2 * tau2);
   } while (greject < mrg_uniform(s));
   double cost = (eps * tau2 - 1.0) / (eps * sqg2m1);
@@ -1420,13 +1509,13 @@ static double msc_lambda1(const Material &m, bool pos, double e) {
 }
 
 // (expm1 and log1p keep full precision for short steps)
-static double msc_geom(double t, double lam) { return -lam * std::expm1(-t / lam); }
+static double msc_geom(double t, double lam) { return -lam * expm1_cw(-t / lam); }
 
 // true path of a geometric path z, at most tmax
 static double msc_true(double z, double lam, double tmax) {
   const double x = z / lam;
   if (x >= 1.0) return tmax;
-  return std::min(-lam * std::log1p(-x), tmax);
+  return std::min(-lam * log1p_cw(-x), tmax);
 }
 
 // Screening of the deflection x = 1 - cos(theta) ~ 1/(x + a)^2 on [0, 2],
@@ -1439,16 +1528,16 @@ static double msc_true(double z, double lam, double tmax) {
 // ((j + 1)(j + 2)) (from the series of ln(1 + u)) is solved for u < 0.17 by
 // twelve Newton steps from u = 3q with 20 terms.
 static double msc_u(double tau) {
-  const double q = std::exp(-tau);
+  const double q = exp_cw(-tau);
   if (q >= 0.05) {
-    const double mean = -std::expm1(-tau);
-    double a = mean < 0.5 ? mean / std::max(std::log(2.0 / mean) - 1.0, 0.5) : 2.0 / (3.0 * q);
+    const double mean = -expm1_cw(-tau);
+    double a = mean < 0.5 ? mean / std::max(log_cw(2.0 / mean) - 1.0, 0.5) : 2.0 / (3.0 * q);
     for (int k = 0; k < 12; ++k) {
-      const double L = std::log1p(2.0 / a);
+      const double L = log1p_cw(2.0 / a);
       const double f = 0.5 * a * (a + 2.0) * L - a - mean;
       const double df = a * ((a + 1.0) * L - 2.0);
       const double du = std::min(std::max(-f / df, -2.0), 2.0);
-      a *= std::exp(du);
+      a *= exp_cw(du);
     }
     return 2.0 / a;
   }
@@ -1602,7 +1691,7 @@ static void push_secondary(const gxref_ctx &c, Worker &w, EventTally &ev, std::v
   t.uid = child_substream(s);
   substream_state(c.seed, t.uid, t.rng);
   ev.nsec[species]++;
-  double lg = std::log10(ekin);
+  double lg = log10_cw(ekin);
   int b = (int)std::floor((lg + 3.0) * 8.0);
   if (b >= 0 && b < 64) w.spectra[species * 64 + b]++;
   stack.push_back(t);
@@ -1782,7 +1871,7 @@ static void one_step(gxref_ctx &c, Worker &w, EventTally &ev, std::vector<Track>
   // PostStep GPIL (ProxyEmProcess.hpp:122-151)
   for (int p = 0; p < np; ++p) {
     xs[p] = macro_xs(m, t.species, p, t.e);
-    if (t.nil[p] <= 0.0) t.nil[p] = -std::log(mrg_uniform(t.rng));
+    if (t.nil[p] <= 0.0) t.nil[p] = -log_cw(mrg_uniform(t.rng));
     const double sp = xs[p] > 0.0 ? t.nil[p] / xs[p] : kHuge;
     if (sp < best) {
       best = sp;
@@ -2679,5 +2768,23 @@ extern "C" int gxref_nav_find_next_boundary(const gxref_nav *g, const int32_t *c
 extern "C" int gxref_sincos(const double *x, double *s, double *c, int n) {
   if (!x || !s || !c || n <= 0) return gxref::fail(GXT_EINVAL, "bad sincos arguments");
   for (int i = 0; i < n; ++i) gxref::sincos_cw(x[i], s[i], c[i]);
+  return GXT_OK;
+}
+
+// The shared FP64 functions, for their accuracy tests: which = 0 log, 1 exp,
+// 2 log1p, 3 expm1, 4 pow(x, y), 5 log10.
+extern "C" int gxref_mathfn(int which, const double *x, const double *y, double *out, int n) {
+  if (!x || !out || n <= 0 || which < 0 || which > 5 || (which == 4 && !y))
+    return gxref::fail(GXT_EINVAL, "bad mathfn arguments");
+  for (int i = 0; i < n; ++i) {
+    switch (which) {
+      case 0: out[i] = gxref::log_cw(x[i]); break;
+      case 1: out[i] = gxref::exp_cw(x[i]); break;
+      case 2: out[i] = gxref::log1p_cw(x[i]); break;
+      case 3: out[i] = gxref::expm1_cw(x[i]); break;
+      case 4: out[i] = gxref::pow_cw(x[i], y[i]); break;
+      default: out[i] = gxref::log10_cw(x[i]); break;
+    }
+  }
   return GXT_OK;
 }

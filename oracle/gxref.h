@@ -91,6 +91,9 @@ int gxref_nav_step(const gxref_nav *nav, const int32_t *touchable, const double 
 int gxref_nav_find_next_boundary(const gxref_nav *nav, const int32_t *touchable, const double *pos, const double *dir,
                                  const double *pstep, double *snext, int32_t *next, double *safety, int32_t *boundary,
                                  int n);
+/* The FP64 log, exp, log1p, expm1, pow, log10 (which = 0..5) the oracle and
+ * the device build share. */
+int gxref_mathfn(int which, const double *x, const double *y, double *out, int n);
 /* The FP64 sine and cosine the oracle and the device build share. */
 int gxref_sincos(const double *x, double *s, double *c, int n);
 /* Sandia photo-absorption macroscopic cross-section (1/mm). */
