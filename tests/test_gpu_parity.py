@@ -201,12 +201,22 @@ def test_transport_calorimeter_field_event_parity(gpu_lib):
         np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-9)
 
 
-def test_trace_first_50_steps(gpu_lib):
-    _flush("trace 50 steps")
-    n = 3
-    cfg = abi.config_calorimeter(max_events=n, trace_tracks=n, trace_steps=50)
-    pg = gpu_lib.make_primaries(cfg.seed, abi.ELECTRON, 2000.0, 2000.0, 10.0, n)
-    pc = O.make_primaries(cfg.seed, abi.ELECTRON, 2000.0, 2000.0, 10.0, n)
+@pytest.mark.parametrize("workload", ["calorimeter_4T_2GeV", "calorimeter_4T_20GeV", "pbwo4_10GeV"])
+def test_trace_first_50_steps(gpu_lib, workload):
+    """The first 50 steps of every track of whole events (primaries and all
+    their descendants) are bit-identical to the oracle's: the same
+    step/species/process sequences and the same FP64 kinematics and
+    deposits.  Both sides do the same correctly rounded operations,
+    sin/cos/log/exp included (the shared functions)."""
+    _flush("trace 50 steps: %s" % workload)
+    if workload == "pbwo4_10GeV":
+        n, e0 = 2, 10000.0
+        cfg = abi.config_pbwo4_block(max_events=n, trace_tracks=n, trace_steps=50)
+    else:
+        n, e0 = (3, 2000.0) if workload.endswith("2GeV") else (2, 20000.0)
+        cfg = abi.config_calorimeter(max_events=n, trace_tracks=n, trace_steps=50)
+    pg = gpu_lib.make_primaries(cfg.seed, abi.ELECTRON, e0, e0, 10.0, n)
+    pc = O.make_primaries(cfg.seed, abi.ELECTRON, e0, e0, 10.0, n)
     gpu, cpu, _, _, _, _, gtr, ctr, _ = _transport_pair(cfg, pg, pc, n)
     assert len(gtr) == len(ctr) and len(gtr) > 1000
 
@@ -218,21 +228,12 @@ def test_trace_first_50_steps(gpu_lib):
     c = ctr[key(ctr)]
     assert np.array_equal(g[:, 0].view(np.uint64), c[:, 0].view(np.uint64))
     assert np.array_equal(g[:, 1:4], c[:, 1:4]), "step/species/process sequences differ"
-    names = ["ekin", "x", "y", "z", "dx", "dy", "dz", "edep"]
-    bad = ~np.isclose(g[:, 4:], c[:, 4:], rtol=1e-10, atol=1e-10)
-    if bad.any():
-        for j, nm in enumerate(names):
-            if bad[:, j].any():
-                d = np.abs(g[:, 4 + j] - c[:, 4 + j])[bad[:, j]]
-                print("column %s: %d mismatches, max abs %.3g" % (nm, bad[:, j].sum(), d.max()))
-    # primaries: the first 50 steps agree within 1e-10 (relative; 1e-10 on unit vectors)
     prim = g[:, 0].copy().view(np.uint64) < n
     assert prim.sum() > 50
-    np.testing.assert_allclose(g[prim, 4:], c[prim, 4:], rtol=1e-10, atol=1e-10)
-    # secondaries start from their ancestors' state after up to thousands of
-    # steps, so they inherit the libm-level drift of those (untraced) steps:
-    # same sequences, kinematics within 1e-6
-    np.testing.assert_allclose(g[:, 4:], c[:, 4:], rtol=1e-6, atol=1e-6)
+    names = ["ekin", "x", "y", "z", "dx", "dy", "dz", "edep"]
+    for j, nm in enumerate(names):
+        assert np.array_equal(g[:, 4 + j], c[:, 4 + j]), nm
+    _compare_events(gpu, cpu)
 
 
 @pytest.mark.parametrize("precision", [abi.FP32, abi.FP64])
