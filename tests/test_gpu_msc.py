@@ -1,9 +1,9 @@
 """GPU parity of e-/e+ multiple scattering: libgxt.so against the CPU oracle,
 through the C ABI.  The model (the oracle's, pinned by its first moment and
-closed forms in test_msc.py) has no counterpart in the reference.  Without a
-field, events agree in every count and within 1e-9 in every energy, as for
-the rest of the transport; in a field all but at most one event have the
-same counts and energies within 1e-6 (see the test); the FP32 precision form
+closed forms in test_msc.py) has no counterpart in the reference.  With or
+without a field, events agree in every count and within 1e-9 in every energy,
+as for the rest of the transport (both sides compute the same correctly
+rounded FP64 operations, sin/cos/log/exp included); the FP32 precision form
 agrees with FP64 in distribution."""
 import numpy as np
 import pytest
@@ -27,27 +27,9 @@ def test_transport_msc_event_parity(gpu_lib, field):
     pg = gpu_lib.make_primaries(cfg.seed, abi.ELECTRON, 1000.0, 3000.0, 10.0, n)
     pc = O.make_primaries(cfg.seed, abi.ELECTRON, 1000.0, 3000.0, 10.0, n)
     gpu, cpu, gprof, cprof, *_ = _transport_pair(cfg, pg, pc, n)
-    if field == "none":
-        _compare_events(gpu, cpu)
-        for a, b in zip(gprof, cprof):
-            np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-9)
-        return
-    # In a field, secondaries inherit the ulp-level drift of device and host
-    # sin/cos over thousands of helix steps (up to 1e-6 relative, as in the
-    # 50-step trace test); with the extra deflections a step or boundary
-    # decision of one track in a 20k-step event can flip on it, after which
-    # that track follows another history.  All other events have the same
-    # counts, and energies within 1e-6 (the drift).
-    same = np.ones(n, dtype=bool)
-    for k in ("n_steps", "n_secondaries", "n_pairs", "n_annihil"):
-        d = gpu[k] != cpu[k]
-        same &= ~(d.any(axis=1) if d.ndim == 2 else d)
-    assert same.sum() >= n - 1, np.nonzero(~same)
-    for k in ("edep", "visible", "leak", "charged_length"):
-        np.testing.assert_allclose(gpu[k][same], cpu[k][same], rtol=1e-6, atol=1e-6, err_msg=k)
-    np.testing.assert_allclose(gpu["n_steps"], cpu["n_steps"], rtol=0.02)
-    for k in ("edep", "visible", "charged_length"):
-        np.testing.assert_allclose(gpu[k], cpu[k], rtol=0.02, err_msg=k)
+    _compare_events(gpu, cpu)
+    for a, b in zip(gprof, cprof):
+        np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-9)
 
 
 def test_transport_msc_fp32(gpu_lib):
