@@ -474,8 +474,13 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
 // uniform field), both through propagate_general and with or without
 // multiple scattering.  Each mode is its own kernel, so the simpler ones keep
 // their register budgets.
+// The plain e-/e+ transport (no field map, RK45 or multiple scattering) fits
+// 3 waves per SIMD with a 28 B/lane spill; that measured 1.6 % faster on the
+// 10 GeV e- / PbWO4 batch than 2 waves without spill.  The other modes spill
+// heavily at 3 and keep 2.
 template <class R, bool kCharged, int kMode = 0>
-__global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_transport(StepArgs a, PoolArgs<R> pa) {
+__global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAVES) void k_transport(StepArgs a,
+                                                                                               PoolArgs<R> pa) {
   __shared__ WaveHist whs[kBlock / 64];
   WaveHist &wh = whs[threadIdx.x >> 6];
   wh_clear(wh);
@@ -694,8 +699,11 @@ __device__ __forceinline__ int sec_fate(const DevParams *P, TrackRegs<R> &t, Dep
   return 1;
 }
 
+// FP64 interactions at 3 waves per SIMD (e-/e+: 160 VGPRs; measured 1.9 %
+// faster on the 10 GeV e- / PbWO4 batch); the FP32 form was 0.6 % slower at
+// 3 and keeps 2.
 template <class R, bool kCharged>
-__global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a, PoolArgs<R> pa) {
+__global__ __launch_bounds__(kBlock, sizeof(R) == 8 ? 3 : GXT_STEP_WAVES) void k_interact(StepArgs a, PoolArgs<R> pa) {
   __shared__ WaveHist whs[kBlock / 64];
   WaveHist &wh = whs[threadIdx.x >> 6];
   wh_clear(wh);
