@@ -321,3 +321,26 @@ def test_find_next_boundary_cases():
     assert (s[4], nxt[4], b[4]) == (tol2, 53, 0)
     assert (s[5], nxt[5], b[5]) == (3.0 + tol2, 53, 0)
     np.testing.assert_array_equal(saf, [5.0, 5.0, 5.0, 5.0, 0.0, 0.0])
+
+
+def test_query_argument_checks_without_device():
+    """Navigation queries and the other [3n] entry points reject bad
+    arguments, and more than 2^29 points, before any device call."""
+    lib = _lib()
+    pos = np.zeros(3)
+    d = np.array([1.0, 0.0, 0.0])
+    t = np.zeros(1, dtype=np.int32)
+    out = np.zeros(1)
+    assert lib.gxt_nav_locate(None, abi.ptr(pos), abi.ptr(d), abi.ptr(t, C.c_int32), 1) == -1
+    assert lib.gxt_nav_step(None, abi.ptr(t, C.c_int32), abi.ptr(pos), abi.ptr(d), abi.ptr(out),
+                            abi.ptr(t, C.c_int32), abi.ptr(out), 1) == -1
+    assert lib.gxt_nav_find_next_boundary(None, abi.ptr(t, C.c_int32), abi.ptr(pos), abi.ptr(d), abi.ptr(out),
+                                          abi.ptr(out), abi.ptr(t, C.c_int32), abi.ptr(out), abi.ptr(t, C.c_int32),
+                                          1) == -1
+    assert lib.gxt_nav_free(None) == -1
+    assert lib.gxt_nav_info(None, None, None, None, None) == -1
+    f = (C.c_double * 3)(0.0, 0.0, 4.0)
+    q, p, s = np.ones(1), np.ones(1), np.ones(1)
+    big = (1 << 29) + 1
+    assert lib.gxt_helix_step(f, abi.FP64, abi.ptr(q), abi.ptr(p), abi.ptr(s), abi.ptr(pos), abi.ptr(d), big) == -1
+    assert b"2^29" in lib.gxt_last_error()
