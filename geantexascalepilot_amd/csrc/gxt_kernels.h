@@ -32,7 +32,8 @@ struct StepArgs {
   int64_t trace_cap;
   int transport_mode;  // e-/e+ transport: 0 plain, 1 with multiple scattering, 2 field map with the
                        // local helix, 3 RK45 stepper
-  int pad;
+  int inner_steps;     // steps a track may take in one transport launch while it neither
+                       // interacts nor ends (>= 1)
 };
 
 template <class R>

@@ -31,10 +31,7 @@ constexpr int kBlock = 256;
 #define GXT_STEP_WAVES 2
 #endif
 
-// Steps a track may take in one transport launch while it neither interacts
-// nor ends; each is the same as one iteration of the step loop, and a
-// track's results do not depend on how its steps are scheduled.
-constexpr int kInnerSteps = 4;
+
 
 // ------------------------------------------------------------------ wave helpers
 __device__ __forceinline__ int lane() { return (int)__lane_id(); }
@@ -526,7 +523,7 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
       const int sp = kCharged ? (t.species == 2 ? 2 : 1) : 0;
       const bool charged = kCharged;
       // a track keeps stepping while it neither interacts nor ends, up to
-      // kInnerSteps steps: each step is the same as one iteration of the loop
+      // a.inner_steps steps: each step is the same as one iteration of the loop
       for (;;) {
       survive = false;
       edep = 0.0;
@@ -631,7 +628,7 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
       dA.flush(wh);
       dB.flush(wh);
       ++nsub;
-      if (!survive || nsub >= kInnerSteps) break;
+      if (!survive || nsub >= a.inner_steps) break;
       }
     }
     // returning atomics first, all in flight together: the survivor's slot in
