@@ -435,28 +435,35 @@ def main():
     value = a.steps * n / r64["seconds"]
     ks = r64["ks"]
     steps_per_s = ks.track_steps / r64["seconds"]
-    # Algorithmic bytes per track-step: one read and one write of a 148 B
-    # track record (FP64 form; 108 B in the FP32 form): position 24, direction
-    # 24, energy 8, 3 interaction lengths 24, 3 inverse mean free paths 24,
-    # safety 8, RNG 24, couple, species/status, job 4 each.
-    rec = 148
+    # Algorithmic bytes of the dominant kernel: one read and one write of the
+    # pool's track record (140 B in the FP64 form: position 24, direction 24,
+    # energy 8, 3 interaction lengths 24, RNG 24, event, volume|species, step
+    # count 4 each, 3 per-track tallies 24) for every track a launch loads.  A
+    # track takes up to inner_steps steps in one launch while it neither
+    # interacts nor ends, with one load and one store, so the unit is the
+    # track loaded, not the track-step.  (The per-track-step figure of 2 x 148 B,
+    # a nominal record with inverse mean free paths and safety, is reported
+    # beside it.)
+    rec = int(ks.record_bytes)
     # The dominant kernel (the rocprofv3 summary in profiles/kernels_c1_fp64.md)
     # is the e-/e+ transport; only its launches are bracketed by HIP events
     # during the timed batches (profile_kernels = 2), which keeps the
     # instrumentation out of the measured throughput.
     name = "k_transport<f64,e-/e+>"
-    ms, units = ks.transport_charged_ms, ks.charged_steps
+    ms, units = ks.transport_charged_ms, ks.charged_loaded
     nbytes = units * 2 * rec
-    per_unit = "2 x %d B per track-step (one read and one write of the track record), %d e-/e+ steps" % (rec, units)
+    per_unit = ("2 x %d B per e-/e+ track loaded by a launch (one read and one write of its record), "
+                "%d tracks loaded for %d e-/e+ steps" % (rec, units, ks.charged_steps))
     launches = max(ks.launches, 1)
     avg_us = 1e3 * ms / launches
     achieved = (nbytes / launches) / (avg_us * 1e-6) / 1e9
+    per_step_gbs = (ks.charged_steps * 2 * 148 / launches) / (avg_us * 1e-6) / 1e9
     roofline = dict(bound="hbm", achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s", frac=achieved / PEAK_HBM_GBS,
                     traffic=profiled_traffic(name), kernel=name, launches=int(ks.launches), avg_launch_us=avg_us,
-                    bytes_per_launch=nbytes / launches, per_unit=per_unit, record_bytes_in_pool=int(ks.record_bytes),
-                    kernel_ms=ms,
-                    # every track-step of the run (photons included) at 2 x 148 B over the wall time
-                    whole_run_gbs=ks.track_steps * 2 * rec / r64["seconds"] / 1e9)
+                    bytes_per_launch=nbytes / launches, per_unit=per_unit, record_bytes_in_pool=rec,
+                    kernel_ms=ms, per_track_step_296b_gbs=per_step_gbs,
+                    # every track loaded by a transport launch (photons included) over the wall time
+                    whole_run_gbs=(ks.charged_loaded + ks.neutral_loaded) * 2 * rec / r64["seconds"] / 1e9)
     e64 = r64["edep"]["edep"]
     out = dict(metric="events/s and track-steps/s, FP32 vs FP64 on 1 MI355X; p99 job latency", value=value,
                unit="events/s", n_gpus=1, steps=a.steps, warmup=a.warmup,

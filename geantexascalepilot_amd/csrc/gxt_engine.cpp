@@ -59,6 +59,22 @@ hipError_t dalloc(T **p, size_t n) {
   return hipMalloc((void **)p, n * sizeof(T));
 }
 
+// Device buffer of a component entry point, freed when it goes out of scope
+// (also on the early returns of HIP_TRY).  n == 0: no buffer, p null.
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  hipError_t err = hipSuccess;
+  explicit DevBuf(size_t n) {
+    if (n) err = hipMalloc((void **)&p, n * sizeof(T));
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+};
+
 struct DeviceData {
   DevParams *P = nullptr;
   Node64 *nd = nullptr;
@@ -248,7 +264,8 @@ struct gxt_ctx {
   std::thread server;
   std::atomic<bool> serving{false};
   std::atomic<bool> stop{false};
-  int serve_rc = GXT_OK;
+  std::atomic<int> serve_rc{GXT_OK};  // first error of the serving thread
+  std::mutex err_mu;                  // guards err (the serving thread writes it)
 
   StepArgs args() {
     StepArgs a;
@@ -270,8 +287,16 @@ struct gxt_ctx {
   }
 
   int fail(int code, const std::string &m) {
-    err = m;
+    {
+      std::lock_guard<std::mutex> lk(err_mu);
+      err = m;
+    }
     return set_err(code, m);
+  }
+
+  std::string error() {
+    std::lock_guard<std::mutex> lk(err_mu);
+    return err;
   }
 };
 
@@ -669,7 +694,13 @@ int rng_params(const DevParams **out) {
 extern "C" {
 
 const char *gxt_last_error(void) { return g_last_error.c_str(); }
-const char *gxt_ctx_error(gxt_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+// A copy per calling thread: the serving thread may replace the context's
+// message while the caller reads it.
+const char *gxt_ctx_error(gxt_ctx *ctx) {
+  static thread_local std::string copy;
+  copy = ctx ? ctx->error() : std::string("null context");
+  return copy.c_str();
+}
 
 int gxt_default_config(gxt_config *c) {
   if (!c) return set_err(GXT_EINVAL, "null config");
@@ -814,19 +845,30 @@ int gxt_destroy(gxt_ctx *ctx) {
 double gxt_now_ms(gxt_ctx *ctx) { return ctx ? now_ms(ctx->t0) : 0.0; }
 
 int gxt_submit(gxt_ctx *c, const gxt_primary *prim, int n, uint64_t job_id) {
-  if (!c || !prim || n <= 0) return set_err(GXT_EINVAL, "bad submit arguments");
-  auto j = std::make_unique<Job>();
-  j->id = job_id;
-  j->prims.assign(prim, prim + n);
+  if (!prim || n <= 0) return set_err(GXT_EINVAL, "bad submit arguments");
+  // every primary is checked before anything is queued: a NaN position would
+  // be located in volume 0, a non-unit direction would scale every distance,
+  // an infinite energy would hold back admission for good
   int maxev = -1;
   for (int i = 0; i < n; ++i) {
     const gxt_primary &p = prim[i];
-    if (p.species < 0 || p.species > 2 || !(p.ekin > 0) || p.event < 0)
-      return c->fail(GXT_EINVAL, "bad primary " + std::to_string(i));
+    const std::string at = " (primary " + std::to_string(i) + ")";
+    if (p.species < 0 || p.species > 2) return set_err(GXT_EINVAL, "bad species" + at);
+    if (!(p.ekin > 0) || !std::isfinite(p.ekin)) return set_err(GXT_EINVAL, "ekin must be finite and > 0" + at);
+    if (p.event < 0) return set_err(GXT_EINVAL, "negative event index" + at);
+    bool finite = true;
+    for (int k = 0; k < 3; ++k) finite = finite && std::isfinite(p.pos[k]) && std::isfinite(p.dir[k]);
+    if (!finite) return set_err(GXT_EINVAL, "non-finite position or direction" + at);
+    const double nn = p.dir[0] * p.dir[0] + p.dir[1] * p.dir[1] + p.dir[2] * p.dir[2];
+    if (!(std::fabs(nn - 1.0) <= 2e-6)) return set_err(GXT_EINVAL, "direction is not a unit vector" + at);
     maxev = std::max(maxev, p.event);
   }
+  if (!c) return set_err(GXT_EINVAL, "null context");
+  if (maxev >= c->cfg.max_events) return c->fail(GXT_ECAPACITY, "job has more events than max_events");
+  auto j = std::make_unique<Job>();
+  j->id = job_id;
+  j->prims.assign(prim, prim + n);
   j->n_events = maxev + 1;
-  if (j->n_events > c->cfg.max_events) return c->fail(GXT_ECAPACITY, "job has more events than max_events");
   std::stable_sort(j->prims.begin(), j->prims.end(),
                    [](const gxt_primary &a, const gxt_primary &b) { return a.event < b.event; });
   j->ev_first.assign(j->n_events + 1, 0);
@@ -901,7 +943,10 @@ int gxt_poll(gxt_ctx *c, gxt_job_result *out, int max, int *n_out) {
     r.track_steps = j->steps;
   }
   *n_out = n;
-  return c->serve_rc;
+  // an error of the serving thread is reported here, to the polling thread
+  const int rc = c->serve_rc.load();
+  if (rc != GXT_OK) return set_err(rc, c->error());
+  return GXT_OK;
 }
 
 int gxt_get_event_tallies(gxt_ctx *c, uint64_t job_id, gxt_event_tally *out, int n) {
@@ -970,6 +1015,8 @@ int gxt_get_kernel_stats(gxt_ctx *c, gxt_kernel_stats *o) {
   o->neutral_steps = (int64_t)(h.steps - h.steps_charged);
   o->interactions = (int64_t)h.interactions;
   o->created = (int64_t)h.created;
+  o->neutral_loaded = (int64_t)h.loaded[0];
+  o->charged_loaded = (int64_t)h.loaded[1];
   // x, y, z (FP64) + dir, E, 3 interaction lengths (R) + 6 RNG words + event, meta, nsteps
   // + 3 FP64 per-track tallies
   const int r = c->precision == GXT_FP32 ? 4 : 8;
@@ -991,6 +1038,7 @@ int gxt_reset_tallies(gxt_ctx *c) {
   h.trace_n = 0;
   h.interactions = 0;
   h.created = 0;
+  h.loaded[0] = h.loaded[1] = 0;
   c->kst = gxt_kernel_stats{};
   HIP_TRY(hipMemcpy(c->cnt, &h, sizeof(h), hipMemcpyHostToDevice));
   c->iterations = 0;
@@ -1032,7 +1080,9 @@ int gxt_serve_stop(gxt_ctx *c) {
   c->serving = false;
   // the last round's look-ahead iterations
   HIP_TRY(hipStreamSynchronize(c->stream));
-  return c->serve_rc;
+  const int rc = c->serve_rc.load();
+  if (rc != GXT_OK) return set_err(rc, c->error());
+  return GXT_OK;
 }
 
 int gxt_make_primaries(uint32_t seed, int species, double emin, double emax, double spot_mm, int n, uint64_t first,
@@ -1172,6 +1222,41 @@ int gxt_sample(gxt_ctx *c, int model, int slot, int precision, gxt_sample_io *io
   HIP_TRY(hipMemcpy(io->sec_dir, sdir, 6 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
   void *ptrs[] = {ek, dir, rng, oek, odir, oed, nsec, ssp, sek, sdir};
   for (void *p : ptrs) (void)hipFree(p);
+  return GXT_OK;
+}
+
+int gxt_slab_distance(gxt_ctx *c, const double *pos, const double *dir, const double *pstep, int32_t *vol,
+                      double *snext, int32_t *next, double *safety, int32_t *boundary, int n) {
+  if (!c || !pos || !dir || !vol || !snext || !next || !safety || n <= 0 || n > kMaxSoaPoints)
+    return set_err(GXT_EINVAL, "bad slab_distance arguments (1 <= n <= 2^29)");
+  const int nvol = c->hd.params.geo.nvol;
+  for (int i = 0; i < n; ++i) {
+    const double d[3] = {dir[i], dir[n + i], dir[2 * n + i]};
+    const double nn = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (!std::isfinite(pos[i]) || !std::isfinite(pos[n + i]) || !std::isfinite(pos[2 * n + i]) ||
+        !(std::fabs(nn - 1.0) <= 2e-6))
+      return set_err(GXT_EINVAL, "slab_distance: non-finite position or non-unit direction at " + std::to_string(i));
+    if (vol[i] >= nvol) return set_err(GXT_EINVAL, "slab_distance: volume out of range at " + std::to_string(i));
+    if (pstep && !(pstep[i] >= 0.0 && std::isfinite(pstep[i])))
+      return set_err(GXT_EINVAL, "slab_distance: bad pstep at " + std::to_string(i));
+  }
+  DevBuf<double> dp(3 * (size_t)n), dd(3 * (size_t)n), ds(n), dsaf(n), dps(pstep ? n : 0);
+  DevBuf<int32_t> dv(n), dn(n), db(boundary ? n : 0);
+  for (hipError_t e : {dp.err, dd.err, ds.err, dsaf.err, dps.err, dv.err, dn.err, db.err})
+    HIP_TRY(e);
+  HIP_TRY(hipMemcpyAsync(dp.p, pos, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(dd.p, dir, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(dv.p, vol, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  if (pstep) HIP_TRY(hipMemcpyAsync(dps.p, pstep, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(launch_slab_distance(c->dd.P, dp.p, dd.p, pstep ? dps.p : nullptr, dv.p, ds.p, dn.p, dsaf.p,
+                               boundary ? db.p : nullptr, n, c->stream));
+  HIP_TRY(hipMemcpyAsync(vol, dv.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(snext, ds.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(next, dn.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(safety, dsaf.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (boundary)
+    HIP_TRY(hipMemcpyAsync(boundary, db.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return GXT_OK;
 }
 

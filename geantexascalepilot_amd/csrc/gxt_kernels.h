@@ -87,6 +87,11 @@ hipError_t launch_macro_xs(const DevParams *P, int precision, int slot, int spec
 hipError_t launch_sample(const DevParams *P, int precision, int slot, int model, const SampleArgs &s, hipStream_t st);
 hipError_t launch_helix(const double B[3], int precision, const double *charge, const double *mom, const double *step,
                         double *pos, double *dir, int n, hipStream_t st);
+// Straight-line navigation of the analytic geometry (gxt_slab_distance);
+// pstep and boundary may be null.
+hipError_t launch_slab_distance(const DevParams *P, const double *pos, const double *dir, const double *pstep,
+                                int32_t *vol, double *snext, int32_t *next, double *safety, int32_t *boundary, int n,
+                                hipStream_t st);
 hipError_t launch_field_value(const DevField &F, const double *pos, double *b, int n, hipStream_t st);
 hipError_t launch_field_step(const DevField &F, int precision, const double *charge, const double *mom,
                              const double *step, double *pos, double *dir, double *err, int n, hipStream_t st);

@@ -663,12 +663,14 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
         atomicOr(&a.cnt->overflow, 1);
     }
     {
-      // the wave's steps, one atomic
+      // the wave's steps and loaded track records, one atomic each
       const int ns = wave_incl_scan(nsub);
       const int tot = __shfl(ns, 63, 64);
+      const unsigned long long nload = (unsigned long long)__popcll(__ballot(active));
       if (tot > 0 && lane() == 63) {
         atomicAdd(&a.cnt->steps, (unsigned long long)tot);
         if (kCharged) atomicAdd(&a.cnt->steps_charged, (unsigned long long)tot);
+        atomicAdd(&a.cnt->loaded[kCharged ? 1 : 0], nload);
       }
     }
   }
@@ -928,6 +930,38 @@ __global__ void k_rng_draw(uint32_t *states, int n, int k, double *out) {
   for (int j = 0; j < 6; ++j) r.s[j] = (double)states[6 * i + j];
   for (int j = 0; j < k; ++j) out[(int64_t)i * k + j] = r.uniform();
   for (int j = 0; j < 6; ++j) states[6 * i + j] = (uint32_t)r.s[j];
+}
+
+// Straight-line navigation of the analytic geometry for n points (the
+// locate, dist_out and safety of the step loop).
+__global__ void k_slab_distance(const DevParams *P, const double *pos, const double *dir, const double *pstep,
+                                int32_t *vol, double *snext, int32_t *next, double *saf, int32_t *boundary, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DevGeometry &g = P->geo;
+  const double p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
+  const double d0 = dir[i], d1 = dir[n + i], d2 = dir[2 * n + i];
+  int v = vol[i];
+  if (v < 0) v = locate(g, p, d0, d1, d2);
+  vol[i] = v;
+  if (v < 0) {
+    snext[i] = 0.0;
+    saf[i] = 0.0;
+    next[i] = -1;
+    if (boundary) boundary[i] = 1;
+    return;
+  }
+  int face;
+  const double s = dist_out(g, v, p, d0, d1, d2, face);
+  int nv = -1;
+  if (face == 0 || face == 1) {
+    nv = face == 1 ? v + 1 : v - 1;
+    if (nv >= g.nvol) nv = -1;
+  }
+  snext[i] = s;
+  next[i] = nv;
+  saf[i] = safety(g, v, p);
+  if (boundary) boundary[i] = pstep ? !(pstep[i] < s) : 1;
 }
 
 template <class R>
@@ -1211,6 +1245,14 @@ hipError_t launch_helix(const double B[3], int precision, const double *charge, 
   else
     hipLaunchKernelGGL(k_helix<double>, dim3(grid_for(n)), dim3(kBlock), 0, st, B[0], B[1], B[2], charge, mom, step,
                        pos, dir, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_slab_distance(const DevParams *P, const double *pos, const double *dir, const double *pstep,
+                                int32_t *vol, double *snext, int32_t *next, double *safety, int32_t *boundary, int n,
+                                hipStream_t st) {
+  hipLaunchKernelGGL(k_slab_distance, dim3(grid_for(n)), dim3(kBlock), 0, st, P, pos, dir, pstep, vol, snext, next,
+                     safety, boundary, n);
   return hipGetLastError();
 }
 

@@ -204,6 +204,7 @@ struct Counters {
   unsigned long long trace_n; // trace records written
   unsigned long long interactions;  // tracks sampled by the interaction kernel
   unsigned long long created;       // tracks created by the interaction kernel
+  unsigned long long loaded[2];     // track records read by the neutral / charged transport launches
 };
 
 // Per-event tallies on the device (FP64, atomics).

@@ -371,6 +371,26 @@ class Transport:
                                ekin, dirs, states)
         return from_io
 
+    def slab_distance(self, pos, dirs, vol=None, pstep=None):
+        """Straight-line navigation of the loop's geometry (FindNextBoundary,
+        geometry/NavigationInterface.hpp:64-98): returns (vol, snext, next, safety,
+        boundary) for points pos, dirs ([3, n] SoA); vol None locates the points."""
+        return _slab_with(lambda *a: lib.gxt_slab_distance(self.ctx, *a), pos, dirs, vol, pstep, self._c)
+
+
+def _slab_with(call, pos, dirs, vol, pstep, check):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64)
+    n = pos.shape[1]
+    v = np.full(n, -1, dtype=np.int32) if vol is None else np.ascontiguousarray(vol, dtype=np.int32).copy()
+    ps = None if pstep is None else np.ascontiguousarray(pstep, dtype=np.float64)
+    s, saf = np.zeros(n), np.zeros(n)
+    nxt = np.zeros(n, dtype=np.int32)
+    b = np.zeros(n, dtype=np.int32)
+    check(call(abi.ptr(pos), abi.ptr(dirs), None if ps is None else abi.ptr(ps), abi.ptr(v, C.c_int32), abi.ptr(s),
+               abi.ptr(nxt, C.c_int32), abi.ptr(saf), abi.ptr(b, C.c_int32), n))
+    return v, s, nxt, saf, b
+
 
 def _sample_with(call, ekin, dirs, states):
     n = len(ekin)

@@ -143,6 +143,9 @@ typedef struct gxt_kernel_stats {
   int64_t created;        /* tracks created by the interaction kernel              */
   int32_t record_bytes;   /* bytes of one track record in the pool                 */
   int32_t pad;
+  int64_t neutral_loaded; /* track records read by the photon transport launches (a
+                             track takes up to several steps per launch)          */
+  int64_t charged_loaded; /* track records read by the e-/e+ transport launches   */
 } gxt_kernel_stats;
 
 /* Completed job, as returned by gxt_poll. */
@@ -242,7 +245,10 @@ int gxt_get_trace(gxt_ctx *ctx, double *out, int64_t max_records, int64_t *n_out
 /* ---- serving form ------------------------------------------------------ */
 /* Start/stop the serving loop: a host thread that keeps the device step loop
  * running, admits queued jobs as pool space frees up, and completes jobs when
- * their last track dies.  gxt_submit stays non-blocking while serving. */
+ * their last track dies.  gxt_submit stays non-blocking while serving.  An
+ * error of the serving loop (e.g. GXT_ECAPACITY) stops it; gxt_poll and
+ * gxt_serve_stop then return its code, with the message in gxt_ctx_error and,
+ * on the calling thread, gxt_last_error. */
 int gxt_serve_start(gxt_ctx *ctx);
 int gxt_serve_stop(gxt_ctx *ctx);
 /* Host-clock milliseconds since the context was created. */
@@ -324,6 +330,29 @@ int gxt_sample(gxt_ctx *ctx, int model, int material_slot, int precision, gxt_sa
  * mm/MeV/tesla instead of cm/GeV/kilogauss).  pos, dir: [3n] SoA, updated in place. */
 int gxt_helix_step(const double field[3], int precision, const double *charge,
                    const double *momentum, const double *step, double *pos, double *dir, int n);
+
+/* Straight-line navigation of the transport loop's analytic geometry (the
+ * context's block, or its z-stacked layers): NavigationInterface::
+ * FindNextBoundary (geometry/NavigationInterface.hpp:64-98) and the volume
+ * bookkeeping of IsSameLocation (magneticfield/src/FieldPropagationHandler.cpp:
+ * 362-397) for the slab/box navigator the step loop runs.  For n points
+ * (pos, dir: [3n] SoA host arrays, dir unit vectors):
+ *   vol[i]      in: the volume holding the point (a track's current volume),
+ *                   or < 0 to locate it (a point on a plane belongs to the
+ *                   volume its direction enters); out: the volume used, -1
+ *                   when the point is outside the setup;
+ *   snext[i]    distance along dir to the volume's boundary;
+ *   next[i]     volume entered there (-1: the track leaves the setup);
+ *   safety[i]   distance to the volume's boundary in any direction;
+ *   boundary[i] (may be NULL) 1 when a straight step of pstep[i] reaches the
+ *               boundary (snext <= pstep; pstep NULL: always 1), the loop's
+ *               choice between a physics step and a boundary crossing.
+ * Outside the setup: snext = safety = 0, next = -1, boundary = 1.  Volumes are
+ * numbered along z from 0 (the calorimeter: absorber 2l, gap 2l+1 of layer
+ * l).  GXT_EINVAL for non-finite or non-unit input, vol[i] >= the number of
+ * volumes, or a negative or non-finite pstep. */
+int gxt_slab_distance(gxt_ctx *ctx, const double *pos, const double *dir, const double *pstep, int32_t *vol,
+                      double *snext, int32_t *next, double *safety, int32_t *boundary, int n);
 
 /* ---- non-uniform field ------------------------------------------------- */
 /* Cylindrical (r, z) field map on a regular grid, resident in device memory

@@ -1144,6 +1144,29 @@ static inline double coulomb_factor(double Z) {
   return (k1 * az4 + k2 + 1.0 / (1.0 + az2)) * az2 - (k3 * az4 + k4) * az4;
 }
 
+// Bethe-Heitler parameters above 2 MeV (ProxyBetheHeitler.hpp:80-104): the
+// nuclear term FZ = 8 ln(Z)/3 (+ 8 f_c(Z) above 50 MeV) and the screening
+// variable's factor, delta = screenfac / (eps (1 - eps)).
+static inline void bh_setup(int Z, double e0, double &FZ, double &screenfac) {
+  const double logZ3 = log_cw(1.0 * Z) / 3.0;
+  FZ = 8.0 * logZ3;
+  if (e0 > 50.0) FZ += 8.0 * coulomb_factor(Z);
+  const double Z3 = pow_cw(1.0 * Z, 1.0 / 3.0);
+  screenfac = 136.0 * (kMe / e0) / Z3;
+}
+
+// Unnormalised energy-sharing density the pair sampler draws eps from above
+// 2 MeV: (eps - 1/2)^2 max(F1, 0) + max(F2, 0) / 2 with F1 = screen1(delta) - FZ,
+// F2 = screen2(delta) - FZ (the two rejection branches of ProxyBetheHeitler.hpp:
+// 106-126 with their weights NormF1, NormF2).
+static double bh_density(int Z, double e0, double eps) {
+  double FZ, screenfac;
+  bh_setup(Z, e0, FZ, screenfac);
+  const double sv = screenfac / (eps * (1.0 - eps));
+  const double F1 = std::max(screen1(sv) - FZ, 0.0), F2 = std::max(screen2(sv) - FZ, 0.0);
+  return (eps - 0.5) * (eps - 0.5) * F1 + 0.5 * F2;
+}
+
 // Bethe-Heitler pair production (proxy/ProxyBetheHeitler.hpp:64-182) with
 // the element chosen from the material and the Tsai angle used as an
 // angle, cos/sin of u*m/E (the reference uses u*m/E as the cosine).
@@ -1162,11 +1185,8 @@ static void sample_pair(const Material &m, double e0, const double d0[3], uint32
   if (e0 < 2.0) {
     eps = eps0 + (0.5 - eps0) * mrg_uniform(s);
   } else {
-    const double logZ3 = log_cw(1.0 * Z) / 3.0;
-    double FZ = 8.0 * logZ3;
-    if (e0 > 50.0) FZ += 8.0 * coulomb_factor(Z);
-    const double Z3 = pow_cw(1.0 * Z, 1.0 / 3.0);
-    const double screenfac = 136.0 * eps0 / Z3;
+    double FZ, screenfac;
+    bh_setup(Z, e0, FZ, screenfac);
     const double screenmax = exp_cw((42.24 - FZ) / 8.368) - 0.952;
     const double screenmin = std::min(4.0 * screenfac, screenmax);
     const double eps1 = 0.5 - 0.5 * std::sqrt(1.0 - screenmin / screenmax);
@@ -2404,6 +2424,64 @@ int gxref_helix_step(const double field[3], const double *charge, const double *
 int gxref_rotate_to_lab(double *u, double *v, double *w, const double *u1, const double *u2, const double *u3,
                         int n) {
   for (int i = 0; i < n; ++i) rotate_to_lab(u[i], v[i], w[i], u1[i], u2[i], u3[i]);
+  return GXT_OK;
+}
+
+// The loop's straight-line navigation of the analytic geometry
+// (NavigationInterface::FindNextBoundary, geometry/NavigationInterface.hpp:64-98,
+// for slabs and boxes), same contract as gxt_slab_distance.
+int gxref_slab_distance(gxref_ctx *c, const double *pos, const double *dir, const double *pstep, int32_t *vol,
+                        double *snext, int32_t *next, double *saf, int32_t *boundary, int n) {
+  if (!c || !pos || !dir || !vol || !snext || !next || !saf || n < 0) return fail(GXT_EINVAL, "bad slab request");
+  const Geometry &g = c->geo;
+  for (int i = 0; i < n; ++i)
+    if (vol[i] >= g.nvol()) return fail(GXT_EINVAL, "volume out of range");
+  for (int i = 0; i < n; ++i) {
+    const double p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
+    const double d[3] = {dir[i], dir[n + i], dir[2 * n + i]};
+    int v = vol[i] >= 0 ? vol[i] : locate(g, p, d);
+    vol[i] = v;
+    if (v < 0) {
+      snext[i] = 0.0;
+      saf[i] = 0.0;
+      next[i] = -1;
+      if (boundary) boundary[i] = 1;
+      continue;
+    }
+    int face;
+    const double s = dist_out(g, v, p, d, face);
+    int nv = -1;
+    if (face == 0 || face == 1) {
+      nv = face == 1 ? v + 1 : v - 1;
+      if (nv >= g.nvol()) nv = -1;
+    }
+    snext[i] = s;
+    next[i] = nv;
+    saf[i] = safety(g, v, p);
+    if (boundary) boundary[i] = pstep ? !(pstep[i] < s) : 1;
+  }
+  return GXT_OK;
+}
+
+// The pair sampler's energy-sharing density above 2 MeV (test hook).
+int gxref_bh_density(int Z, double e0, const double *eps, double *out, int n) {
+  if (Z < 1 || Z > 100 || !(e0 >= 2.0)) return fail(GXT_EINVAL, "bad Bethe-Heitler density request");
+  for (int i = 0; i < n; ++i) out[i] = bh_density(Z, e0, eps[i]);
+  return GXT_OK;
+}
+
+// Along-step pieces of the oracle's e-/e+ step (test hook): the continuous-loss
+// step limit of kinetic energy e[i], and the energy after a step of step[i]
+// with its loss (ProxyIonization.hpp:108-168 under the physics contract).
+int gxref_along_step(gxref_ctx *c, int slot, int species, const double *e, const double *step, double *limit,
+                     double *fin, double *eloss, int n) {
+  if (!slot_ok(c, slot) || species < 1 || species > 2) return fail(GXT_EINVAL, "bad along-step request");
+  const Material &m = c->phys.mat[slot];
+  const double mine = c->phys.tracking_cut;
+  for (int i = 0; i < n; ++i) {
+    limit[i] = along_limit(m, species == 2, e[i], mine);
+    fin[i] = along_doit(m, species == 2, e[i], step[i], mine, eloss[i]);
+  }
   return GXT_OK;
 }
 

@@ -98,6 +98,17 @@ int gxref_mathfn(int which, const double *x, const double *y, double *out, int n
 int gxref_sincos(const double *x, double *s, double *c, int n);
 /* Sandia photo-absorption macroscopic cross-section (1/mm). */
 int gxref_sandia_xs(gxref_ctx *ctx, int material_slot, const double *ekin, double *out, int n);
+/* The pair sampler's unnormalised energy-sharing density at eps[i] for a
+ * photon of e0 >= 2 MeV on element Z (test hook). */
+int gxref_bh_density(int Z, double e0, const double *eps, double *out, int n);
+/* Straight-line navigation of the analytic geometry, same contract as gxt_slab_distance. */
+int gxref_slab_distance(gxref_ctx *ctx, const double *pos, const double *dir, const double *pstep, int32_t *vol,
+                        double *snext, int32_t *next, double *safety, int32_t *boundary, int n);
+/* Along-step pieces of the oracle's e-/e+ step (test hook; the library has them
+ * only inside its transport kernel): the continuous-loss step limit at e[i],
+ * and the kinetic energy after a step of step[i] with its loss. */
+int gxref_along_step(gxref_ctx *ctx, int material_slot, int species, const double *e, const double *step,
+                     double *limit, double *fin, double *eloss, int n);
 
 #ifdef __cplusplus
 }

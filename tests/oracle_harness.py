@@ -106,6 +106,31 @@ class Oracle:
     def sample(self, model, slot, ekin, dirs, states):
         return sample_with(lambda io: lib().gxref_sample(self.ctx, model, slot, C.byref(io)), ekin, dirs, states)
 
+    def slab_distance(self, pos, dirs, vol=None, pstep=None):
+        """(vol, snext, next, safety, boundary), the contract of gxt_slab_distance."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        dirs = np.ascontiguousarray(dirs, dtype=np.float64)
+        n = pos.shape[1]
+        v = np.full(n, -1, dtype=np.int32) if vol is None else np.ascontiguousarray(vol, dtype=np.int32).copy()
+        ps = None if pstep is None else np.ascontiguousarray(pstep, dtype=np.float64)
+        s, saf = np.zeros(n), np.zeros(n)
+        nxt = np.zeros(n, dtype=np.int32)
+        b = np.zeros(n, dtype=np.int32)
+        check(lib().gxref_slab_distance(self.ctx, abi.ptr(pos), abi.ptr(dirs), None if ps is None else abi.ptr(ps),
+                                        abi.ptr(v, C.c_int32), abi.ptr(s), abi.ptr(nxt, C.c_int32), abi.ptr(saf),
+                                        abi.ptr(b, C.c_int32), n))
+        return v, s, nxt, saf, b
+
+    def along_step(self, slot, species, e, step):
+        """(limit, fin, eloss): the oracle's continuous-loss step limit at e and the
+        energy after a step of length step, with its loss."""
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        step = np.ascontiguousarray(np.broadcast_to(step, e.shape), dtype=np.float64)
+        lim, fin, loss = np.zeros_like(e), np.zeros_like(e), np.zeros_like(e)
+        check(lib().gxref_along_step(self.ctx, slot, species, abi.ptr(e), abi.ptr(step), abi.ptr(lim), abi.ptr(fin),
+                                     abi.ptr(loss), len(e)))
+        return lim, fin, loss
+
 
 def sample_with(call, ekin, dirs, states):
     """Run a sampler entry point on SoA inputs; returns a dict of outputs and

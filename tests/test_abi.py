@@ -145,3 +145,36 @@ def test_host_gun_matches_oracle():
         assert x.ekin == y.ekin and list(x.pos) == list(y.pos) and x.substream == y.substream
     e = [x.ekin for x in a]
     assert 1000.0 <= min(e) and max(e) <= 50000.0
+
+
+def test_submit_checks_primaries_without_device():
+    """gxt_submit rejects a bad primary before anything is queued (and before
+    it needs a context): non-finite or non-positive energy, non-finite
+    position or direction, a direction that is not a unit vector, a bad
+    species or a negative event index."""
+    lib = C.CDLL(LIB)
+    lib.gxt_last_error.restype = C.c_char_p
+
+    def submit(**kw):
+        p = abi.primaries_array(2)
+        for q in p:
+            q.species, q.ekin = abi.ELECTRON, 100.0
+            q.dir[:] = (0.0, 0.0, 1.0)
+        for k, v in kw.items():
+            if k in ("pos", "dir"):
+                getattr(p[1], k)[:] = v
+            else:
+                setattr(p[1], k, v)
+        return lib.gxt_submit(None, p, 2, C.c_uint64(1))
+
+    cases = [(dict(ekin=float("inf")), b"ekin"), (dict(ekin=0.0), b"ekin"), (dict(ekin=float("nan")), b"ekin"),
+             (dict(pos=(float("nan"), 0.0, 0.0)), b"finite"), (dict(dir=(0.0, float("inf"), 0.0)), b"finite"),
+             (dict(dir=(0.0, 0.0, 2.0)), b"unit"), (dict(dir=(0.0, 0.0, 1.0 + 1e-5)), b"unit"),
+             (dict(species=3), b"species"), (dict(event=-1), b"event")]
+    for kw, needle in cases:
+        assert submit(**kw) == -1, kw
+        msg = lib.gxt_last_error()
+        assert needle in msg and b"primary 1" in msg, (kw, msg)
+    # good primaries pass the checks and stop at the missing context
+    assert submit() == -1 and b"null context" in lib.gxt_last_error()
+    assert submit(dir=(0.6, 0.0, 0.8)) == -1 and b"null context" in lib.gxt_last_error()

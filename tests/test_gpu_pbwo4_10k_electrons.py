@@ -61,6 +61,49 @@ def test_pbwo4_10k_electrons_fp64(gpu_lib):
     _compare_events({k: v[idx] for k, v in ev.items()}, {k: v[idx] for k, v in cpu.items()})
 
 
+def test_pbwo4_profiles_spectra_and_events_vs_oracle(gpu_lib):
+    """The first 2,400 events of the batch on the GPU (FP64 and FP32) and on
+    the oracle (FP64, host threads): longitudinal and radial deposit profiles,
+    secondary spectra (3 species x 64 bins of log10 E) and per-event
+    distributions.  FP64: profiles within 1e-9 per bin, spectra equal (the
+    same substreams give the same histories).  FP32: profile bins holding
+    more than 1 % of the deposit within 1 %; two-sample KS p > 0.01 on the
+    per-event deposit, charged length and step count; each species' spectrum
+    compatible with the oracle's (chi-square p > 0.01)."""
+    from scipy.stats import chi2_contingency, ks_2samp
+    m = 2400
+    _flush("C1 first %d events: profiles, spectra, KS against the oracle" % m)
+    prims = gpu_lib.make_primaries(777, abi.ELECTRON, E0, E0, 0.0, m, first=0, n_total=N)
+    out = {}
+    for prec in (abi.FP64, abi.FP32):
+        cfg = abi.config_pbwo4_block(max_events=m, precision=prec)
+        with gpu_lib.Transport(cfg) as t:
+            t.submit(prims, 1)
+            t.run()
+            out[prec] = (t.event_tallies(1, m), t.profiles(), t.spectra())
+    ref = O.Oracle(abi.config_pbwo4_block(max_events=m))
+    ref.run(O.make_primaries(777, abi.ELECTRON, E0, E0, 0.0, m, first=0, n_total=N), nthreads=16)
+    cpu, cprof, cspec = ref.event_tallies(m), ref.profiles(1), ref.spectra()
+    ref.close()
+    g64, p64, s64 = out[abi.FP64]
+    for a, b in zip(p64[:2], cprof[:2]):
+        np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-9)
+    assert np.array_equal(s64, cspec)
+    g32, p32, s32 = out[abi.FP32]
+    for a, b in zip(p32[:2], cprof[:2]):
+        big = b > 0.01 * b.sum()
+        np.testing.assert_allclose(a[big], b[big], rtol=1e-2)
+    for k in ("edep", "charged_length", "n_steps"):
+        p = ks_2samp(np.round(g32[k], 6), np.round(cpu[k], 6)).pvalue
+        _flush("  KS fp32 %s: p = %.3g" % (k, p))
+        assert p > 0.01, k
+    for s in range(3):
+        keep = (s32[s] + cspec[s]) > 0
+        p = chi2_contingency(np.array([s32[s][keep], cspec[s][keep]]))[1]
+        _flush("  spectrum species %d: chi2 p = %.3g" % (s, p))
+        assert p > 0.01, s
+
+
 def test_pbwo4_10k_electrons_fp32_vs_fp64(gpu_lib):
     _flush("full size: 10000 x 10 GeV e- in PbWO4, FP32 against FP64")
     _, e64, _ = _run(gpu_lib, abi.FP64)
