@@ -51,6 +51,13 @@ class Primary(C.Structure):
                 ("dir", C.c_double * 3), ("substream", C.c_uint64)]
 
 
+class Track(C.Structure):
+    """A track in flight (gxt_track): the hand-off state of a basket."""
+    _fields_ = [("species", C.c_int32), ("event", C.c_int32), ("ekin", C.c_double), ("pos", C.c_double * 3),
+                ("dir", C.c_double * 3), ("nil", C.c_double * 3), ("rng", C.c_uint32 * 6), ("volume", C.c_int32),
+                ("nsteps", C.c_int32), ("substream", C.c_uint64)]
+
+
 class EventTally(C.Structure):
     _fields_ = [("edep", C.c_double), ("visible", C.c_double), ("leak", C.c_double),
                 ("charged_length", C.c_double), ("n_secondaries", C.c_int64 * 3), ("n_steps", C.c_int64),
@@ -214,6 +221,10 @@ def config_calorimeter_cms(precision=FP64, stepper=STEPPER_HELIX, **kw):
 
 def primaries_array(n):
     return (Primary * n)()
+
+
+def tracks_array(n):
+    return (Track * n)()
 
 
 def tallies_to_numpy(t):

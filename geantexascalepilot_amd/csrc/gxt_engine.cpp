@@ -186,9 +186,11 @@ double now_ms(std::chrono::steady_clock::time_point t0) {
 
 struct Job {
   uint64_t id = 0;
-  std::vector<gxt_primary> prims;   // sorted by event
+  std::vector<gxt_primary> prims;   // sorted by event (a job of primaries)
   gxt_primary *dprims = nullptr;    // device copy of prims (made at submission, freed when the job ends)
-  std::vector<int> ev_first;        // prims[ev_first[e] .. ev_first[e+1]) belong to event e
+  std::vector<gxt_track> tracks;    // sorted by event (a job of tracks in flight)
+  gxt_track *dtracks = nullptr;     // device copy of tracks
+  std::vector<int> ev_first;        // entries [ev_first[e] .. ev_first[e+1]) belong to event e
   std::vector<double> ev_gev;       // summed primary energy of each event (GeV)
   std::vector<int> ev_max;          // highest live-track count seen for an event in flight
   std::vector<char> ev_state;       // 0 pending, 1 in flight, 2 finished
@@ -349,7 +351,8 @@ int alloc_events(gxt_ctx *c, int n) {
 int admit(gxt_ctx *c) {
   const double budget = 0.8 * (double)c->cap;
   struct Range {
-    const gxt_primary *prim;
+    const gxt_primary *prim;  // a job of primaries, or
+    const gxt_track *trk;     // of tracks in flight
     int n, event_base;
   };
   std::vector<Range> ranges;
@@ -385,7 +388,8 @@ int admit(gxt_ctx *c) {
       }
       if (j->next_event > e0) {
         const int p0 = j->ev_first[e0], p1 = j->ev_first[j->next_event];
-        ranges.push_back(Range{j->dprims + p0, p1 - p0, j->ev_base});
+        ranges.push_back(Range{j->dprims ? j->dprims + p0 : nullptr, j->dtracks ? j->dtracks + p0 : nullptr,
+                               p1 - p0, j->ev_base});
       }
       if (full) break;
     }
@@ -403,10 +407,12 @@ int admit(gxt_ctx *c) {
     hipError_t e;
     if (c->precision == GXT_FP32) {
       PoolArgs<float> pa{c->pf[c->parity].p, c->pf[1 - c->parity].p};
-      e = launch_inject<float>(a, pa, r.prim, r.n, r.event_base, c->stream);
+      e = r.prim ? launch_inject<float>(a, pa, r.prim, r.n, r.event_base, c->stream)
+                 : launch_inject<float>(a, pa, r.trk, r.n, r.event_base, c->stream);
     } else {
       PoolArgs<double> pa{c->pd[c->parity].p, c->pd[1 - c->parity].p};
-      e = launch_inject<double>(a, pa, r.prim, r.n, r.event_base, c->stream);
+      e = r.prim ? launch_inject<double>(a, pa, r.prim, r.n, r.event_base, c->stream)
+                 : launch_inject<double>(a, pa, r.trk, r.n, r.event_base, c->stream);
     }
     if (e != hipSuccess) return c->fail(GXT_EHIP, std::string("inject: ") + hipGetErrorString(e));
   }
@@ -536,9 +542,13 @@ int collect(gxt_ctx *c, bool &idle) {
         c->jobs.erase(it);
         // its injections ran before the snapshot that showed it finished
         if (owned->dprims) HIP_TRY(hipFreeAsync(owned->dprims, c->stream));
+        if (owned->dtracks) HIP_TRY(hipFreeAsync(owned->dtracks, c->stream));
         owned->dprims = nullptr;
+        owned->dtracks = nullptr;
         owned->prims.clear();
         owned->prims.shrink_to_fit();
+        owned->tracks.clear();
+        owned->tracks.shrink_to_fit();
         c->done_unpolled.emplace_back(new Job(*owned));  // completion order for gxt_poll
         c->finished[owned->id] = std::move(owned);
         break;
@@ -627,8 +637,10 @@ void destroy_ctx(gxt_ctx *c) {
   c->dd.release();
   release_field_map(c->fmap);
   if (c->snap) (void)hipEventDestroy(c->snap);
-  for (auto &jp : c->jobs)
+  for (auto &jp : c->jobs) {
     if (jp->dprims) (void)hipFree(jp->dprims);
+    if (jp->dtracks) (void)hipFree(jp->dtracks);
+  }
   (void)hipFree(c->queues);
   (void)hipFree(c->newlist);
   (void)hipFree(c->cnt);
@@ -687,6 +699,59 @@ int rng_params(const DevParams **out) {
   }
   *out = g_rng_dd.P;
   return GXT_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// Queue job j whose entries (primaries or tracks, src[0..n)) span n_events
+// events: sort them by event, index the events, copy the entries to the
+// device on the context's stream (ahead of any admission of the job, which
+// follows it in stream order; the host copy stays alive until the job ends).
+template <class In>
+int queue_job(gxt_ctx *c, std::vector<In> &host, In *&dev, std::unique_ptr<Job> j, const In *src, int n,
+              int n_events) {
+  host.assign(src, src + n);
+  j->n_events = n_events;
+  std::stable_sort(host.begin(), host.end(), [](const In &a, const In &b) { return a.event < b.event; });
+  j->ev_first.assign(j->n_events + 1, 0);
+  for (const In &p : host) j->ev_first[p.event + 1]++;
+  for (int e = 0; e < j->n_events; ++e) j->ev_first[e + 1] += j->ev_first[e];
+  j->ev_gev.assign(j->n_events, 0.0);
+  j->ev_max.assign(j->n_events, 0);
+  for (const In &p : host) j->ev_gev[p.event] += p.ekin * 1.0e-3;
+  j->ev_state.assign(j->n_events, 0);
+  j->submit_ms = now_ms(c->t0);
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (c->finished.count(j->id)) return c->fail(GXT_EINVAL, "job id already used");
+  for (const auto &q : c->jobs)
+    if (q->id == j->id) return c->fail(GXT_EINVAL, "job id already used");
+  const size_t bytes = host.size() * sizeof(In);
+  HIP_TRY(hipMallocAsync((void **)&dev, bytes, c->stream));
+  const hipError_t ce = hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, c->stream);
+  if (ce != hipSuccess) {
+    (void)hipFreeAsync(dev, c->stream);
+    dev = nullptr;
+    return set_err(GXT_EHIP, std::string("submit copy: ") + hipGetErrorString(ce));
+  }
+  c->jobs.push_back(std::move(j));
+  return GXT_OK;
+}
+
+// A kinematic state a job may carry: finite energy > 0, finite position,
+// unit direction.
+template <class In>
+std::string check_kinematics(const In &p) {
+  if (p.species < 0 || p.species > 2) return "bad species";
+  if (!(p.ekin > 0) || !std::isfinite(p.ekin)) return "ekin must be finite and > 0";
+  if (p.event < 0) return "negative event index";
+  bool finite = true;
+  for (int k = 0; k < 3; ++k) finite = finite && std::isfinite(p.pos[k]) && std::isfinite(p.dir[k]);
+  if (!finite) return "non-finite position or direction";
+  const double nn = p.dir[0] * p.dir[0] + p.dir[1] * p.dir[1] + p.dir[2] * p.dir[2];
+  if (!(std::fabs(nn - 1.0) <= 2e-6)) return "direction is not a unit vector";
+  return "";
 }
 
 }  // namespace
@@ -851,50 +916,58 @@ int gxt_submit(gxt_ctx *c, const gxt_primary *prim, int n, uint64_t job_id) {
   // an infinite energy would hold back admission for good
   int maxev = -1;
   for (int i = 0; i < n; ++i) {
-    const gxt_primary &p = prim[i];
-    const std::string at = " (primary " + std::to_string(i) + ")";
-    if (p.species < 0 || p.species > 2) return set_err(GXT_EINVAL, "bad species" + at);
-    if (!(p.ekin > 0) || !std::isfinite(p.ekin)) return set_err(GXT_EINVAL, "ekin must be finite and > 0" + at);
-    if (p.event < 0) return set_err(GXT_EINVAL, "negative event index" + at);
-    bool finite = true;
-    for (int k = 0; k < 3; ++k) finite = finite && std::isfinite(p.pos[k]) && std::isfinite(p.dir[k]);
-    if (!finite) return set_err(GXT_EINVAL, "non-finite position or direction" + at);
-    const double nn = p.dir[0] * p.dir[0] + p.dir[1] * p.dir[1] + p.dir[2] * p.dir[2];
-    if (!(std::fabs(nn - 1.0) <= 2e-6)) return set_err(GXT_EINVAL, "direction is not a unit vector" + at);
-    maxev = std::max(maxev, p.event);
+    const std::string bad = check_kinematics(prim[i]);
+    if (!bad.empty()) return set_err(GXT_EINVAL, bad + " (primary " + std::to_string(i) + ")");
+    maxev = std::max(maxev, prim[i].event);
   }
   if (!c) return set_err(GXT_EINVAL, "null context");
   if (maxev >= c->cfg.max_events) return c->fail(GXT_ECAPACITY, "job has more events than max_events");
   auto j = std::make_unique<Job>();
   j->id = job_id;
-  j->prims.assign(prim, prim + n);
-  j->n_events = maxev + 1;
-  std::stable_sort(j->prims.begin(), j->prims.end(),
-                   [](const gxt_primary &a, const gxt_primary &b) { return a.event < b.event; });
-  j->ev_first.assign(j->n_events + 1, 0);
-  for (const gxt_primary &p : j->prims) j->ev_first[p.event + 1]++;
-  for (int e = 0; e < j->n_events; ++e) j->ev_first[e + 1] += j->ev_first[e];
-  j->ev_gev.assign(j->n_events, 0.0);
-  j->ev_max.assign(j->n_events, 0);
-  for (const gxt_primary &p : j->prims) j->ev_gev[p.event] += p.ekin * 1.0e-3;
-  j->ev_state.assign(j->n_events, 0);
-  j->submit_ms = now_ms(c->t0);
-  std::lock_guard<std::mutex> lk(c->mu);
-  if (c->finished.count(job_id)) return c->fail(GXT_EINVAL, "job id already used");
-  for (const auto &q : c->jobs)
-    if (q->id == job_id) return c->fail(GXT_EINVAL, "job id already used");
-  // the primaries go to the device now, on the context's stream, ahead of
-  // any admission of the job (which follows it in stream order); the host
-  // copy stays alive until the job ends
-  const size_t bytes = j->prims.size() * sizeof(gxt_primary);
-  HIP_TRY(hipMallocAsync((void **)&j->dprims, bytes, c->stream));
-  const hipError_t ce = hipMemcpyAsync(j->dprims, j->prims.data(), bytes, hipMemcpyHostToDevice, c->stream);
-  if (ce != hipSuccess) {
-    (void)hipFreeAsync(j->dprims, c->stream);
-    return set_err(GXT_EHIP, std::string("submit copy: ") + hipGetErrorString(ce));
+  Job *jr = j.get();
+  return queue_job(c, jr->prims, jr->dprims, std::move(j), prim, n, maxev + 1);
+}
+
+int gxt_submit_tracks(gxt_ctx *c, const gxt_track *trk, int n, int n_events, uint64_t job_id) {
+  if (!trk || n <= 0 || n_events <= 0) return set_err(GXT_EINVAL, "bad submit_tracks arguments");
+  // what a track carries is checked first, then what depends on the context
+  // (the step limit, the geometry)
+  for (int i = 0; i < n; ++i) {
+    const gxt_track &t = trk[i];
+    std::string bad = check_kinematics(t);
+    if (bad.empty() && t.event >= n_events) bad = "event index >= n_events";
+    if (bad.empty() && !(std::isfinite(t.nil[0]) && std::isfinite(t.nil[1]) && std::isfinite(t.nil[2])))
+      bad = "non-finite number of interaction lengths";
+    if (bad.empty() && t.nsteps < 0) bad = "negative nsteps";
+    if (bad.empty()) {
+      const bool z1 = t.rng[0] == 0 && t.rng[1] == 0 && t.rng[2] == 0;
+      const bool z2 = t.rng[3] == 0 && t.rng[4] == 0 && t.rng[5] == 0;
+      bool range = true;
+      for (int k = 0; k < 3; ++k) range = range && t.rng[k] < 4294967087u && t.rng[3 + k] < 4294944443u;
+      if (z1 || z2 || !range) bad = "invalid MRG32k3a state";
+    }
+    if (!bad.empty()) return set_err(GXT_EINVAL, bad + " (track " + std::to_string(i) + ")");
   }
-  c->jobs.push_back(std::move(j));
-  return GXT_OK;
+  if (!c) return set_err(GXT_EINVAL, "null context");
+  if (n_events > c->cfg.max_events) return c->fail(GXT_ECAPACITY, "job has more events than max_events");
+  const DevGeometry &g = c->hd.params.geo;
+  for (int i = 0; i < n; ++i) {
+    const gxt_track &t = trk[i];
+    std::string bad;
+    if (t.nsteps >= c->cfg.max_steps) bad = "nsteps >= max_steps";
+    if (bad.empty() && t.volume != -1) {
+      if (t.volume < 0 || t.volume >= g.nvol)
+        bad = "volume out of range";
+      else if (!(std::fabs(t.pos[0]) <= g.hx && std::fabs(t.pos[1]) <= g.hy && t.pos[2] >= g.z[t.volume] &&
+                 t.pos[2] <= g.z[t.volume + 1]))
+        bad = "position outside its volume";
+    }
+    if (!bad.empty()) return c->fail(GXT_EINVAL, bad + " (track " + std::to_string(i) + ")");
+  }
+  auto j = std::make_unique<Job>();
+  j->id = job_id;
+  Job *jr = j.get();
+  return queue_job(c, jr->tracks, jr->dtracks, std::move(j), trk, n, n_events);
 }
 
 int gxt_run(gxt_ctx *c, int64_t max_iterations) {

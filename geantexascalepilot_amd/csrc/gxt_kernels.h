@@ -72,9 +72,10 @@ constexpr int kStepEvents = 10;
 template <class R>
 hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const StepStreams &s,
                        hipEvent_t *ev = nullptr);
-template <class R>
-// Injects n primaries from device memory; event slots are prim[i].event + event_base.
-hipError_t launch_inject(const StepArgs &a, const PoolArgs<R> &pa, const gxt_primary *prim, int n, int event_base,
+// Injects n primaries or tracks in flight (In = gxt_primary, gxt_track) from
+// device memory; event slots are prim[i].event + event_base.
+template <class R, class In>
+hipError_t launch_inject(const StepArgs &a, const PoolArgs<R> &pa, const In *prim, int n, int event_base,
                          hipStream_t st);
 
 hipError_t launch_rng_init(const DevParams *P, const uint32_t *seed, const uint64_t *idx, int n, uint32_t *out,

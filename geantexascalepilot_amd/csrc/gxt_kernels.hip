@@ -416,8 +416,40 @@ __device__ __forceinline__ void gpil(const DevParams *P, const Nodes<R> &N, cons
 }
 
 // ------------------------------------------------------------------ step loop kernels
+// The state a primary starts from: no interaction lengths yet, the
+// substream's initial RNG state, located from its position and direction.
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, const gxt_primary *prim, int n,
+__device__ __forceinline__ void inject_state(const DevParams *P, const StepArgs &a, const gxt_primary &p,
+                                             TrackRegs<R> &t) {
+  t.nil0 = t.nil1 = t.nil2 = R(-1);
+  t.nsteps = 0;
+  t.vol = locate(P->geo, t.pos, p.dir[0], p.dir[1], p.dir[2]);
+  if (t.vol >= 0) substream_state(P, t.uid, t.rng.s);
+}
+
+// A track in flight continues from its own interaction lengths, RNG state,
+// volume (located when not given) and step count; the event's step tally
+// is credited with -nsteps now, so that the track's full count added when it
+// dies leaves only the steps taken here (modulo 2^64 arithmetic).
+template <class R>
+__device__ __forceinline__ void inject_state(const DevParams *P, const StepArgs &a, const gxt_track &p,
+                                             TrackRegs<R> &t) {
+  t.nil0 = (R)p.nil[0];
+  t.nil1 = (R)p.nil[1];
+  t.nil2 = (R)p.nil[2];
+  t.nsteps = p.nsteps;
+  t.vol = p.volume >= 0 ? p.volume : locate(P->geo, t.pos, p.dir[0], p.dir[1], p.dir[2]);
+  if (t.vol >= 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) t.rng.s[k] = (double)p.rng[k];
+    if (p.nsteps > 0) atomicAdd(&a.ev[t.event].nsteps, (unsigned long long)(-(long long)p.nsteps));
+  }
+}
+
+// Entries In (gxt_primary or gxt_track) become live tracks of the current
+// pool; their event slots are entry.event + event_base.
+template <class R, class In>
+__global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, const In *prim, int n,
                                                    int event_base) {
   const DevParams *P = a.P;
   const Pool<R> &cur = pa.cur;
@@ -428,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
     TrackRegs<R> t;
     int region = -1;
     if (active) {
-      const gxt_primary &p = prim[i];
+      const In &p = prim[i];
       t.species = p.species;
       t.event = p.event + event_base;
       t.e = (R)p.ekin;
@@ -436,15 +468,12 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
         t.pos[k] = p.pos[k];
         t.dir[k] = (R)p.dir[k];
       }
-      t.nil0 = t.nil1 = t.nil2 = R(-1);
-      t.nsteps = 0;
       t.aed = t.avis = t.alen = 0.0;
       t.uid = p.substream;
-      t.vol = locate(P->geo, t.pos, p.dir[0], p.dir[1], p.dir[2]);
+      inject_state(P, a, p, t);
       if (t.vol < 0) {
         unsafeAtomicAdd(&a.ev[t.event].leak, p.ekin);
       } else {
-        substream_state(P, t.uid, t.rng.s);
         region = t.species == 0 ? 0 : 1;
         atomicAdd(&a.ev_live[t.event], 1);
       }
@@ -1181,13 +1210,13 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
   return hipGetLastError();
 }
 
-template <class R>
-hipError_t launch_inject(const StepArgs &a, const PoolArgs<R> &pa, const gxt_primary *prim, int n, int event_base,
+template <class R, class In>
+hipError_t launch_inject(const StepArgs &a, const PoolArgs<R> &pa, const In *prim, int n, int event_base,
                          hipStream_t st) {
   if (n <= 0) return hipSuccess;
   int g = grid_for(n);
   if (g > 1024) g = 1024;
-  hipLaunchKernelGGL(k_inject<R>, dim3(g), dim3(kBlock), 0, st, a, pa, prim, n, event_base);
+  hipLaunchKernelGGL((k_inject<R, In>), dim3(g), dim3(kBlock), 0, st, a, pa, prim, n, event_base);
   return hipGetLastError();
 }
 
@@ -1195,10 +1224,14 @@ template hipError_t launch_step<double>(const StepArgs &, const PoolArgs<double>
                                         hipEvent_t *);
 template hipError_t launch_step<float>(const StepArgs &, const PoolArgs<float> &, int, const StepStreams &,
                                        hipEvent_t *);
-template hipError_t launch_inject<double>(const StepArgs &, const PoolArgs<double> &, const gxt_primary *, int, int,
-                                          hipStream_t);
-template hipError_t launch_inject<float>(const StepArgs &, const PoolArgs<float> &, const gxt_primary *, int, int,
-                                         hipStream_t);
+template hipError_t launch_inject<double, gxt_primary>(const StepArgs &, const PoolArgs<double> &,
+                                                       const gxt_primary *, int, int, hipStream_t);
+template hipError_t launch_inject<float, gxt_primary>(const StepArgs &, const PoolArgs<float> &, const gxt_primary *,
+                                                      int, int, hipStream_t);
+template hipError_t launch_inject<double, gxt_track>(const StepArgs &, const PoolArgs<double> &, const gxt_track *,
+                                                     int, int, hipStream_t);
+template hipError_t launch_inject<float, gxt_track>(const StepArgs &, const PoolArgs<float> &, const gxt_track *, int,
+                                                    int, hipStream_t);
 
 hipError_t launch_rng_init(const DevParams *P, const uint32_t *seed, const uint64_t *idx, int n, uint32_t *out,
                            hipStream_t st) {

@@ -299,6 +299,10 @@ class Transport:
     def submit(self, prims, job_id):
         self._c(lib.gxt_submit(self.ctx, prims, len(prims), C.c_uint64(job_id)))
 
+    def submit_tracks(self, tracks, n_events, job_id):
+        """A basket of tracks in flight (abi.Track array) over n_events events."""
+        self._c(lib.gxt_submit_tracks(self.ctx, tracks, len(tracks), n_events, C.c_uint64(job_id)))
+
     def run(self, max_iterations=0):
         self._c(lib.gxt_run(self.ctx, C.c_int64(max_iterations)))
 

@@ -218,6 +218,45 @@ int gxt_egamma_gun(uint32_t seed, int n_events, int per_event, double gamma_frac
  * the call); the job is admitted into the shared pool when its estimated
  * footprint fits. */
 int gxt_submit(gxt_ctx *ctx, const gxt_primary *prim, int n, uint64_t job_id);
+/* One track in flight, in the form the reference's CPU transport would hand a
+ * basket over to the device at its "memory transfer to GPU" point of DoStep<>
+ * (proxy/test/test_BasicCpuTransport.cpp:527-531): the parts of TrackState
+ * (track/TrackState.hpp:177-232) that decide how the track continues, plus
+ * its own RNG state (per-track substreams; the reference keeps the RNG in
+ * each process object). */
+typedef struct gxt_track {
+  int32_t species;      /* 0 gamma, 1 e-, 2 e+                                      */
+  int32_t event;        /* event index inside its job                               */
+  double ekin;          /* kinetic energy (MeV), fPhysicsState.fEkin                */
+  double pos[3];        /* mm, fPos                                                 */
+  double dir[3];        /* unit vector, fDir                                        */
+  double nil[3];        /* interaction lengths left to the next discrete interaction of
+                           each process of the species (gamma: Compton, conversion,
+                           photoelectric; e-: brems, ionisation; e+: brems, ionisation,
+                           annihilation), fPhysicsProcessState[p].fNumOfInteractLengthLeft
+                           (TrackState.hpp:126-135); <= 0: sampled afresh at the next step,
+                           as after a DoIt (ProxyEmProcess.hpp:122-163)      */
+  uint32_t rng[6];      /* the track's MRG32k3a state: words 0-2 < m1, 3-5 < m2,
+                           neither triple all zero                                  */
+  int32_t volume;       /* volume index of the analytic geometry (fGeometryState), or -1:
+                           located from pos and dir like a primary                  */
+  int32_t nsteps;       /* steps taken so far (fHistoryState.fNsteps); the step limit
+                           max_steps counts them; the event's step tally counts only the
+                           steps taken here                                         */
+  uint64_t substream;   /* the track's substream index (its identity in step traces) */
+} gxt_track;
+
+/* Queue a job of n tracks in flight (a basket of a mid-shower CPU transport)
+ * over n_events events: every track and all its descendants are transported
+ * to the end; the job's event tallies are those of this continuation (the
+ * deposits, steps and secondaries before the hand-off stay with the caller).
+ * A track continues exactly as it would have in the loop: with its own nil,
+ * RNG state, volume and step count.  Every track is checked (species, event
+ * in [0, n_events), finite ekin > 0, finite position, unit direction, finite
+ * nil, a valid RNG state, 0 <= nsteps < max_steps, a volume of the geometry
+ * that holds the position) before anything is queued.  No reference
+ * counterpart beyond the hand-off point cited above. */
+int gxt_submit_tracks(gxt_ctx *ctx, const gxt_track *tracks, int n, int n_events, uint64_t job_id);
 /* Run the step loop until every submitted job has finished (replaces
  * ProxyDeviceManager::DoStep/DoGPIL, ProxyDeviceManager.cpp:206-276, and the
  * host wave loop DoStep<>, proxy/test/test_BasicCpuTransport.cpp:517-550).

@@ -1991,9 +1991,21 @@ static void one_step(gxref_ctx &c, Worker &w, EventTally &ev, std::vector<Track>
   }
 }
 
-static void run_event(gxref_ctx &c, Worker &w, const gxt_primary &p) {
-  EventTally &ev = c.events[p.event];
-  std::vector<Track> stack;
+// Transport every track of an event's stack, and all it creates, to the end
+// (depth first).
+static void drain(gxref_ctx &c, Worker &w, EventTally &ev, std::vector<Track> &stack) {
+  while (!stack.empty()) {
+    Track cur = stack.back();
+    stack.pop_back();
+    bool alive = true;
+    while (alive) one_step(c, w, ev, stack, cur, alive);
+    if ((int64_t)stack.size() > w.maxlive) w.maxlive = (int64_t)stack.size();
+  }
+}
+
+// The track a primary starts as: no interaction lengths yet, its substream's
+// initial state, located from its position and direction (vol < 0: outside).
+static Track primary_track(const gxref_ctx &c, const gxt_primary &p) {
   Track t;
   t.species = p.species;
   t.event = p.event;
@@ -2007,18 +2019,55 @@ static void run_event(gxref_ctx &c, Worker &w, const gxt_primary &p) {
   t.uid = p.substream;
   substream_state(c.seed, t.uid, t.rng);
   t.vol = locate(c.geo, t.pos, t.dir);
+  return t;
+}
+
+static void run_event(gxref_ctx &c, Worker &w, const gxt_primary &p) {
+  EventTally &ev = c.events[p.event];
+  std::vector<Track> stack;
+  const Track t = primary_track(c, p);
   if (t.vol < 0) {
     ev.leak += t.e;
     return;
   }
   stack.push_back(t);
-  while (!stack.empty()) {
-    Track cur = stack.back();
-    stack.pop_back();
-    bool alive = true;
-    while (alive) one_step(c, w, ev, stack, cur, alive);
-    if ((int64_t)stack.size() > w.maxlive) w.maxlive = (int64_t)stack.size();
+  drain(c, w, ev, stack);
+}
+
+// A basket track in flight (gxt_track) as the loop's track: its own
+// interaction lengths, RNG state, step count and volume (located when -1).
+static Track basket_track(const gxref_ctx &c, const gxt_track &in) {
+  Track t;
+  t.species = in.species;
+  t.event = in.event;
+  t.nsteps = in.nsteps;
+  t.e = in.ekin;
+  for (int i = 0; i < 3; ++i) {
+    t.pos[i] = in.pos[i];
+    t.dir[i] = in.dir[i];
+    t.nil[i] = in.nil[i];
   }
+  for (int k = 0; k < 6; ++k) t.rng[k] = in.rng[k];
+  t.uid = in.substream;
+  t.vol = in.volume >= 0 ? in.volume : locate(c.geo, t.pos, t.dir);
+  return t;
+}
+
+static gxt_track to_basket(const Track &t) {
+  gxt_track o;
+  o.species = t.species;
+  o.event = t.event;
+  o.ekin = t.e;
+  for (int i = 0; i < 3; ++i) {
+    o.pos[i] = t.pos[i];
+    o.dir[i] = t.dir[i];
+    o.nil[i] = t.nil[i];
+  }
+  for (int k = 0; k < 6; ++k) o.rng[k] = t.rng[k];
+  o.volume = t.vol;
+  o.nsteps = t.nsteps;
+  o.substream = t.uid;
+  return o;
 }
 
 }  // namespace gxref
@@ -2274,6 +2323,125 @@ int gxref_run(gxref_ctx *c, const gxt_primary *prim, int n, int nthreads) {
     c->tot.killed += w.killed;
     c->tot.maxlive = std::max(c->tot.maxlive, w.maxlive);
   }
+  return GXT_OK;
+}
+
+// Continue a basket of tracks in flight (gxt_submit_tracks): the tracks of
+// an event go on one stack, which is transported to the end.
+int gxref_run_tracks(gxref_ctx *c, const gxt_track *trk, int n, int nthreads) {
+  if (!c || (n > 0 && !trk)) return fail(GXT_EINVAL, "null argument");
+  for (int i = 0; i < n; ++i)
+    if (trk[i].event < 0 || trk[i].event >= (int)c->events.size()) return fail(GXT_EINVAL, "event index out of range");
+  if (nthreads < 1) nthreads = 1;
+  std::vector<std::vector<int>> byev(c->events.size());
+  for (int i = 0; i < n; ++i) byev[trk[i].event].push_back(i);
+  std::vector<int> order;
+  for (size_t e = 0; e < byev.size(); ++e)
+    if (!byev[e].empty()) order.push_back((int)e);
+  std::atomic<size_t> next(0);
+  std::vector<Worker> ws(nthreads);
+  for (auto &w : ws) {
+    w.ez.assign(c->cfg.n_zbins, 0.0);
+    w.er.assign(c->cfg.n_rbins, 0.0);
+    w.ev.assign(c->geo.nvol(), 0.0);
+    w.spectra.assign(3 * 64, 0);
+  }
+  auto work = [&](int id) {
+    Worker &w = ws[id];
+    for (;;) {
+      const size_t k = next.fetch_add(1);
+      if (k >= order.size()) break;
+      EventTally &ev = c->events[order[k]];
+      std::vector<Track> stack;
+      for (int i : byev[order[k]]) {
+        const Track t = basket_track(*c, trk[i]);
+        if (t.vol < 0)
+          ev.leak += t.e;
+        else
+          stack.push_back(t);
+      }
+      drain(*c, w, ev, stack);
+    }
+  };
+  if (nthreads == 1) {
+    work(0);
+  } else {
+    std::vector<std::thread> th;
+    for (int i = 0; i < nthreads; ++i) th.emplace_back(work, i);
+    for (auto &t : th) t.join();
+  }
+  for (auto &w : ws) {
+    for (size_t i = 0; i < w.ez.size(); ++i) c->tot.ez[i] += w.ez[i];
+    for (size_t i = 0; i < w.er.size(); ++i) c->tot.er[i] += w.er[i];
+    for (size_t i = 0; i < w.ev.size(); ++i) c->tot.ev[i] += w.ev[i];
+    for (size_t i = 0; i < w.spectra.size(); ++i) c->tot.spectra[i] += w.spectra[i];
+    c->tot.steps += w.steps;
+    c->tot.killed += w.killed;
+    c->tot.maxlive = std::max(c->tot.maxlive, w.maxlive);
+  }
+  return GXT_OK;
+}
+
+// Run each event for at most steps_per_event track-steps (one thread, its
+// primaries in order, each depth first), then hand what is left over as a
+// basket: the track in flight, its stack and the primaries not yet started,
+// in that order.  The event tallies hold the steps run here.
+int gxref_run_partial(gxref_ctx *c, const gxt_primary *prim, int n, int64_t steps_per_event, gxt_track *out,
+                      int capacity, int *n_out) {
+  if (!c || (n > 0 && !prim) || !n_out || (capacity > 0 && !out) || steps_per_event < 0)
+    return fail(GXT_EINVAL, "bad run_partial arguments");
+  for (int i = 0; i < n; ++i)
+    if (prim[i].event < 0 || prim[i].event >= (int)c->events.size()) return fail(GXT_EINVAL, "event index out of range");
+  std::vector<std::vector<int>> byev(c->events.size());
+  for (int i = 0; i < n; ++i) byev[prim[i].event].push_back(i);
+  Worker w;
+  w.ez.assign(c->cfg.n_zbins, 0.0);
+  w.er.assign(c->cfg.n_rbins, 0.0);
+  w.ev.assign(c->geo.nvol(), 0.0);
+  w.spectra.assign(3 * 64, 0);
+  std::vector<gxt_track> basket;
+  for (size_t e = 0; e < byev.size(); ++e) {
+    EventTally &ev = c->events[e];
+    int64_t left = steps_per_event;
+    std::vector<Track> stack;
+    for (size_t k = 0; k < byev[e].size(); ++k) {
+      const Track t0 = primary_track(*c, prim[byev[e][k]]);
+      if (left <= 0) {
+        // not started: handed over as it starts (interaction lengths -1)
+        if (t0.vol < 0)
+          ev.leak += t0.e;
+        else
+          basket.push_back(to_basket(t0));
+        continue;
+      }
+      if (t0.vol < 0) {
+        ev.leak += t0.e;
+        continue;
+      }
+      stack.push_back(t0);
+      while (!stack.empty() && left > 0) {
+        Track cur = stack.back();
+        stack.pop_back();
+        bool alive = true;
+        while (alive && left > 0) {
+          one_step(*c, w, ev, stack, cur, alive);
+          --left;
+        }
+        if (alive) basket.push_back(to_basket(cur));
+      }
+      for (auto it = stack.rbegin(); it != stack.rend(); ++it) basket.push_back(to_basket(*it));
+      stack.clear();
+    }
+  }
+  for (size_t i = 0; i < w.ez.size(); ++i) c->tot.ez[i] += w.ez[i];
+  for (size_t i = 0; i < w.er.size(); ++i) c->tot.er[i] += w.er[i];
+  for (size_t i = 0; i < w.ev.size(); ++i) c->tot.ev[i] += w.ev[i];
+  for (size_t i = 0; i < w.spectra.size(); ++i) c->tot.spectra[i] += w.spectra[i];
+  c->tot.steps += w.steps;
+  c->tot.killed += w.killed;
+  *n_out = (int)basket.size();
+  if ((int)basket.size() > capacity) return fail(GXT_ECAPACITY, "basket larger than capacity");
+  std::copy(basket.begin(), basket.end(), out);
   return GXT_OK;
 }
 

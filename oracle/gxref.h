@@ -37,6 +37,14 @@ int gxref_egamma_gun(uint32_t seed, int n_events, int per_event, double gamma_fr
 /* Transport n primaries of one job (events numbered by gxt_primary.event)
  * with nthreads host threads (events handed out dynamically). */
 int gxref_run(gxref_ctx *ctx, const gxt_primary *prim, int n, int nthreads);
+/* Continue a basket of tracks in flight (gxt_submit_tracks). */
+int gxref_run_tracks(gxref_ctx *ctx, const gxt_track *tracks, int n, int nthreads);
+/* Run each event for at most steps_per_event track-steps, then return what
+ * is left as a basket (test infrastructure for the hand-off identity:
+ * partial + continuation = the whole run).  *n_out is the basket size even
+ * when it exceeds capacity (GXT_ECAPACITY). */
+int gxref_run_partial(gxref_ctx *ctx, const gxt_primary *prim, int n, int64_t steps_per_event, gxt_track *out,
+                      int capacity, int *n_out);
 int gxref_get_profiles(gxref_ctx *ctx, double *edep_z, double *edep_r, double *edep_volume);
 int gxref_get_event_tallies(gxref_ctx *ctx, gxt_event_tally *out, int n);
 int gxref_get_spectra(gxref_ctx *ctx, int64_t *counts);

@@ -5,6 +5,13 @@
 //                                 (seed 777); prints one line per event
 //   gxt_driver DATA_DIR nav       the reference calorimeter as a volume
 //                                 tree; FindNextBoundary for fixed tracks
+//   gxt_driver DATA_DIR basket FILE N
+//                                 a basket of tracks in flight (raw
+//                                 gxt_track records in FILE, N events)
+//                                 continued in the PbWO4 block; one line
+//                                 per event
+//   gxt_driver DATA_DIR slab      gxt_slab_distance of fixed points in the
+//                                 PbWO4 block; one line per point
 //
 // Test infrastructure: tests/test_gpu_cxx_driver.py compares its output with
 // the Python binding and the oracle.
@@ -20,9 +27,10 @@ static int fail(const char *what) {
   return 1;
 }
 
-static int run_block(const char *data_dir, int n) {
+// The 30 cm PbWO4 block, seed 777, n event slots.
+static gxt_ctx *block_context(const char *data_dir, int n) {
   gxt_config cfg;
-  if (gxt_default_config(&cfg) != GXT_OK) return fail("gxt_default_config");
+  if (gxt_default_config(&cfg) != GXT_OK) return nullptr;
   cfg.geometry = GXT_GEOM_BOX;
   cfg.material = GXT_MAT_PBWO4;
   cfg.box_xy = 300.0;
@@ -33,7 +41,20 @@ static int run_block(const char *data_dir, int n) {
   cfg.max_events = n;
   cfg.data_dir = data_dir;
   gxt_ctx *ctx = nullptr;
-  if (gxt_create(&cfg, &ctx) != GXT_OK) return fail("gxt_create");
+  if (gxt_create(&cfg, &ctx) != GXT_OK) return nullptr;
+  return ctx;
+}
+
+static void print_events(const std::vector<gxt_event_tally> &t) {
+  for (size_t i = 0; i < t.size(); ++i)
+    std::printf("event %d %.17g %.17g %.17g %lld %lld %lld %lld\n", (int)i, t[i].edep, t[i].visible, t[i].leak,
+                (long long)t[i].n_steps, (long long)t[i].n_secondaries[0], (long long)t[i].n_secondaries[1],
+                (long long)t[i].n_secondaries[2]);
+}
+
+static int run_block(const char *data_dir, int n) {
+  gxt_ctx *ctx = block_context(data_dir, n);
+  if (!ctx) return fail("gxt_create");
   std::vector<gxt_primary> prim(n);
   if (gxt_make_primaries(777, 1, 10000.0, 10000.0, 0.0, n, 0, n, prim.data()) != GXT_OK)
     return fail("gxt_make_primaries");
@@ -44,10 +65,63 @@ static int run_block(const char *data_dir, int n) {
     gxt_destroy(ctx);
     return 1;
   }
+  print_events(t);
+  gxt_destroy(ctx);
+  return 0;
+}
+
+// A basket handed over by a CPU transport: raw gxt_track records.
+static int run_basket(const char *data_dir, const char *path, int n_events) {
+  std::FILE *f = std::fopen(path, "rb");
+  if (!f) {
+    std::fprintf(stderr, "cannot open %s\n", path);
+    return 1;
+  }
+  std::vector<gxt_track> trk;
+  gxt_track rec;
+  while (std::fread(&rec, sizeof rec, 1, f) == 1) trk.push_back(rec);
+  std::fclose(f);
+  gxt_ctx *ctx = block_context(data_dir, n_events);
+  if (!ctx) return fail("gxt_create");
+  std::vector<gxt_event_tally> t(n_events);
+  if (gxt_submit_tracks(ctx, trk.data(), (int)trk.size(), n_events, 3) != GXT_OK || gxt_run(ctx, 0) != GXT_OK ||
+      gxt_get_event_tallies(ctx, 3, t.data(), n_events) != GXT_OK) {
+    std::fprintf(stderr, "basket: %s\n", gxt_ctx_error(ctx));
+    gxt_destroy(ctx);
+    return 1;
+  }
+  std::printf("tracks %d\n", (int)trk.size());
+  print_events(t);
+  gxt_destroy(ctx);
+  return 0;
+}
+
+// gxt_slab_distance in the block: from inside, on the faces, from outside.
+static int run_slab(const char *data_dir) {
+  gxt_ctx *ctx = block_context(data_dir, 1);
+  if (!ctx) return fail("gxt_create");
+  const int n = 6;
+  const double px[n] = {0.0, 10.0, 150.0, -149.5, 0.0, 200.0};
+  const double py[n] = {0.0, -20.0, 0.0, 3.0, 0.0, 0.0};
+  const double pz[n] = {0.0, 150.0, 30.0, 299.0, 300.0, 10.0};
+  const double dx[n] = {0.0, 0.6, 1.0, -0.6, 0.0, -1.0};
+  const double dy[n] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const double dz[n] = {1.0, 0.8, 0.0, 0.8, -1.0, 0.0};
+  const double pstep[n] = {5.0, 1000.0, 1.0, 0.25, 1e20, 3.0};
+  std::vector<double> pos(3 * n), dir(3 * n), snext(n), safety(n);
+  std::vector<int32_t> vol(n, -1), next(n), boundary(n);
+  for (int i = 0; i < n; ++i) {
+    pos[i] = px[i], pos[n + i] = py[i], pos[2 * n + i] = pz[i];
+    dir[i] = dx[i], dir[n + i] = dy[i], dir[2 * n + i] = dz[i];
+  }
+  if (gxt_slab_distance(ctx, pos.data(), dir.data(), pstep, vol.data(), snext.data(), next.data(), safety.data(),
+                        boundary.data(), n) != GXT_OK) {
+    std::fprintf(stderr, "slab: %s\n", gxt_ctx_error(ctx));
+    gxt_destroy(ctx);
+    return 1;
+  }
   for (int i = 0; i < n; ++i)
-    std::printf("event %d %.17g %.17g %.17g %lld %lld %lld %lld\n", i, t[i].edep, t[i].visible, t[i].leak,
-                (long long)t[i].n_steps, (long long)t[i].n_secondaries[0], (long long)t[i].n_secondaries[1],
-                (long long)t[i].n_secondaries[2]);
+    std::printf("point %d %d %.17g %d %.17g %d\n", i, vol[i], snext[i], next[i], safety[i], boundary[i]);
   gxt_destroy(ctx);
   return 0;
 }
@@ -116,11 +190,13 @@ static int run_nav() {
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s DATA_DIR block N | nav\n", argv[0]);
+    std::fprintf(stderr, "usage: %s DATA_DIR block N | nav | basket FILE N | slab\n", argv[0]);
     return 2;
   }
   if (!std::strcmp(argv[2], "block")) return run_block(argv[1], argc > 3 ? std::atoi(argv[3]) : 16);
   if (!std::strcmp(argv[2], "nav")) return run_nav();
+  if (!std::strcmp(argv[2], "basket") && argc > 4) return run_basket(argv[1], argv[3], std::atoi(argv[4]));
+  if (!std::strcmp(argv[2], "slab")) return run_slab(argv[1]);
   std::fprintf(stderr, "unknown mode %s\n", argv[2]);
   return 2;
 }

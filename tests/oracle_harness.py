@@ -49,6 +49,21 @@ class Oracle:
     def run(self, prims, nthreads=1):
         check(lib().gxref_run(self.ctx, prims, len(prims), nthreads))
 
+    def run_tracks(self, tracks, nthreads=1):
+        check(lib().gxref_run_tracks(self.ctx, tracks, len(tracks), nthreads))
+
+    def run_partial(self, prims, steps_per_event, capacity=None):
+        """Run each event for at most steps_per_event steps; returns the basket
+        of what is left (abi.Track array)."""
+        cap = capacity if capacity is not None else 8192 * max(len(prims), 1)
+        out = abi.tracks_array(cap)
+        n = C.c_int(0)
+        check(lib().gxref_run_partial(self.ctx, prims, len(prims), C.c_int64(steps_per_event), out, cap,
+                                      C.byref(n)))
+        res = abi.tracks_array(n.value)
+        C.memmove(res, out, n.value * C.sizeof(abi.Track))
+        return res
+
     def event_tallies(self, n):
         t = (abi.EventTally * n)()
         check(lib().gxref_get_event_tallies(self.ctx, t, n))

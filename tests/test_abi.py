@@ -35,6 +35,7 @@ def test_library_exports_every_declared_function():
 STRUCTS = {
     "gxt_config": abi.Config,
     "gxt_primary": abi.Primary,
+    "gxt_track": abi.Track,
     "gxt_event_tally": abi.EventTally,
     "gxt_stats": abi.Stats,
     "gxt_job_result": abi.JobResult,
@@ -178,3 +179,45 @@ def test_submit_checks_primaries_without_device():
     # good primaries pass the checks and stop at the missing context
     assert submit() == -1 and b"null context" in lib.gxt_last_error()
     assert submit(dir=(0.6, 0.0, 0.8)) == -1 and b"null context" in lib.gxt_last_error()
+
+
+def test_submit_tracks_checks_tracks_without_device():
+    """gxt_submit_tracks rejects a bad track in flight before anything is
+    queued, and what a track carries before it needs a context: the
+    kinematic checks of a primary, an event outside [0, n_events), a
+    non-finite number of interaction lengths, a negative step count, an
+    MRG32k3a state with a word at or above its modulus or an all-zero
+    triple."""
+    lib = C.CDLL(LIB)
+    lib.gxt_last_error.restype = C.c_char_p
+
+    def submit(n_events=2, **kw):
+        t = abi.tracks_array(2)
+        for q in t:
+            q.species, q.ekin, q.volume = abi.GAMMA, 5.0, -1
+            q.dir[:] = (0.0, 0.0, 1.0)
+            q.nil[:] = (-1.0, 0.3, 2.0)
+            q.rng[:] = (1, 2, 3, 4, 5, 6)
+            q.event = 1
+        for k, v in kw.items():
+            if k in ("pos", "dir", "nil", "rng"):
+                getattr(t[1], k)[:] = v
+            else:
+                setattr(t[1], k, v)
+        return lib.gxt_submit_tracks(None, t, 2, n_events, C.c_uint64(1))
+
+    cases = [(dict(ekin=float("inf")), b"ekin"), (dict(ekin=-1.0), b"ekin"),
+             (dict(pos=(0.0, float("nan"), 0.0)), b"finite"), (dict(dir=(0.0, 0.0, 1.5)), b"unit"),
+             (dict(species=-1), b"species"), (dict(event=2), b"n_events"), (dict(event=-3), b"event"),
+             (dict(nil=(0.0, float("nan"), 1.0)), b"interaction lengths"), (dict(nsteps=-1), b"nsteps"),
+             (dict(rng=(4294967087, 1, 1, 1, 1, 1)), b"MRG32k3a"), (dict(rng=(1, 1, 1, 1, 4294944443, 1)), b"MRG32k3a"),
+             (dict(rng=(0, 0, 0, 1, 1, 1)), b"MRG32k3a"), (dict(rng=(1, 0, 0, 0, 0, 0)), b"MRG32k3a")]
+    for kw, needle in cases:
+        assert submit(**kw) == -1, kw
+        msg = lib.gxt_last_error()
+        assert needle in msg and b"track 1" in msg, (kw, msg)
+    assert submit(n_events=0) == -1 and b"bad submit_tracks arguments" in lib.gxt_last_error()
+    # good tracks pass the checks and stop at the missing context
+    assert submit() == -1 and b"null context" in lib.gxt_last_error()
+    assert submit(rng=(4294967086, 0, 0, 0, 0, 4294944442), nsteps=7) == -1
+    assert b"null context" in lib.gxt_last_error()

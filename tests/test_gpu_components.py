@@ -73,7 +73,9 @@ def test_slab_distance_bit_exact(gpu_lib, which):
         for a, b, name in zip(g, c, ("vol", "snext", "next", "safety", "boundary")):
             assert np.array_equal(a.view(np.int64) if a.dtype == np.float64 else a,
                                   b.view(np.int64) if b.dtype == np.float64 else b), name
-    assert (g[0] == -1).any() and (g[0] >= 0).any()
+        if vol_in is None:
+            # located by the entry point: the sample holds points outside and inside
+            assert (g[0] == -1).any() and (g[0] >= 0).any()
     # without pstep every boundary flag is 1
     g = gt.slab_distance(pos[:, :100], d[:, :100])
     assert np.all(g[4] == 1)
@@ -90,6 +92,18 @@ MODEL_E = {
     "BHABHA": (1.5, 5e4),
     "ANNIHIL": (1e-3, 1e4),
 }
+
+
+def assert_close(a, b, rtol, atol, what):
+    """|a - b| <= atol + rtol |b| elementwise (atol may be an array, which
+    numpy.testing.assert_allclose does not accept)."""
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    atol = np.broadcast_to(np.asarray(atol, dtype=np.float64), b.shape)
+    err = np.abs(a - b) - (atol + rtol * np.abs(b))
+    if err.size and np.nanmax(err) > 0 or np.isnan(err).any():
+        k = np.unravel_index(np.nanargmax(err), err.shape)
+        raise AssertionError("%s: %d of %d outside |a-b| <= atol + %g|b|; worst at %s: %r vs %r (atol %g)"
+                             % (what, int((err > 0).sum()), err.size, rtol, k, a[k], b[k], atol[k]))
 
 
 def energy_out(o, e0, model):
@@ -121,11 +135,11 @@ def test_samplers_fp32_against_fp64_oracle(gpu_lib, model):
     same = np.all(g["rng"] == c["rng"], axis=1) & (g["nsec"] == c["nsec"])
     assert same.mean() >= 0.97, same.mean()
     s = same
-    np.testing.assert_allclose(g["ekin"][s], c["ekin"][s], rtol=2e-4, atol=1e-7 * ekin[s])
-    np.testing.assert_allclose(g["sec_ekin"][s], c["sec_ekin"][s], rtol=2e-4, atol=(1e-7 * ekin[s])[:, None])
-    np.testing.assert_allclose(g["edep"][s], c["edep"][s], rtol=2e-4, atol=1e-7 * ekin[s])
-    np.testing.assert_allclose(g["dir"][:, s], c["dir"][:, s], rtol=0, atol=2e-4)
-    np.testing.assert_allclose(g["sec_dir"][s], c["sec_dir"][s], rtol=0, atol=2e-4)
+    assert_close(g["ekin"][s], c["ekin"][s], 2e-4, 1e-7 * ekin[s], "ekin")
+    assert_close(g["sec_ekin"][s], c["sec_ekin"][s], 2e-4, (1e-7 * ekin[s])[:, None], "sec_ekin")
+    assert_close(g["edep"][s], c["edep"][s], 2e-4, 1e-7 * ekin[s], "edep")
+    assert_close(g["dir"][:, s], c["dir"][:, s], 0.0, 2e-4, "dir")
+    assert_close(g["sec_dir"][s], c["sec_dir"][s], 0.0, 2e-4, "sec_dir")
     # distributions of the relative energies over all tracks
     for k in ("ekin",):
         p = stats.ks_2samp(g[k] / ekin, c[k] / ekin).pvalue
@@ -136,5 +150,5 @@ def test_samplers_fp32_against_fp64_oracle(gpu_lib, model):
             p = stats.ks_2samp(g["sec_ekin"][has_g, j] / ekin[has_g], c["sec_ekin"][has_c, j] / ekin[has_c]).pvalue
             assert p > 0.01, ("sec", j, p)
     # energy is conserved on every track, to FP32 precision
-    np.testing.assert_allclose(energy_out(g, ekin, model), ekin, rtol=1e-5, atol=1e-9)
+    assert_close(energy_out(g, ekin, model), ekin, 1e-5, 1e-9, "energy balance")
     gt.close()

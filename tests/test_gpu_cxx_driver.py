@@ -64,3 +64,46 @@ def test_cxx_find_next_boundary_matches_oracle():
     for i, r in enumerate(rows):
         assert int(r[1]) == i and int(r[2]) == t[i]
         assert float(r[3]) == s[i] and int(r[4]) == nxt[i] and float(r[5]) == saf[i] and int(r[6]) == b[i]
+
+
+def test_cxx_basket_matches_oracle(gpu_lib, tmp_path):
+    """A basket of tracks in flight written by a CPU transport (the oracle
+    cut after 400 steps per event) and continued through gxt_submit_tracks
+    from C++ equals the oracle's continuation: counts exact, energies 1e-9."""
+    n = 8
+    _flush("C++ driver: basket of %d PbWO4 events" % n)
+    cfg = abi.config_pbwo4_block(max_events=n)
+    prims = O.make_primaries(777, abi.ELECTRON, 10000.0, 10000.0, 0.0, n)
+    basket = O.Oracle(cfg).run_partial(prims, 400)
+    path = tmp_path / "basket.bin"
+    path.write_bytes(bytes(basket))
+    out = _driver("basket", str(path), str(n))
+    assert out[0] == "tracks %d" % len(basket)
+    rows = [line.split() for line in out if line.startswith("event")]
+    assert len(rows) == n
+    cpu = O.Oracle(cfg)
+    cpu.run_tracks(basket, 16)
+    ev = cpu.event_tallies(n)
+    for i, r in enumerate(rows):
+        assert int(r[1]) == i
+        np.testing.assert_allclose([float(x) for x in r[2:5]], [ev["edep"][i], ev["visible"][i], ev["leak"][i]],
+                                   rtol=1e-9, atol=1e-9)
+        assert int(r[5]) == ev["n_steps"][i]
+        assert [int(x) for x in r[6:9]] == list(ev["n_secondaries"][i])
+
+
+def test_cxx_slab_distance_matches_oracle():
+    """gxt_slab_distance from C++ equals the oracle bit for bit."""
+    out = _driver("slab")
+    rows = [line.split() for line in out if line.startswith("point")]
+    assert len(rows) == 6
+    # the driver's points and directions
+    pos = np.array([[0.0, 10.0, 150.0, -149.5, 0.0, 200.0], [0.0, -20.0, 0.0, 3.0, 0.0, 0.0],
+                    [0.0, 150.0, 30.0, 299.0, 300.0, 10.0]])
+    d = np.array([[0.0, 0.6, 1.0, -0.6, 0.0, -1.0], [0.0] * 6, [1.0, 0.8, 0.0, 0.8, -1.0, 0.0]])
+    pstep = np.array([5.0, 1000.0, 1.0, 0.25, 1e20, 3.0])
+    o = O.Oracle(abi.config_pbwo4_block(max_events=1))
+    vol, s, nxt, saf, b = o.slab_distance(pos, d, pstep=pstep)
+    for i, r in enumerate(rows):
+        assert int(r[1]) == i and int(r[2]) == vol[i] and int(r[4]) == nxt[i] and int(r[6]) == b[i], (i, r)
+        assert float(r[3]) == s[i] and float(r[5]) == saf[i], (i, r)
