@@ -121,6 +121,8 @@ __device__ __forceinline__ float m_exp(float x) { return expf(x); }
 __device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float m_sqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double m_pow(double x, double y) { return pow_cw(x, y); }
+__device__ __forceinline__ double m_expm1(double x) { return expm1_cw(x); }
+__device__ __forceinline__ float m_expm1(float x) { return expm1f(x); }
 __device__ __forceinline__ float m_pow(float x, float y) { return powf(x, y); }
 // FP64 sine and cosine, computed the same way by the CPU oracle so that both
 // give the same bits (a helix turns by sin/cos every step, and a last-ulp
@@ -578,18 +580,26 @@ __device__ __forceinline__ void sample_compton(R e0, const R d0[3], Rng &rng, In
   const R E0_m = e0 / R(kMe);
   const R eps0 = R(1) / (R(1) + R(2) * E0_m);
   const R eps0sq = eps0 * eps0;
+  // 1 - eps0^2 and 1 - eps without cancellation (eps -> 1 is forward
+  // scattering): 1 - eps0 = 2 E0_m eps0; 1 - eps = -expm1(ln eps) on the
+  // first branch, (1 - eps^2) / (1 + eps) on the second
+  const R omeps0sq = R(2) * E0_m * eps0 * (R(1) + eps0);
   const R alpha1 = -m_log(eps0);
-  const R alpha2 = R(0.5) * (R(1) - eps0sq);
-  R eps, epssq, onecost, sint2, greject;
+  const R alpha2 = R(0.5) * omeps0sq;
+  R eps, omeps, epssq, onecost, sint2, greject;
   do {
     if (alpha1 / (alpha1 + alpha2) > rng.template u<R>()) {
-      eps = m_exp(-alpha1 * rng.template u<R>());
+      const R le = -alpha1 * rng.template u<R>();
+      eps = m_exp(le);
+      omeps = -m_expm1(le);
       epssq = eps * eps;
     } else {
-      epssq = eps0sq + (R(1) - eps0sq) * rng.template u<R>();
+      const R u = rng.template u<R>();
+      epssq = eps0sq + omeps0sq * u;
       eps = m_sqrt(epssq);
+      omeps = omeps0sq * (R(1) - u) / (R(1) + eps);
     }
-    onecost = (R(1) - eps) / (eps * E0_m);
+    onecost = omeps / (eps * E0_m);
     sint2 = onecost * (R(2) - onecost);
     greject = R(1) - eps * sint2 / (R(1) + epssq);
   } while (greject < rng.template u<R>());
@@ -885,7 +895,9 @@ __device__ __forceinline__ void sample_annihil(R t0, const R d0[3], Rng &rng, In
   const R tau2 = tau + R(2);
   const R sqgrate = m_sqrt(tau / tau2) * R(0.5);
   const R sqg2m1 = m_sqrt(tau * tau2);
-  const R epsmin = R(0.5) - sqgrate;
+  // 1/2 - sqgrate without cancellation at high energy: (1 - s) / 2 with
+  // s = sqrt(tau / tau2) is 1 / (tau2 (1 + s))
+  const R epsmin = R(1) / (tau2 * (R(1) + R(2) * sqgrate));
   const R epsmax = R(0.5) + sqgrate;
   const R epsqot = epsmax / epsmin;
   R eps, greject;

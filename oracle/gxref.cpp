@@ -1089,18 +1089,26 @@ static void sample_compton(const Material &, double e0, const double d0[3], uint
   const double E0_m = e0 / kMe;
   const double eps0 = 1.0 / (1.0 + 2.0 * E0_m);
   const double eps0sq = eps0 * eps0;
+  // 1 - eps0^2 and 1 - eps in forms without cancellation (the reference
+  // writes 1 - eps, ProxyKleinNishina.hpp:124): 1 - eps0 = 2 E0_m eps0;
+  // 1 - eps = -expm1(ln eps), or (1 - eps^2) / (1 + eps)
+  const double omeps0sq = 2.0 * E0_m * eps0 * (1.0 + eps0);
   const double alpha1 = -log_cw(eps0);
-  const double alpha2 = 0.5 * (1.0 - eps0sq);
-  double eps, epssq, onecost, sint2, greject;
+  const double alpha2 = 0.5 * omeps0sq;
+  double eps, omeps, epssq, onecost, sint2, greject;
   do {
     if (alpha1 / (alpha1 + alpha2) > mrg_uniform(s)) {
-      eps = exp_cw(-alpha1 * mrg_uniform(s));
+      const double le = -alpha1 * mrg_uniform(s);
+      eps = exp_cw(le);
+      omeps = -expm1_cw(le);
       epssq = eps * eps;
     } else {
-      epssq = eps0sq + (1.0 - eps0sq) * mrg_uniform(s);
+      const double u = mrg_uniform(s);
+      epssq = eps0sq + omeps0sq * u;
       eps = std::sqrt(epssq);
+      omeps = omeps0sq * (1.0 - u) / (1.0 + eps);
     }
-    onecost = (1.0 - eps) / (eps * E0_m);
+    onecost = omeps / (eps * E0_m);
     sint2 = onecost * (2.0 - onecost);
     greject = 1.0 - eps * sint2 / (1.0 + epssq);
   } while (greject < mrg_uniform(s));
@@ -1442,7 +1450,9 @@ static void sample_annihil(const Material &, double t0, const double d0[3], uint
   const double tau2 = tau + 2.0;
   const double sqgrate = std::sqrt(tau / tau2) * 0.5;
   const double sqg2m1 = std::sqrt(tau * tau2);
-  const double epsmin = 0.5 - sqgrate;
+  // 1/2 - sqgrate (G4eeToTwoGammaModel's form) without cancellation at high
+  // energy: (1 - s) / 2 with s = sqrt(tau / tau2) is 1 / (tau2 (1 + s))
+  const double epsmin = 1.0 / (tau2 * (1.0 + 2.0 * sqgrate));
   const double epsmax = 0.5 + sqgrate;
   const double epsqot = epsmax / epsmin;
   double eps, greject;

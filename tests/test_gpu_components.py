@@ -7,9 +7,15 @@
   oracle's FP64 ones, so most tracks take the same number of draws and agree
   to FP32 precision; a rejection test that falls the other way in FP32 takes
   a different path.  Tolerances: at least 97 % of tracks with the same
-  draws, their outputs within 2e-4 relative (energies) and 2e-4 absolute
-  (unit vectors); two-sample KS p > 0.01 on every output energy; energy
-  conservation to 1e-5 relative on every track.
+  draws, their outputs within 2e-4 relative or 1e-6 of the incoming energy
+  (about 17 FP32 ulps of it: an energy left after a loss that takes nearly
+  all, t0 - k, carries the rounding of t0), whichever is larger, and unit
+  vectors within 2e-4 + 1e-6 p0 / p1 absolute (a primary direction that
+  balances momentum, p1 d1 = p0 d0 - k d_k, is the difference of two nearly
+  equal vectors when the loss takes nearly all: its FP32 error grows as
+  p0 / p1); two-sample KS p > 0.01 on every output energy; energy
+  conservation to 1e-5 of the energy involved on every track (for an
+  annihilation, t0 + 2 m_e, which the two photons carry).
 """
 import numpy as np
 import pytest
@@ -135,10 +141,13 @@ def test_samplers_fp32_against_fp64_oracle(gpu_lib, model):
     same = np.all(g["rng"] == c["rng"], axis=1) & (g["nsec"] == c["nsec"])
     assert same.mean() >= 0.97, same.mean()
     s = same
-    assert_close(g["ekin"][s], c["ekin"][s], 2e-4, 1e-7 * ekin[s], "ekin")
-    assert_close(g["sec_ekin"][s], c["sec_ekin"][s], 2e-4, (1e-7 * ekin[s])[:, None], "sec_ekin")
-    assert_close(g["edep"][s], c["edep"][s], 2e-4, 1e-7 * ekin[s], "edep")
-    assert_close(g["dir"][:, s], c["dir"][:, s], 0.0, 2e-4, "dir")
+    assert_close(g["ekin"][s], c["ekin"][s], 2e-4, 1e-6 * ekin[s], "ekin")
+    assert_close(g["sec_ekin"][s], c["sec_ekin"][s], 2e-4, (1e-6 * ekin[s])[:, None], "sec_ekin")
+    assert_close(g["edep"][s], c["edep"][s], 2e-4, 1e-6 * ekin[s], "edep")
+    p0 = np.sqrt(ekin * (ekin + 2 * ME))
+    p1 = np.sqrt(c["ekin"] * (c["ekin"] + 2 * ME))
+    dir_tol = 2e-4 + 1e-6 * p0 / np.maximum(p1, 1e-300)
+    assert_close(g["dir"][:, s], c["dir"][:, s], 0.0, dir_tol[s][None, :], "dir")
     assert_close(g["sec_dir"][s], c["sec_dir"][s], 0.0, 2e-4, "sec_dir")
     # distributions of the relative energies over all tracks
     for k in ("ekin",):
@@ -150,5 +159,6 @@ def test_samplers_fp32_against_fp64_oracle(gpu_lib, model):
             p = stats.ks_2samp(g["sec_ekin"][has_g, j] / ekin[has_g], c["sec_ekin"][has_c, j] / ekin[has_c]).pvalue
             assert p > 0.01, ("sec", j, p)
     # energy is conserved on every track, to FP32 precision
-    assert_close(energy_out(g, ekin, model), ekin, 1e-5, 1e-9, "energy balance")
+    involved = ekin + (2 * ME if model == "ANNIHIL" else 0.0)
+    assert_close(energy_out(g, ekin, model), ekin, 0.0, 1e-5 * involved, "energy balance")
     gt.close()
