@@ -19,6 +19,7 @@ STEPPER_HELIX, STEPPER_RK45 = 0, 1
 RK_ANGLE = 0.25
 CMS_FIELD_MAP = os.path.join(DATA_DIR, "field", "cmsmagfield2015.txt")
 GAMMA, ELECTRON, POSITRON = 0, 1, 2
+SAMPLE_NOALLOC_MAX = 65536  # GXT_SAMPLE_NOALLOC_MAX
 
 TAB = dict(LAMBDA_COMPT=0, LAMBDAPRIM_COMPT=1, LAMBDA_CONV=2, LAMBDAPRIM_PHOT=3, LAMBDA_BREM_EM=4,
            LAMBDA_IONI_EM=5, DEDX_EM=6, RANGE_EM=7, INVRANGE_EM=8, LAMBDA_BREM_EP=9, LAMBDA_IONI_EP=10,

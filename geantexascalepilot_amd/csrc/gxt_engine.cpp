@@ -75,6 +75,51 @@ struct DevBuf {
   DevBuf &operator=(const DevBuf &) = delete;
 };
 
+// ------------------------------------------------ gxt_sample_noalloc staging
+// The input and output arrays of up to GXT_SAMPLE_NOALLOC_MAX tracks of
+// gxt_sample_noalloc live in one file-scope device array, `scratch` (a copy
+// on every GPU the library runs on, held until the process ends: 11.3 MB).
+// Its device address is looked up once per device, with that device current,
+// when a context is created; host code uses only that address (the array's
+// name is a host-side stand-in).  Calls that overlap on one GPU share it and
+// race.
+constexpr int kNoAllocSampleTracks = GXT_SAMPLE_NOALLOC_MAX;
+constexpr int kMaxDevices = 64;
+constexpr size_t stage_align(size_t x) { return (x + 255) & ~(size_t)255; }
+// Byte offsets of the staged arrays of n tracks (each 256-byte aligned).
+struct SampleStaging {
+  size_t ek, dir, rng, oek, odir, oed, nsec, ssp, sek, sdir, total;
+  constexpr explicit SampleStaging(size_t n)
+      : ek(0),
+        dir(stage_align(ek + 8 * n)),
+        rng(stage_align(dir + 24 * n)),
+        oek(stage_align(rng + 24 * n)),
+        odir(stage_align(oek + 8 * n)),
+        oed(stage_align(odir + 24 * n)),
+        nsec(stage_align(oed + 8 * n)),
+        ssp(stage_align(nsec + 4 * n)),
+        sek(stage_align(ssp + 8 * n)),
+        sdir(stage_align(sek + 16 * n)),
+        total(stage_align(sdir + 48 * n)) {}
+};
+constexpr size_t kScratchBytes = SampleStaging(kNoAllocSampleTracks).total;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Record scratch's address on `device` (the current device), once.
+int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return set_err(GXT_EHIP, "device index beyond the staging table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return GXT_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return set_err(GXT_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return GXT_OK;
+}
+
 struct DeviceData {
   DevParams *P = nullptr;
   Node64 *nd = nullptr;
@@ -754,6 +799,50 @@ std::string check_kinematics(const In &p) {
   return "";
 }
 
+// ------------------------------------------------------------ samplers
+// Inputs of gxt_sample checked before any device work: finite kinetic energy
+// > 0, unit direction, a valid MRG32k3a state for every track.
+int check_sample(gxt_ctx *c, int model, int slot, const gxt_sample_io *io) {
+  if (!c || !io || io->n <= 0 || io->n > kMaxSoaPoints || model < 0 || model > GXT_MODEL_ANNIHIL || slot < 0 ||
+      slot >= (c->cfg.geometry == GXT_GEOM_BOX ? 1 : 2))
+    return set_err(GXT_EINVAL, "bad sample arguments");
+  if (!io->ekin || !io->dir || !io->rng || !io->out_ekin || !io->out_dir || !io->out_edep || !io->nsec ||
+      !io->sec_species || !io->sec_ekin || !io->sec_dir)
+    return set_err(GXT_EINVAL, "sample: null array");
+  const int n = io->n;
+  for (int i = 0; i < n; ++i) {
+    const double d0 = io->dir[i], d1 = io->dir[n + i], d2 = io->dir[2 * n + i];
+    const double nn = d0 * d0 + d1 * d1 + d2 * d2;
+    const uint32_t *r = io->rng + 6 * (size_t)i;
+    bool ok = io->ekin[i] > 0 && std::isfinite(io->ekin[i]) && std::fabs(nn - 1.0) <= 2e-6;
+    ok = ok && !(r[0] == 0 && r[1] == 0 && r[2] == 0) && !(r[3] == 0 && r[4] == 0 && r[5] == 0);
+    for (int k = 0; k < 3; ++k) ok = ok && r[k] < 4294967087u && r[3 + k] < 4294944443u;
+    if (!ok) return set_err(GXT_EINVAL, "sample: bad energy, direction or RNG state at " + std::to_string(i));
+  }
+  return GXT_OK;
+}
+
+// Copy the inputs to the device arrays of s, sample, copy the outputs back;
+// everything on the context's stream.
+int run_sample(gxt_ctx *c, int model, int slot, int precision, gxt_sample_io *io, const SampleArgs &s) {
+  const size_t n = (size_t)io->n;
+  hipStream_t st = c->stream;
+  HIP_TRY(hipMemcpyAsync(const_cast<double *>(s.ekin), io->ekin, n * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(const_cast<double *>(s.dir), io->dir, 3 * n * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s.rng, io->rng, 6 * n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(launch_sample(c->dd.P, precision, slot, model, s, st));
+  HIP_TRY(hipMemcpyAsync(io->rng, s.rng, 6 * n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(io->out_ekin, s.out_ekin, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(io->out_dir, s.out_dir, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(io->out_edep, s.out_edep, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(io->nsec, s.nsec, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(io->sec_species, s.sec_species, 2 * n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(io->sec_ekin, s.sec_ekin, 2 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(io->sec_dir, s.sec_dir, 6 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return GXT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -812,6 +901,8 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
   c->prof_level = cfg->profile_kernels > 1 ? 2 : 1;
   c->t0 = std::chrono::steady_clock::now();
   HIP_TRY(hipGetDevice(&c->device));
+  rc = scratch_setup(c->device);
+  if (rc) return rc;
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_TRY(hipStreamCreateWithFlags(&c->ss.side, hipStreamNonBlocking));
   HIP_TRY(hipEventCreateWithFlags(&c->ss.fork, hipEventDisableTiming));
@@ -1251,51 +1342,43 @@ int gxt_macro_xs(gxt_ctx *c, int slot, int species, int process, int precision, 
 }
 
 int gxt_sample(gxt_ctx *c, int model, int slot, int precision, gxt_sample_io *io) {
-  if (!c || !io || io->n <= 0 || io->n > kMaxSoaPoints || model < 0 || model > GXT_MODEL_ANNIHIL || slot < 0 ||
-      slot >= nslots(c))
-    return set_err(GXT_EINVAL, "bad sample arguments");
-  const int n = io->n;
-  SampleArgs s;
-  s.n = n;
-  double *ek = nullptr, *dir = nullptr, *oek = nullptr, *odir = nullptr, *oed = nullptr, *sek = nullptr, *sdir = nullptr;
-  uint32_t *rng = nullptr;
-  int32_t *nsec = nullptr, *ssp = nullptr;
-  HIP_TRY(dalloc(&ek, n));
-  HIP_TRY(dalloc(&dir, 3 * (size_t)n));
-  HIP_TRY(dalloc(&rng, 6 * (size_t)n));
-  HIP_TRY(dalloc(&oek, n));
-  HIP_TRY(dalloc(&odir, 3 * (size_t)n));
-  HIP_TRY(dalloc(&oed, n));
-  HIP_TRY(dalloc(&nsec, n));
-  HIP_TRY(dalloc(&ssp, 2 * (size_t)n));
-  HIP_TRY(dalloc(&sek, 2 * (size_t)n));
-  HIP_TRY(dalloc(&sdir, 6 * (size_t)n));
-  HIP_TRY(hipMemcpy(ek, io->ekin, n * sizeof(double), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(dir, io->dir, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(rng, io->rng, 6 * (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice));
-  s.ekin = ek;
-  s.dir = dir;
-  s.rng = rng;
-  s.out_ekin = oek;
-  s.out_dir = odir;
-  s.out_edep = oed;
-  s.nsec = nsec;
-  s.sec_species = ssp;
-  s.sec_ekin = sek;
-  s.sec_dir = sdir;
-  HIP_TRY(launch_sample(c->dd.P, precision, slot, model, s, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipMemcpy(io->rng, rng, 6 * (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(io->out_ekin, oek, n * sizeof(double), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(io->out_dir, odir, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(io->out_edep, oed, n * sizeof(double), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(io->nsec, nsec, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(io->sec_species, ssp, 2 * (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(io->sec_ekin, sek, 2 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(io->sec_dir, sdir, 6 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
-  void *ptrs[] = {ek, dir, rng, oek, odir, oed, nsec, ssp, sek, sdir};
-  for (void *p : ptrs) (void)hipFree(p);
-  return GXT_OK;
+  const int rc = check_sample(c, model, slot, io);
+  if (rc) return rc;
+  const size_t n = (size_t)io->n;
+  DevBuf<double> ek(n), dir(3 * n), oek(n), odir(3 * n), oed(n), sek(2 * n), sdir(6 * n);
+  DevBuf<uint32_t> rng(6 * n);
+  DevBuf<int32_t> nsec(n), ssp(2 * n);
+  for (hipError_t e : {ek.err, dir.err, oek.err, odir.err, oed.err, sek.err, sdir.err, rng.err, nsec.err, ssp.err})
+    HIP_TRY(e);
+  SampleArgs s{io->n, ek.p, dir.p, rng.p, oek.p, odir.p, oed.p, nsec.p, ssp.p, sek.p, sdir.p};
+  return run_sample(c, model, slot, precision, io, s);
+}
+
+int gxt_sample_noalloc(gxt_ctx *c, int model, int slot, int precision, gxt_sample_io *io) {
+  const int rc = check_sample(c, model, slot, io);
+  if (rc) return rc;
+  if (io->n > kNoAllocSampleTracks)
+    return c->fail(GXT_EINVAL, "gxt_sample_noalloc: n = " + std::to_string(io->n) + " exceeds its staging limit of " +
+                                   std::to_string(kNoAllocSampleTracks) + " tracks");
+  unsigned char *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (c->device >= 0 && c->device < kMaxDevices) base = static_cast<unsigned char *>(g_scratch_ptr[c->device]);
+  }
+  if (!base) return c->fail(GXT_ESTATE, "gxt_sample_noalloc: no staging address for this device");
+  const SampleStaging L(io->n);
+  SampleArgs s{io->n,
+               reinterpret_cast<double *>(base + L.ek),
+               reinterpret_cast<double *>(base + L.dir),
+               reinterpret_cast<uint32_t *>(base + L.rng),
+               reinterpret_cast<double *>(base + L.oek),
+               reinterpret_cast<double *>(base + L.odir),
+               reinterpret_cast<double *>(base + L.oed),
+               reinterpret_cast<int32_t *>(base + L.nsec),
+               reinterpret_cast<int32_t *>(base + L.ssp),
+               reinterpret_cast<double *>(base + L.sek),
+               reinterpret_cast<double *>(base + L.sdir)};
+  return run_sample(c, model, slot, precision, io, s);
 }
 
 int gxt_slab_distance(gxt_ctx *c, const double *pos, const double *dir, const double *pstep, int32_t *vol,

@@ -370,10 +370,11 @@ class Transport:
         self._c(lib.gxt_macro_xs(self.ctx, slot, species, process, precision, abi.ptr(e), abi.ptr(out), len(e)))
         return out
 
-    def sample(self, model, slot, ekin, dirs, states, precision=abi.FP64):
-        from_io = _sample_with(lambda io: lib.gxt_sample(self.ctx, model, slot, precision, C.byref(io)),
-                               ekin, dirs, states)
-        return from_io
+    def sample(self, model, slot, ekin, dirs, states, precision=abi.FP64, noalloc=False):
+        """Final-state sampler of a model on n tracks; noalloc stages through the
+        library's fixed device array (gxt_sample_noalloc, n <= SAMPLE_NOALLOC_MAX)."""
+        fn = lib.gxt_sample_noalloc if noalloc else lib.gxt_sample
+        return _sample_with(lambda io: fn(self.ctx, model, slot, precision, C.byref(io)), ekin, dirs, states)
 
     def slab_distance(self, pos, dirs, vol=None, pstep=None):
         """Straight-line navigation of the loop's geometry (FindNextBoundary,

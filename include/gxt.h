@@ -363,6 +363,15 @@ typedef struct gxt_sample_io {
   double *sec_dir;           /* [6n]: 3 components of secondary j of track i at [6i+3j] */
 } gxt_sample_io;
 int gxt_sample(gxt_ctx *ctx, int model, int material_slot, int precision, gxt_sample_io *io);
+/* gxt_sample that allocates no device memory: the inputs and outputs are
+ * staged through one fixed device array of the library (one per GPU, held
+ * until the process ends), for up to GXT_SAMPLE_NOALLOC_MAX tracks per call
+ * (a larger n is GXT_EINVAL).  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (serving loops and harnesses
+ * that forbid hipMalloc).  Every call on a GPU shares that array, so calls
+ * that overlap on one GPU race. */
+#define GXT_SAMPLE_NOALLOC_MAX 65536
+int gxt_sample_noalloc(gxt_ctx *ctx, int model, int material_slot, int precision, gxt_sample_io *io);
 
 /* Constant-field helix step (replaces ConstFieldHelixStepper::DoStep /
  * DoStepArr, magneticfield/ConstFieldHelixStepper.hpp:57-86,164-302, in
