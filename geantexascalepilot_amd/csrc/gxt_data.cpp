@@ -8,6 +8,7 @@
 // reference's elimination (ProxyPhysicsVector.cu:308-386).
 #include "gxt_data.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -167,6 +168,59 @@ struct Comp {
 
 }  // namespace
 
+// Merged Sandia intervals of material m (DevMaterial::sx_*): an element's
+// photo-absorption at E uses its row r = the last with edge[r] <= E (r >= 1),
+// else row 0 (ProxySandiaData's table, as the reference's GetSandiaCofPerAtom
+// reads it).  Between two consecutive edges of the union of all elements'
+// edges every element's row is fixed, so one search over the union gives all
+// rows, and the per-element arithmetic is unchanged.
+static int build_sandia_intervals(HostData &hd, DevMaterial &m, std::string &err) {
+  std::vector<double> B;
+  for (int i = 0; i < m.nel; ++i) {
+    const DevElement &e = m.el[i];
+    const double *rows = hd.sandia.data() + e.sandia;
+    for (int r = 1; r < e.nint; ++r) {
+      if (!(rows[5 * r] > rows[5 * (r - 1)])) {
+        err = "sandia.dat: edges of Z=" + std::to_string(e.Z) + " not ascending";
+        return 1;
+      }
+      B.push_back(rows[5 * r]);
+    }
+  }
+  std::sort(B.begin(), B.end());
+  B.erase(std::unique(B.begin(), B.end()), B.end());
+  const int nb = (int)B.size();
+  m.sx_b = (int)hd.sx.size();
+  m.sx_n = nb;
+  hd.sx.insert(hd.sx.end(), B.begin(), B.end());
+  m.sx_c = (int)hd.sx.size();
+  for (int j = 0; j <= nb; ++j) {
+    for (int i = 0; i < m.nel; ++i) {
+      const DevElement &e = m.el[i];
+      const double *rows = hd.sandia.data() + e.sandia;
+      int r = 0;
+      if (j > 0)
+        for (int k = 1; k < e.nint; ++k)
+          if (rows[5 * k] <= B[j - 1]) r = k;
+      for (int k = 1; k <= 4; ++k) hd.sx.push_back(rows[5 * r + k]);
+    }
+  }
+  // hint k: the interval of u_k = 2^(l0 + k / hscale), the lower end of bin k
+  const int K = 128;
+  const double l0 = nb ? std::log2(B.front()) : 0.0, l1 = nb ? std::log2(B.back()) : 1.0;
+  m.sx_hint = (int)hd.hints.size();
+  m.sx_nhint = K;
+  m.sx_hlog0 = (float)l0;
+  m.sx_hscale = (float)(K / std::max(l1 - l0, 1e-3));
+  for (int k = 0; k < K; ++k) {
+    const double u = std::exp2(l0 + k / (double)m.sx_hscale);
+    int j = 0;
+    while (j < nb && B[j] <= u) ++j;
+    hd.hints.push_back(j);
+  }
+  return 0;
+}
+
 int load_material(const std::string &data, int which, double lar_density, HostData &hd, int slot, std::string &err) {
   std::string dir;
   int couple = 0;
@@ -228,6 +282,7 @@ int load_material(const std::string &data, int which, double lar_density, HostDa
     e.sb = (int)hd.sb.size();
     hd.sb.insert(hd.sb.end(), grid.begin(), grid.end());
   }
+  if (build_sandia_intervals(hd, m, err)) return GXT_EDATA;
   m.migdal = m.ne * 4.0 * kPi * kClassicRe * kComptonLength * kComptonLength;
   m.cut_length = 0.7;
   if (!read_cut_file(dir + "/cut.dat", couple, m.cut_gamma, m.cut_e)) {

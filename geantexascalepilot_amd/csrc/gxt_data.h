@@ -13,6 +13,7 @@ struct HostData {
   std::vector<double> x, y, d2;
   std::vector<double> sb;
   std::vector<double> sandia;
+  std::vector<double> sx;      // merged Sandia intervals of the materials
   std::vector<double> shells;
   std::vector<int> hints;
 };

@@ -445,21 +445,40 @@ __device__ __forceinline__ R sb_value(const R *g, R x, R y) {
          ((x2 - x1) * (y2 - y1));
 }
 
-// Sandia photo-absorption per atom (mm2), interval = last edge <= E.
-__device__ __forceinline__ double sandia_atom(const DevParams *P, const DevElement &el, double ekin) {
-  const double ek = ekin * 1.0e3;
-  const double *rows = P->sandia + el.sandia;
-  int row = el.nint - 1;
-  while (row > 0 && ek < rows[5 * row]) --row;
-  const double *c = rows + 5 * row;
+// Merged Sandia interval of material m at ek (keV): j with B[j-1] <= ek <
+// B[j] over its breakpoints B (DevMaterial::sx_*).  The search starts from
+// the hint of the log2 bin one below ek's (the FP32 log2 is off by far less
+// than a bin) and steps to the interval: one or two dependent loads, where a
+// walk over each element's rows took up to its number of intervals.
+__device__ __forceinline__ int sandia_interval(const DevParams *P, const DevMaterial &m, double ek) {
+  const double *B = P->sx + m.sx_b;
+  const int nb = m.sx_n;
+  int k = (int)floorf((__log2f((float)ek) - m.sx_hlog0) * m.sx_hscale) - 1;
+  k = k < 0 ? 0 : (k >= m.sx_nhint ? m.sx_nhint - 1 : k);
+  int j = P->hints[m.sx_hint + k];
+  while (j > 0 && B[j - 1] > ek) --j;
+  while (j < nb && B[j] <= ek) ++j;
+  return j;
+}
+
+// Sandia photo-absorption per atom (mm2) of an element with coefficients c
+// (a1..a4 of its row at ek, in keV).
+__device__ __forceinline__ double sandia_atom(const DevElement &el, const double *c, double ek) {
   const double inv = 1.0 / ek;
-  const double s = inv * (c[1] + inv * (c[2] + inv * (c[3] + inv * c[4])));
+  const double s = inv * (c[0] + inv * (c[1] + inv * (c[2] + inv * c[3])));
   return (el.Z / el.zoa) / kAvogadro * s * 100.0;
 }
 
+// Coefficients of the elements of m at ek: element i's at the result + 4 i.
+__device__ __forceinline__ const double *sandia_coefs(const DevParams *P, const DevMaterial &m, double ek) {
+  return P->sx + m.sx_c + sandia_interval(P, m, ek) * m.nel * 4;
+}
+
 __device__ __forceinline__ double sandia_macro(const DevParams *P, const DevMaterial &m, double ekin) {
+  const double ek = ekin * 1.0e3;
+  const double *c = sandia_coefs(P, m, ek);
   double s = 0;
-  for (int i = 0; i < m.nel; ++i) s += m.el[i].natoms * sandia_atom(P, m.el[i], ekin);
+  for (int i = 0; i < m.nel; ++i) s += m.el[i].natoms * sandia_atom(m.el[i], c + 4 * i, ek);
   return s;
 }
 
@@ -558,8 +577,10 @@ template <class R>
 __device__ __forceinline__ int select_element(const DevParams *P, const DevMaterial &m, int kind, R e, Rng &rng) {
   if (m.nel == 1) return 0;
   double w[kMaxElem], tot = 0;
+  const double ek = (double)e * 1.0e3;
+  const double *c = kind == 0 ? nullptr : sandia_coefs(P, m, ek);
   for (int i = 0; i < m.nel; ++i) {
-    w[i] = kind == 0 ? m.el[i].wbrem : m.el[i].natoms * sandia_atom(P, m.el[i], (double)e);
+    w[i] = kind == 0 ? m.el[i].wbrem : m.el[i].natoms * sandia_atom(m.el[i], c + 4 * i, ek);
     tot += w[i];
   }
   const double r = rng.uniform() * tot;
@@ -798,11 +819,18 @@ __device__ __forceinline__ void sample_brems(const DevParams *P, const DevMateri
   const R a = m_fabs(c);
   const R delta = R(0.5) * (m_sqrt(R(4.0) + a * a) + a);
   const R cofA = -signc * m_pow(delta, R(1.0) / R(3.0));
-  R cost = cofA - R(1) / cofA;
+  const R cost0 = cofA - R(1) / cofA;
+  // the boost to the lab, (cos + beta) / (1 + cos beta), in forms without
+  // cancellation near the forward direction: 1 - beta =
+  // 1 / (gam (gam + sqrt(tau (tau + 2)))) and 1 - cos' = (1 - cos)(1 - beta) /
+  // (1 + cos beta); beta itself rounds to 1 in FP32 above about 1 GeV
   const R tau = t0 / R(kMe);
-  const R beta = m_sqrt(tau * (tau + R(2))) / (tau + R(1));
-  cost = (cost + beta) / (R(1) + cost * beta);
-  const R sint = m_sqrt((R(1) - cost) * (R(1) + cost));
+  const R gam = tau + R(1);
+  const R bgam = m_sqrt(tau * (tau + R(2)));
+  const R omb = R(1) / (gam * (gam + bgam));
+  const R omc = (R(1) - cost0) * omb / ((R(1) + cost0) - cost0 * omb);
+  const R cost = R(1) - omc;
+  const R sint = m_sqrt(rmax(omc * (R(2) - omc), R(0)));
   const R phi = R(kTwoPi) * rng.template u<R>();
   Sec<R> &g = o.sec[0];
   g.species = 0;

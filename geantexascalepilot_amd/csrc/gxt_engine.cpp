@@ -124,7 +124,7 @@ struct DeviceData {
   DevParams *P = nullptr;
   Node64 *nd = nullptr;
   Node32 *nf = nullptr;
-  double *sbd = nullptr, *sandia = nullptr, *shells = nullptr;
+  double *sbd = nullptr, *sandia = nullptr, *shells = nullptr, *sx = nullptr;
   float *sbf = nullptr;
   uint32_t *jump8 = nullptr;
   int *hints = nullptr;
@@ -136,6 +136,7 @@ struct DeviceData {
     (void)hipFree(nf);
     (void)hipFree(sbd);
     (void)hipFree(sandia);
+    (void)hipFree(sx);
     (void)hipFree(shells);
     (void)hipFree(sbf);
     *this = DeviceData();
@@ -165,6 +166,7 @@ int upload(HostData &hd, DeviceData &dd) {
   HIP_TRY(put(&dd.sbd, hd.sb));
   HIP_TRY(put(&dd.sbf, tof(hd.sb)));
   HIP_TRY(put(&dd.sandia, hd.sandia));
+  HIP_TRY(put(&dd.sx, hd.sx));
   HIP_TRY(put(&dd.shells, hd.shells));
   HIP_TRY(put(&dd.hints, hd.hints));
   DevParams &p = hd.params;
@@ -182,6 +184,7 @@ int upload(HostData &hd, DeviceData &dd) {
   p.sbd = dd.sbd;
   p.sbf = dd.sbf;
   p.sandia = dd.sandia;
+  p.sx = dd.sx;
   p.shells = dd.shells;
   HIP_TRY(dalloc(&dd.P, 1));
   HIP_TRY(hipMemcpy(dd.P, &p, sizeof(p), hipMemcpyHostToDevice));
@@ -272,7 +275,8 @@ struct gxt_ctx {
   Counters *cnt = nullptr;
   Counters *hcnt = nullptr;  // pinned
   DevEvent *ev = nullptr;
-  int *ev_live = nullptr;    // live tracks per event slot
+  int *ev_live = nullptr;    // live tracks of event slot e at ev_live[e * kLiveStride]
+  int *ev_live_packed = nullptr;  // ev_live[e * kLiveStride] at [e], for the host's snapshot
   int *hev_live = nullptr;   // pinned mirror
   double *rows = nullptr;
   unsigned long long *spec = nullptr;
@@ -351,7 +355,7 @@ namespace {
 
 int64_t live_tracks(const Counters &h) {
   int64_t n = 0;
-  for (int s = 0; s < kStripes; ++s) n += (int64_t)h.cur[0][s] + h.cur[1][s];
+  for (int s = 0; s < kStripes; ++s) n += (int64_t)h.ctr(kCur0, s) + h.ctr(kCur1, s);
   return n;
 }
 
@@ -445,7 +449,8 @@ int admit(gxt_ctx *c) {
   // the look-ahead iterations still running do not touch them
   for (auto &r : zero_ranges) {
     HIP_TRY(hipMemsetAsync(c->ev + r.first, 0, r.second * sizeof(DevEvent), c->stream));
-    HIP_TRY(hipMemsetAsync(c->ev_live + r.first, 0, r.second * sizeof(int), c->stream));
+    HIP_TRY(hipMemsetAsync(c->ev_live + (size_t)r.first * kLiveStride, 0, (size_t)r.second * kLiveStride * sizeof(int),
+                           c->stream));
   }
   const StepArgs a = c->args();
   for (const Range &r : ranges) {
@@ -633,7 +638,8 @@ int round_trip(gxt_ctx *c, int k, bool &idle) {
     if (rc) return rc;
   }
   HIP_TRY(hipMemcpyAsync(c->hcnt, c->cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->hev_live, c->ev_live, c->cfg.max_events * sizeof(int), hipMemcpyDeviceToHost,
+  HIP_TRY(launch_pack_live(c->ev_live, c->ev_live_packed, c->cfg.max_events, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->hev_live, c->ev_live_packed, c->cfg.max_events * sizeof(int), hipMemcpyDeviceToHost,
                          c->stream));
   HIP_TRY(hipEventRecord(c->snap, c->stream));
   c->ev_snap = c->ev_pending.size();
@@ -692,6 +698,7 @@ void destroy_ctx(gxt_ctx *c) {
   (void)hipHostFree(c->hcnt);
   (void)hipFree(c->ev);
   (void)hipFree(c->ev_live);
+  (void)hipFree(c->ev_live_packed);
   (void)hipHostFree(c->hev_live);
   (void)hipFree(c->rows);
   (void)hipFree(c->spec);
@@ -972,8 +979,10 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
   std::memset(c->hcnt, 0, sizeof(Counters));
   HIP_TRY(dalloc(&c->ev, cfg->max_events));
   HIP_TRY(hipMemset(c->ev, 0, sizeof(DevEvent) * cfg->max_events));
-  HIP_TRY(dalloc(&c->ev_live, cfg->max_events));
-  HIP_TRY(hipMemset(c->ev_live, 0, sizeof(int) * cfg->max_events));
+  HIP_TRY(dalloc(&c->ev_live, (size_t)cfg->max_events * kLiveStride));
+  HIP_TRY(hipMemset(c->ev_live, 0, sizeof(int) * (size_t)cfg->max_events * kLiveStride));
+  HIP_TRY(dalloc(&c->ev_live_packed, cfg->max_events));
+  HIP_TRY(hipMemset(c->ev_live_packed, 0, sizeof(int) * cfg->max_events));
   HIP_TRY(hipHostMalloc((void **)&c->hev_live, sizeof(int) * cfg->max_events, hipHostMallocDefault));
   std::memset(c->hev_live, 0, sizeof(int) * cfg->max_events);
   c->row_stride = cfg->n_zbins + cfg->n_rbins + P.geo.nvol;
@@ -1160,10 +1169,10 @@ int gxt_get_stats(gxt_ctx *c, gxt_stats *o) {
   HIP_TRY(hipStreamSynchronize(c->stream));
   Counters h;
   HIP_TRY(hipMemcpy(&h, c->cnt, sizeof(h), hipMemcpyDeviceToHost));
-  o->track_steps = (int64_t)h.steps;
+  o->track_steps = (int64_t)h.total(kStatSteps);
   o->iterations = c->iterations;
   o->max_live = c->max_live;
-  o->killed_by_limit = (int64_t)h.killed;
+  o->killed_by_limit = (int64_t)h.total(kStatKilled);
   o->wall_ms = c->wall_ms;
   return GXT_OK;
 }
@@ -1174,13 +1183,13 @@ int gxt_get_kernel_stats(gxt_ctx *c, gxt_kernel_stats *o) {
   Counters h;
   HIP_TRY(hipMemcpy(&h, c->cnt, sizeof(h), hipMemcpyDeviceToHost));
   *o = c->kst;
-  o->track_steps = (int64_t)h.steps;
-  o->charged_steps = (int64_t)h.steps_charged;
-  o->neutral_steps = (int64_t)(h.steps - h.steps_charged);
-  o->interactions = (int64_t)h.interactions;
-  o->created = (int64_t)h.created;
-  o->neutral_loaded = (int64_t)h.loaded[0];
-  o->charged_loaded = (int64_t)h.loaded[1];
+  o->track_steps = (int64_t)h.total(kStatSteps);
+  o->charged_steps = (int64_t)h.total(kStatStepsCharged);
+  o->neutral_steps = (int64_t)(h.total(kStatSteps) - h.total(kStatStepsCharged));
+  o->interactions = (int64_t)h.total(kStatInteractions);
+  o->created = (int64_t)h.total(kStatCreated);
+  o->neutral_loaded = (int64_t)h.total(kStatLoaded0);
+  o->charged_loaded = (int64_t)h.total(kStatLoaded1);
   // x, y, z (FP64) + dir, E, 3 interaction lengths (R) + 6 RNG words + event, meta, nsteps
   // + 3 FP64 per-track tallies
   const int r = c->precision == GXT_FP32 ? 4 : 8;
@@ -1196,13 +1205,8 @@ int gxt_reset_tallies(gxt_ctx *c) {
   HIP_TRY(hipMemset(c->spec, 0, sizeof(unsigned long long) * c->grid * 3 * kSpecBins));
   Counters h;
   HIP_TRY(hipMemcpy(&h, c->cnt, sizeof(h), hipMemcpyDeviceToHost));
-  h.steps = 0;
-  h.steps_charged = 0;
-  h.killed = 0;
+  std::memset(h.stat, 0, sizeof(h.stat));
   h.trace_n = 0;
-  h.interactions = 0;
-  h.created = 0;
-  h.loaded[0] = h.loaded[1] = 0;
   c->kst = gxt_kernel_stats{};
   HIP_TRY(hipMemcpy(c->cnt, &h, sizeof(h), hipMemcpyHostToDevice));
   c->iterations = 0;

@@ -23,7 +23,7 @@ struct StepArgs {
   const DevParams *P;
   Counters *cnt;
   DevEvent *ev;
-  int *ev_live;      // live tracks per event slot
+  int *ev_live;      // live tracks of event slot e at ev_live[e * kLiveStride]
   int32_t *queues;  // kNumQueues x cap
   int32_t *newlist; // next-pool slots of the tracks created this iteration: 2 lists of cap
                     // (photon interactions, e-/e+ interactions)
@@ -78,6 +78,8 @@ template <class R, class In>
 hipError_t launch_inject(const StepArgs &a, const PoolArgs<R> &pa, const In *prim, int n, int event_base,
                          hipStream_t st);
 
+// out[e] = ev_live[e * kLiveStride] for e < n (the host's snapshot of the live counts).
+hipError_t launch_pack_live(const int *ev_live, int *out, int n, hipStream_t st);
 hipError_t launch_rng_init(const DevParams *P, const uint32_t *seed, const uint64_t *idx, int n, uint32_t *out,
                            hipStream_t st);
 hipError_t launch_rng_draw(uint32_t *states, int n, int k, double *out, hipStream_t st);

@@ -121,7 +121,7 @@ __device__ __forceinline__ int64_t queue_reserve(int q, Counters *cnt, int64_t c
   for (int k = 0; k < NQ; ++k) {
     m[k] = __ballot(q == Q0 + k);
     base[k] = 0;
-    if (m[k] && lane() == __ffsll((long long)m[k]) - 1) base[k] = atomicAdd(&cnt->q[Q0 + k][st], (int)__popcll(m[k]));
+    if (m[k] && lane() == __ffsll((long long)m[k]) - 1) base[k] = atomicAdd(&cnt->ctr(kQ0 + Q0 + k, st), (int)__popcll(m[k]));
   }
   int64_t pos = -1;
 #pragma unroll
@@ -475,12 +475,12 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
         unsafeAtomicAdd(&a.ev[t.event].leak, p.ekin);
       } else {
         region = t.species == 0 ? 0 : 1;
-        atomicAdd(&a.ev_live[t.event], 1);
+        atomicAdd(&a.ev_live[(int64_t)t.event * kLiveStride], 1);
       }
     }
     const int st = my_stripe();
-    const int b0 = wave_reserve(region == 0 ? 1 : 0, &a.cnt->cur[0][st]);
-    const int b1 = wave_reserve(region == 1 ? 1 : 0, &a.cnt->cur[1][st]);
+    const int b0 = wave_reserve(region == 0 ? 1 : 0, &a.cnt->ctr(kCur0, st));
+    const int b1 = wave_reserve(region == 1 ? 1 : 0, &a.cnt->ctr(kCur1, st));
     if (region >= 0) {
       const int64_t s = out_slot(region, st, region == 0 ? b0 : b1, stripe_cap(cur));
       if (s >= 0)
@@ -523,7 +523,7 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
   bool bad = false;
 #pragma unroll
   for (int k = 0; k < kStripes; ++k) {
-    const int64_t r0 = a.cnt->cur[0][k], r1 = a.cnt->cur[1][k];
+    const int64_t r0 = a.cnt->ctr(kCur0, k), r1 = a.cnt->ctr(kCur1, k);
     bad = bad || r0 < 0 || r1 < 0 || r0 + r1 > sc;
     const int64_t n0 = min(max(r0, (int64_t)0), sc);
     const int64_t n1 = min(max(r1, (int64_t)0), sc - n0);
@@ -663,16 +663,16 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
     // returning atomics first, all in flight together: the survivor's slot in
     // the next pool and the queue entry
     WaveReserve rs;
-    rs.issue(survive ? 1 : 0, &a.cnt->nxt[kCharged ? 1 : 0][st]);
+    rs.issue(survive ? 1 : 0, &a.cnt->ctr(kCharged ? kNxt1 : kNxt0, st));
     const int64_t qpos = queue_reserve<kCharged ? 3 : 0, kCharged ? kNumQueues - 3 : 3>(q, a.cnt, cur.cap, sc);
     const int bs = rs.get();
     // then stores and no-return atomics
     if (active) {
       if (escaped) unsafeAtomicAdd(&a.ev[t.event].leak, (double)t.e);
-      if (killed) atomicAdd(&a.cnt->killed, 1ULL);
+      if (killed) atomicAdd(&a.cnt->stat[my_stripe()][kStatKilled], 1ULL);
       if (dies) {
         retire(a, t);
-        atomicAdd(&a.ev_live[t.event], -1);
+        atomicAdd(&a.ev_live[(int64_t)t.event * kLiveStride], -1);
       }
       if (q >= 0) {
         // interacting (or stopped) track: back in place for the interaction kernel
@@ -697,9 +697,10 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
       const int tot = __shfl(ns, 63, 64);
       const unsigned long long nload = (unsigned long long)__popcll(__ballot(active));
       if (tot > 0 && lane() == 63) {
-        atomicAdd(&a.cnt->steps, (unsigned long long)tot);
-        if (kCharged) atomicAdd(&a.cnt->steps_charged, (unsigned long long)tot);
-        atomicAdd(&a.cnt->loaded[kCharged ? 1 : 0], nload);
+        unsigned long long *sp = a.cnt->stat[st];
+        atomicAdd(&sp[kStatSteps], (unsigned long long)tot);
+        if (kCharged) atomicAdd(&sp[kStatStepsCharged], (unsigned long long)tot);
+        atomicAdd(&sp[kCharged ? kStatLoaded1 : kStatLoaded0], nload);
       }
     }
   }
@@ -744,14 +745,14 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 8 ? 3 : GXT_STEP_WAVES) void k
   // photon queues 0-2 in one variant, e-/e+ queues 3-8 in the other; the
   // variant's (queue, stripe) segments are walked in order through a prefix
   // of their lengths (clamped to the stripe capacity)
-  constexpr int kQ0 = kCharged ? 3 : 0;
+  constexpr int kQ0q = kCharged ? 3 : 0;
   constexpr int kNQ = kCharged ? kNumQueues - 3 : 3;
   __shared__ int seg[kNQ * kStripes + 1];
   if (threadIdx.x == 0) {
     int acc = 0;
     seg[0] = 0;
     for (int k = 0; k < kNQ * kStripes; ++k) {
-      acc += (int)min((int64_t)max(a.cnt->q[kQ0 + k / kStripes][k % kStripes], 0), sc);
+      acc += (int)min((int64_t)max(a.cnt->ctr(kQ0 + kQ0q + k / kStripes, k % kStripes), 0), sc);
       seg[k + 1] = acc;
     }
   }
@@ -780,7 +781,7 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 8 ? 3 : GXT_STEP_WAVES) void k
         else
           hi = mid;
       }
-      const int q = kQ0 + lo / kStripes;
+      const int q = kQ0q + lo / kStripes;
       const int qs = lo % kStripes;
       const int slot = a.queues[(int64_t)q * cur.cap + (int64_t)qs * sc + (i - seg[lo])];
       load_track(cur, slot, t, trace);
@@ -831,18 +832,18 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 8 ? 3 : GXT_STEP_WAVES) void k
     // returning atomics first, all in flight together; then the stores
     const int made = outN + outC - (survive ? 1 : 0);
     WaveReserve rN, rC, rK;
-    rN.issue(outN, &a.cnt->nxt[0][st]);
-    rC.issue(outC, &a.cnt->nxt[1][st]);
-    rK.issue(made, &a.cnt->nnew[kCharged ? 1 : 0][st]);
+    rN.issue(outN, &a.cnt->ctr(kNxt0, st));
+    rC.issue(outC, &a.cnt->ctr(kNxt1, st));
+    rK.issue(made, &a.cnt->ctr(kCharged ? kNew1 : kNew0, st));
     const int bN = rN.get(), bC = rC.get();
     int nk = rK.get();
     if (active) {
       dp.flush(wh);
       if (pair) atomicAdd(&a.ev[t.event].npairs, 1ULL);
       if (nannih) atomicAdd(&a.ev[t.event].nannihil, (unsigned long long)nannih);
-      if (killed) atomicAdd(&a.cnt->killed, 1ULL);
+      if (killed) atomicAdd(&a.cnt->stat[my_stripe()][kStatKilled], 1ULL);
       if (!survive) retire(a, t);
-      if (outN + outC != 1) atomicAdd(&a.ev_live[t.event], outN + outC - 1);
+      if (outN + outC != 1) atomicAdd(&a.ev_live[(int64_t)t.event * kLiveStride], outN + outC - 1);
       // new tracks in the oracle's draw order: secondary 0, secondary 1, then
       // the primary's annihilation at rest; the survivor last
       int kN = bN, kC = bC;
@@ -880,10 +881,10 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 8 ? 3 : GXT_STEP_WAVES) void k
           atomicOr(&a.cnt->overflow, 1);
       }
     }
-    wave_add_u64(&a.cnt->interactions, active);
+    wave_add_u64(&a.cnt->stat[st][kStatInteractions], active);
     {
       const int tot = __shfl(rK.incl, 63, 64);
-      if (tot > 0 && lane() == 63) atomicAdd(&a.cnt->created, (unsigned long long)tot);
+      if (tot > 0 && lane() == 63) atomicAdd(&a.cnt->stat[st][kStatCreated], (unsigned long long)tot);
     }
   }
   wh_flush(a, wh);
@@ -902,7 +903,7 @@ __global__ __launch_bounds__(kBlock) void k_derive(StepArgs a, PoolArgs<R> pa) {
   span.pre[0] = 0;
 #pragma unroll
   for (int k = 0; k < kStripes; ++k)
-    span.pre[k + 1] = span.pre[k] + (int)min((int64_t)max(a.cnt->nnew[kList][k], 0), sc);
+    span.pre[k + 1] = span.pre[k] + (int)min((int64_t)max(a.cnt->ctr(kNew0 + kList, k), 0), sc);
   const int n = span.total();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     int is, off;
@@ -921,19 +922,19 @@ __global__ void k_advance(Counters *c, int64_t cap) {
     const int64_t sc = cap / kStripes;
     bool bad = c->overflow != 0;
     for (int s = 0; s < kStripes; ++s) {
-      const int64_t n0 = c->nxt[0][s], n1 = c->nxt[1][s];
+      const int64_t n0 = c->ctr(kNxt0, s), n1 = c->ctr(kNxt1, s);
       bad = bad || n0 < 0 || n1 < 0 || n0 + n1 > sc;
     }
     // an overflowing pool is dropped whole: the host reports GXT_ECAPACITY
     if (bad) c->overflow = 1;
     for (int s = 0; s < kStripes; ++s) {
-      c->cur[0][s] = bad ? 0 : c->nxt[0][s];
-      c->cur[1][s] = bad ? 0 : c->nxt[1][s];
-      c->nxt[0][s] = 0;
-      c->nxt[1][s] = 0;
-      c->nnew[0][s] = 0;
-      c->nnew[1][s] = 0;
-      for (int k = 0; k < kNumQueues; ++k) c->q[k][s] = 0;
+      c->ctr(kCur0, s) = bad ? 0 : c->ctr(kNxt0, s);
+      c->ctr(kCur1, s) = bad ? 0 : c->ctr(kNxt1, s);
+      c->ctr(kNxt0, s) = 0;
+      c->ctr(kNxt1, s) = 0;
+      c->ctr(kNew0, s) = 0;
+      c->ctr(kNew1, s) = 0;
+      for (int k = 0; k < kNumQueues; ++k) c->ctr(kQ0 + k, s) = 0;
     }
   }
 }
@@ -1232,6 +1233,17 @@ template hipError_t launch_inject<double, gxt_track>(const StepArgs &, const Poo
                                                      int, int, hipStream_t);
 template hipError_t launch_inject<float, gxt_track>(const StepArgs &, const PoolArgs<float> &, const gxt_track *, int,
                                                     int, hipStream_t);
+
+__global__ void k_pack_live(const int *ev_live, int *out, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) out[e] = ev_live[(int64_t)e * kLiveStride];
+}
+
+hipError_t launch_pack_live(const int *ev_live, int *out, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pack_live, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, ev_live, out, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_rng_init(const DevParams *P, const uint32_t *seed, const uint64_t *idx, int n, uint32_t *out,
                            hipStream_t st) {

@@ -73,6 +73,14 @@ struct DevMaterial {
   int pad;
   DevElement el[kMaxElem];
   DevTable tab[GXT_NTAB];
+  // Sandia photo-absorption by merged interval: the sx_n breakpoints
+  // sx[sx_b .. sx_b + sx_n) (keV, ascending) are every element's interval
+  // edges but its first; interval j (sx[sx_b + j - 1] <= E < sx[sx_b + j])
+  // holds each element's coefficients a1..a4 at sx[sx_c + (j * nel + i) * 4].
+  // A search starts at hints[sx_hint + k], the interval of the lower end of
+  // log2 bin k (sx_hscale bins per octave from sx_hlog0).
+  int sx_b, sx_c, sx_n, sx_hint, sx_nhint;
+  float sx_hlog0, sx_hscale;
 };
 
 struct DevGeometry {
@@ -166,6 +174,7 @@ struct DevParams {
   const double *sbd;            // Seltzer-Berger grids, FP64
   const float *sbf;             // FP32 mirror
   const double *sandia;         // Sandia rows (edge keV, a1..a4)
+  const double *sx;             // merged Sandia breakpoints and coefficients (DevMaterial::sx_*)
   const double *shells;         // binding energies (MeV)
   const int *hints;             // search hints of the inverse-range vectors
 };
@@ -189,26 +198,40 @@ struct Pool {
   double *tedep;      // step deposit carried to the interaction kernel (tracing only)
 };
 
-// Device counters of the step loop, one per stripe.
-struct Counters {
-  int32_t cur[2][kStripes];          // live neutral / charged tracks in the current pool
-  int32_t nxt[2][kStripes];          // written to the next pool
-  int32_t q[kNumQueues][kStripes];   // interaction queue lengths
-  int32_t nnew[2][kStripes];         // tracks created this iteration by the photon / e-/e+
-                                     // interactions (entries of the two new-track lists)
+// Device counters of the step loop.  The stripe counters (kinds below) of
+// stripe s are addressed ctr(kind, s); each stripe's counters sit on cache
+// lines no other stripe uses, so the returning atomics of one stripe's
+// workgroups never wait behind another stripe's.  Statistics are per stripe
+// as well (summed by the host).
+enum CounterKind { kCur0 = 0, kCur1 = 1, kNxt0 = 2, kNxt1 = 3, kQ0 = 4, kNew0 = 4 + kNumQueues, kNew1 = 5 + kNumQueues,
+                   kNumKinds = 6 + kNumQueues };
+constexpr int kKindStride = 32;      // ints between the counters of one stripe
+constexpr int kStripeStride = 512;  // ints between stripes
+enum StatKind { kStatSteps = 0, kStatStepsCharged, kStatKilled, kStatInteractions, kStatCreated, kStatLoaded0,
+                kStatLoaded1, kNumStats };
+struct alignas(256) Counters {
+  int32_t c[kStripes * kStripeStride];
+  unsigned long long stat[kStripes][16];  // 128 B per stripe
   int32_t overflow;        // a pool or queue write would have gone out of range
-  int32_t pad[3];
-  unsigned long long steps;   // track-steps
-  unsigned long long steps_charged;  // of which e-/e+ steps
-  unsigned long long killed;  // tracks killed by the step limit
+  int32_t pad[31];
   unsigned long long trace_n; // trace records written
-  unsigned long long interactions;  // tracks sampled by the interaction kernel
-  unsigned long long created;       // tracks created by the interaction kernel
-  unsigned long long loaded[2];     // track records read by the neutral / charged transport launches
+  __host__ __device__ int32_t &ctr(int kind, int s) { return c[s * kStripeStride + kind * kKindStride]; }
+  __host__ __device__ int32_t ctr(int kind, int s) const { return c[s * kStripeStride + kind * kKindStride]; }
+  __host__ __device__ unsigned long long total(int k) const {
+    unsigned long long t = 0;
+    for (int s = 0; s < kStripes; ++s) t += stat[s][k];
+    return t;
+  }
 };
 
-// Per-event tallies on the device (FP64, atomics).
-struct DevEvent {
+// Live-track count of event e at ev_live[e * kLiveStride]: one 128-byte line
+// per event, so the +/- 1 atomics of one event's tracks never queue behind
+// another event's.
+constexpr int kLiveStride = 32;
+
+// Per-event tallies on the device (FP64, atomics), one or more whole 128-byte
+// lines per event for the same reason.
+struct alignas(128) DevEvent {
   double edep, visible, leak, clen;
   unsigned long long nsec[3];
   unsigned long long nsteps, npairs, nannihil;

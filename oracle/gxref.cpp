@@ -1343,11 +1343,18 @@ static void sample_brems(const Material &m, double t0, const double d0[3], uint3
   const double a = std::fabs(c);
   const double delta = 0.5 * (std::sqrt(4.0 + a * a) + a);
   const double cofA = -signc * pow_cw(delta, 1.0 / 3.0);
-  double cost = cofA - 1.0 / cofA;
+  const double cost0 = cofA - 1.0 / cofA;
+  // the boost to the lab (ProxySeltzerBerger.hpp:295-310 writes (cos + beta) /
+  // (1 + cos beta)) in forms without cancellation near the forward
+  // direction: 1 - beta = 1 / (gam (gam + sqrt(tau (tau + 2)))) and
+  // 1 - cos' = (1 - cos)(1 - beta) / (1 + cos beta)
   const double tau = t0 / kMe;
-  const double beta = std::sqrt(tau * (tau + 2.0)) / (tau + 1.0);
-  cost = (cost + beta) / (1.0 + cost * beta);
-  const double sint = std::sqrt((1.0 - cost) * (1.0 + cost));
+  const double gam = tau + 1.0;
+  const double bgam = std::sqrt(tau * (tau + 2.0));
+  const double omb = 1.0 / (gam * (gam + bgam));
+  const double omc = (1.0 - cost0) * omb / ((1.0 + cost0) - cost0 * omb);
+  const double cost = 1.0 - omc;
+  const double sint = std::sqrt(std::max(omc * (2.0 - omc), 0.0));
   const double phi = kTwoPi * mrg_uniform(s);
   Secondary &g = o.sec[0];
   g.species = 0;
