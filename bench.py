@@ -244,61 +244,100 @@ def navigation(g, abi, n=1 << 24, reps=20, cpu=False, seed=99):
     return out
 
 
-def serving(g, abi, seconds, rate_jobs_s, seed=4242):
-    """Poisson stream of jobs (40 % one 1 GeV gamma, 40 % 10 x 10 GeV e-,
-    20 % 100 x 50 GeV e-) into the PbWO4 block, FP32 form, one shared pool."""
+MIXES = {
+    # kind: (probability, events per job, species, energy MeV)
+    "c4": [("gamma1", 0.4, 1, 0, 1000.0), ("e10x10", 0.4, 10, 1, 10000.0), ("e100x50", 0.2, 100, 1, 50000.0)],
+    # 10 GeV e- in jobs of 100 events: the headline workload as a stream (the
+    # job size keeps the submitting thread far below its limit of a few
+    # thousand jobs/s)
+    "e10": [("e10x100", 1.0, 100, 1, 10000.0)],
+}
+
+
+def mix_gev_per_job(mix):
+    return sum(p * n * e * 1e-3 for _, p, n, _, e in MIXES[mix])
+
+
+def serving(g, abi, seconds, rate_jobs_s, mix="c4", warm_s=2.0, drain_s=60.0, seed=4242):
+    """Poisson stream of jobs into the PbWO4 block, FP32 form, one shared pool,
+    submitted for `seconds`; mix "c4": 40 % one 1 GeV gamma, 40 % 10 x 10 GeV
+    e-, 20 % 100 x 50 GeV e-; mix "e10": every job 10 x 10 GeV e-.  The
+    throughput is the completion rate inside the submission window after
+    warm_s (so it saturates at the capacity when the offered rate exceeds
+    it); latencies are those of the jobs submitted after warm_s, with a
+    drain of at most drain_s after the window."""
     import numpy as np
     rs = np.random.default_rng(seed)
-    cfg = abi.config_pbwo4_block(max_events=20000, precision=abi.FP32, max_jobs=256, pool_capacity=8_000_000)
-    kinds = []
+    cfg = abi.config_pbwo4_block(max_events=40000, precision=abi.FP32, max_jobs=4096, pool_capacity=8_000_000)
+    kinds, sub_t, done = [], [], []
     with g.Transport(cfg) as t:
         t.serve_start()
         t_start = time.perf_counter()
-        next_sub = 0
-        first = 0
-        submitted = 0
-        done = []
+        next_sub, first, submitted = 0.0, 0, 0
         while True:
             now = time.perf_counter() - t_start
             if now >= seconds:
                 break
             if now >= next_sub:
-                u = rs.random()
-                if u < 0.4:
-                    kind, n, sp, e = "gamma1", 1, abi.GAMMA, 1000.0
-                elif u < 0.8:
-                    kind, n, sp, e = "e10x10", 10, abi.ELECTRON, 10000.0
-                else:
-                    kind, n, sp, e = "e100x50", 100, abi.ELECTRON, 50000.0
-                p = g.make_primaries(777, sp, e, e, 0.0, n, first=first, n_total=1 << 40)
+                u, acc = rs.random(), 0.0
+                for kind, p, n, sp, e in MIXES[mix]:
+                    acc += p
+                    if u < acc:
+                        break
+                prims = g.make_primaries(777, sp, e, e, 0.0, n, first=first, n_total=1 << 40)
                 first += n
-                t.submit(p, submitted)
+                t.submit(prims, submitted)
                 kinds.append(kind)
+                sub_t.append(now)
                 submitted += 1
                 next_sub = now + rs.exponential(1.0 / rate_jobs_s)
-            done += t.poll()
-            time.sleep(0.0005)
-        # drain
-        t_end_submit = time.perf_counter() - t_start
-        while len(done) < submitted:
-            done += t.poll()
+            for r in t.poll():
+                done.append((time.perf_counter() - t_start, r))
+            time.sleep(0.0002)
+        t_window = time.perf_counter() - t_start
+        while len(done) < submitted and time.perf_counter() - t_start < seconds + drain_s:
+            for r in t.poll():
+                done.append((time.perf_counter() - t_start, r))
             time.sleep(0.002)
-            if time.perf_counter() - t_start > seconds + 120:
-                break
-        wall = time.perf_counter() - t_start
         st = t.stats()
         t.serve_stop()
-    lat = np.array([r.done_ms - r.submit_ms for r in done])
-    kind_of = {i: k for i, k in enumerate(kinds)}
-    single = np.array([r.done_ms - r.submit_ms for r in done if kind_of[r.job_id] == "gamma1"])
-    events = sum(r.n_events for r in done)
-    steps = sum(r.track_steps for r in done)
-    return dict(jobs=len(done), submitted=submitted, events=events, wall_s=wall, submit_s=t_end_submit,
-                events_per_s=events / wall, track_steps_per_s=steps / wall,
+    inwin = [r for tc, r in done if warm_s <= tc <= t_window]
+    events = sum(r.n_events for r in inwin)
+    steps = sum(r.track_steps for r in inwin)
+    win = t_window - warm_s
+    late = [r for tc, r in done if sub_t[r.job_id] >= warm_s]
+    lat = np.array([r.done_ms - r.submit_ms for r in late]) if late else np.zeros(1)
+    single = np.array([r.done_ms - r.submit_ms for r in late if kinds[r.job_id] == "gamma1"])
+    backlog = sum(1 for j in range(submitted) if sub_t[j] <= t_window) - sum(1 for tc, _ in done if tc <= t_window)
+    return dict(mix=mix, offered_jobs_per_s=rate_jobs_s, offered_gev_per_s=rate_jobs_s * mix_gev_per_job(mix),
+                seconds=seconds, jobs_submitted=submitted, jobs_completed=len(done), backlog_at_window_end=backlog,
+                events_per_s=events / win, track_steps_per_s=steps / win, gev_per_s=events / win * (
+                    sum(p * n * e for _, p, n, _, e in MIXES[mix]) / sum(p * n for _, p, n, _, _ in MIXES[mix])) * 1e-3,
                 p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                 single_event_p50_ms=float(np.percentile(single, 50)) if len(single) else None,
                 single_event_p99_ms=float(np.percentile(single, 99)) if len(single) else None,
-                rate_jobs_s=rate_jobs_s, max_live=int(st.max_live))
+                max_live=int(st.max_live))
+
+
+def serving_sweep(g, abi, capacity_gev_s, mix, loads, seconds):
+    """The serving loop at offered loads given as fractions of capacity_gev_s
+    (GeV/s of primaries); the sustained capacity is the highest completion
+    rate among the loads the loop kept up with (backlog at the window's end
+    below 5 % of the jobs submitted)."""
+    runs = []
+    for f in loads:
+        rate = f * capacity_gev_s / mix_gev_per_job(mix)
+        r = serving(g, abi, seconds, rate, mix=mix)
+        r["load"] = f
+        runs.append(r)
+        log("serving %s at %.2f x: %.0f events/s, backlog %d of %d, p99 %.0f ms"
+            % (mix, f, r["events_per_s"], r["backlog_at_window_end"], r["jobs_submitted"], r["p99_ms"]))
+    kept = [r for r in runs if r["backlog_at_window_end"] <= max(2, 0.05 * r["jobs_submitted"])]
+    best = max(kept, key=lambda r: r["events_per_s"]) if kept else None
+    return dict(runs=runs, sustained=dict(events_per_s=best["events_per_s"], track_steps_per_s=best["track_steps_per_s"],
+                                          load=best["load"], p50_ms=best["p50_ms"], p99_ms=best["p99_ms"],
+                                          single_event_p99_ms=best["single_event_p99_ms"]) if best else None,
+                saturated_events_per_s=max(r["events_per_s"] for r in runs))
 
 
 def c0_block(g, abi, cpu=True):
@@ -390,7 +429,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--events", type=int, default=10000)
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=8.0, help="seconds of submission per offered load")
+    ap.add_argument("--serve-loads", default="0.5,1.0,1.5,2.0,2.5",
+                    help="offered loads of the serving sweep, as fractions of the FP32 batch's GeV/s")
     ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp32", action="store_true")
@@ -492,10 +533,16 @@ def main():
     if not a.skip_nav:
         out["navigation"] = navigation(g, abi, cpu=not a.skip_cpu)
     if not a.skip_serving:
+        # offered loads relative to the FP32 batch's energy throughput on the
+        # same block (GeV/s of primaries)
         gev_per_s = (out.get("fp32", {}).get("value", value)) * 10.0
-        rate = 0.6 * gev_per_s / (0.4 * 1.0 + 0.4 * 100.0 + 0.2 * 5000.0)
-        log("serving for %.0f s at %.2f jobs/s" % (a.serve_seconds, rate))
-        out["serving"] = serving(g, abi, a.serve_seconds, rate)
+        loads = [float(x) for x in a.serve_loads.split(",")]
+        out["serving"] = serving_sweep(g, abi, gev_per_s, "c4", loads, a.serve_seconds)
+        e10 = serving_sweep(g, abi, gev_per_s, "e10", loads, a.serve_seconds)
+        out["serving_e10"] = e10
+        if e10["sustained"]:
+            # the serving form (FP32) on 10 GeV e- jobs against the plain FP64 batch
+            out["serving_e10"]["ratio_to_plain_fp64"] = e10["sustained"]["events_per_s"] / value
     if not a.skip_c0:
         out["c0"] = c0_block(g, abi, cpu=not a.skip_cpu)
     if not a.skip_cpu:

@@ -322,17 +322,27 @@ __device__ __forceinline__ uint64_t child_substream(Rng &r) {
 }
 
 // ------------------------------------------------------------------ tables
-template <class R> struct Nodes;
-template <> struct Nodes<double> {
-  const Node64 *v;
-  const int *hints;
-  __device__ Nodes(const DevParams *P) : v(P->nd), hints(P->hints) {}
+template <class R> struct NodeType;
+template <> struct NodeType<double> {
+  using T = Node64;
+  __device__ static const Node64 *global(const DevParams *P) { return P->nd; }
 };
-template <> struct Nodes<float> {
-  const Node32 *v;
-  const int *hints;
-  __device__ Nodes(const DevParams *P) : v(P->nf), hints(P->hints) {}
+template <> struct NodeType<float> {
+  using T = Node32;
+  __device__ static const Node32 *global(const DevParams *P) { return P->nf; }
 };
+// Where a kernel reads physics-vector nodes: the node array in device memory
+// (DevTable::off), or with kLds its LDS copy (DevTable::loff).
+template <class R, bool kLds = false>
+struct NodesT {
+  using T = typename NodeType<R>::T;
+  const T *v;
+  const int *hints;
+  __device__ NodesT(const DevParams *P) : v(NodeType<R>::global(P)), hints(P->hints) {}
+  __device__ NodesT(const T *lds, const int *h) : v(lds), hints(h) {}
+  __device__ __forceinline__ const T &at(const DevTable &t, int i) const { return v[(kLds ? t.loff : t.off) + i]; }
+};
+template <class R> using Nodes = NodesT<R, false>;
 
 template <class R> __device__ __forceinline__ R tdl(const DevTable &t);
 template <> __device__ __forceinline__ double tdl<double>(const DevTable &t) { return t.dl; }
@@ -368,9 +378,9 @@ __device__ __forceinline__ R spline(const V &lo, const V &hi, R e) {
 }
 
 // ProxyPhysicsVector::Value on a log-binned vector (ProxyPhysicsVector.cu:388-394,469-483).
-template <class R>
-__device__ __forceinline__ R pvec_value(const Nodes<R> &N, const DevTable &t, R e) {
-  const int o = t.off, n = t.n;
+template <class R, class NS>
+__device__ __forceinline__ R pvec_value(const NS &N, const DevTable &t, R e) {
+  const int n = t.n;
   R x0, y0, xn, yn;
   tends<R>(t, x0, y0, xn, yn);
   if (e <= x0) return y0;
@@ -378,20 +388,20 @@ __device__ __forceinline__ R pvec_value(const Nodes<R> &N, const DevTable &t, R 
   int bin = (int)(m_log(e) / tdl<R>(t) - tbase<R>(t));
   if (bin > n - 2) bin = n - 2;
   if (bin < 0) bin = 0;
-  auto lo = N.v[o + bin];
-  auto hi = N.v[o + bin + 1];
+  auto lo = N.at(t, bin);
+  auto hi = N.at(t, bin + 1);
   if (bin > 0 && e < lo.x) {
     hi = lo;
-    lo = N.v[o + bin - 1];
+    lo = N.at(t, bin - 1);
   }
   return spline<R>(lo, hi, e);
 }
 
 // InvRangeValue (ProxyPhysicsVector.cu:491-524): binary search (the reference calls the
 // log-bin Value() on this non-log vector).
-template <class R>
-__device__ __forceinline__ R pvec_inv_value(const Nodes<R> &N, const DevTable &t, R r) {
-  const int o = t.off, n = t.n;
+template <class R, class NS>
+__device__ __forceinline__ R pvec_inv_value(const NS &N, const DevTable &t, R r) {
+  const int n = t.n;
   R x0, y0, xn, yn;
   tends<R>(t, x0, y0, xn, yn);
   if (r <= x0) return y0;
@@ -403,9 +413,9 @@ __device__ __forceinline__ R pvec_inv_value(const Nodes<R> &N, const DevTable &t
   int k = (int)floorf((__log2f((float)r) - t.hlog0) * t.hscale) - 1;
   k = k < 0 ? 0 : (k >= t.nhint ? t.nhint - 1 : k);
   int j = N.hints[t.hint + k];
-  while (j > 0 && !(N.v[o + j].x < r)) --j;
-  while (j < n - 2 && N.v[o + j + 1].x < r) ++j;
-  return spline<R>(N.v[o + j], N.v[o + j + 1], r);
+  while (j > 0 && !(N.at(t, j).x < r)) --j;
+  while (j < n - 2 && N.at(t, j + 1).x < r) ++j;
+  return spline<R>(N.at(t, j), N.at(t, j + 1), r);
 }
 
 // Seltzer-Berger grid: x[32], y[57], v[57][32] at offset sb.
@@ -497,8 +507,8 @@ __device__ __forceinline__ R heitler_xs(R ekin) {
 // Macroscopic cross-section (1/mm) of discrete process p of a species
 // (ProxyEmProcess::GetLambda and the processes' GetLambda) with the track's
 // couple, Compton above 1 MeV and photoelectric from LambdaPrim/E.
-template <class R>
-__device__ __forceinline__ R macro_xs(const DevParams *P, const Nodes<R> &N, const DevMaterial &m, int species, int p, R e) {
+template <class R, class NS>
+__device__ __forceinline__ R macro_xs(const DevParams *P, const NS &N, const DevMaterial &m, int species, int p, R e) {
   if (species == 0) {
     if (p == 0) {
       if ((double)e < m.compt_split) return pvec_value(N, m.tab[GXT_TAB_LAMBDA_COMPT], e);
@@ -814,12 +824,22 @@ __device__ __forceinline__ void sample_brems(const DevParams *P, const DevMateri
     k = m_sqrt(x);
     v = sb_value(sb, k / t0, y);
   } while (v < vmax * rng.template u<R>());
-  const R c = R(4.0) - R(8.0) * rng.template u<R>();
+  const R u0 = rng.template u<R>();
+  const R c = R(4.0) - R(8.0) * u0;
   const R signc = c < R(0) ? R(-1) : R(1);
   const R a = m_fabs(c);
   const R delta = R(0.5) * (m_sqrt(R(4.0) + a * a) + a);
-  const R cofA = -signc * m_pow(delta, R(1.0) / R(3.0));
-  const R cost0 = cofA - R(1) / cofA;
+  // cos = cofA - 1/cofA with cofA = -sign(c) delta^(1/3) is -sign(c) x, where
+  // x = d - 1/d (d = delta^(1/3)) solves x^3 + 3x = a; so 1 - x =
+  // (4 - a) / (4 + x + x^2), and 4 - a = 8 min(u0, 1 - u0): neither 1 - cos
+  // nor 1 + cos cancels
+  const R d = m_pow(delta, R(1.0) / R(3.0));
+  const R x = d - R(1) / d;
+  const R omx = R(8) * rmin(u0, R(1) - u0) / (R(4) + x + x * x);
+  const R opx = R(1) + x;
+  const R cost0 = -signc * x;
+  const R omc0 = signc > R(0) ? opx : omx;
+  const R opc0 = signc > R(0) ? omx : opx;
   // the boost to the lab, (cos + beta) / (1 + cos beta), in forms without
   // cancellation near the forward direction: 1 - beta =
   // 1 / (gam (gam + sqrt(tau (tau + 2)))) and 1 - cos' = (1 - cos)(1 - beta) /
@@ -828,7 +848,7 @@ __device__ __forceinline__ void sample_brems(const DevParams *P, const DevMateri
   const R gam = tau + R(1);
   const R bgam = m_sqrt(tau * (tau + R(2)));
   const R omb = R(1) / (gam * (gam + bgam));
-  const R omc = (R(1) - cost0) * omb / ((R(1) + cost0) - cost0 * omb);
+  const R omc = omc0 * omb / (opc0 - cost0 * omb);
   const R cost = R(1) - omc;
   const R sint = m_sqrt(rmax(omc * (R(2) - omc), R(0)));
   const R phi = R(kTwoPi) * rng.template u<R>();
@@ -1016,8 +1036,8 @@ __device__ __forceinline__ int queue_model(int q) {
 // ------------------------------------------------------------------ continuous loss
 // ProxyIonization::AlongStepGPIL (ProxyIonization.hpp:108-128): production-cut
 // length for finR and Geant4's (2 - finR/range) form.
-template <class R>
-__device__ __forceinline__ R along_limit(const Nodes<R> &N, const DevMaterial &m, bool pos, R e, R mine) {
+template <class R, class NS>
+__device__ __forceinline__ R along_limit(const NS &N, const DevMaterial &m, bool pos, R e, R mine) {
   R range = pvec_value(N, m.tab[pos ? GXT_TAB_RANGE_EP : GXT_TAB_RANGE_EM], e);
   if (e < mine) range *= e / mine;
   const R finR = rmin(R(10.0), (R)m.cut_length);
@@ -1026,8 +1046,8 @@ __device__ __forceinline__ R along_limit(const Nodes<R> &N, const DevMaterial &m
 
 // ProxyIonization::AlongStepDoIt (ProxyIonization.hpp:130-168): binary-search
 // inverse range.
-template <class R>
-__device__ __forceinline__ R along_doit(const Nodes<R> &N, const DevMaterial &m, bool pos, R e, R step, R mine,
+template <class R, class NS>
+__device__ __forceinline__ R along_doit(const NS &N, const DevMaterial &m, bool pos, R e, R step, R mine,
                                         R &eloss) {
   const R range = pvec_value(N, m.tab[pos ? GXT_TAB_RANGE_EP : GXT_TAB_RANGE_EM], e);
   if (step >= range || e <= mine) {
@@ -1383,8 +1403,8 @@ __device__ __forceinline__ int propagate(const DevParams *P, int species, R e, d
 constexpr double kMscFacRange = 0.04, kMscFacSafety = 0.6, kMscTmin = 1.0e-5;
 constexpr double kMscNone = 1.0e30;  // lambda1 where a material has no msc table
 
-template <class R>
-__device__ __forceinline__ double msc_lambda1(const Nodes<R> &N, const DevMaterial &m, bool pos, R e) {
+template <class R, class NS>
+__device__ __forceinline__ double msc_lambda1(const NS &N, const DevMaterial &m, bool pos, R e) {
   const DevTable &t = m.tab[pos ? GXT_TAB_MSC_EP : GXT_TAB_MSC_EM];
   if (t.n == 0) return kMscNone;
   const R v = pvec_value(N, t, e);

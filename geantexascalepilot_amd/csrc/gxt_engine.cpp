@@ -47,6 +47,10 @@ int gxt::set_error(int code, const std::string &msg) { return set_err(code, msg)
 
 namespace {
 
+// Largest table copy the transports stage in LDS (beside 11 KB of wave
+// histograms per workgroup).
+constexpr int kLdsTablesMax = 40 * 1024;
+
 #define HIP_TRY(expr)                                                                        \
   do {                                                                                       \
     hipError_t e_ = (expr);                                                                  \
@@ -334,6 +338,13 @@ struct gxt_ctx {
     a.trace_cap = trace_cap;
     a.transport_mode = transport_mode;
     a.inner_steps = inner_steps;
+    // the mode-0 transports stage their tables in LDS when they take at most
+    // kLdsTablesMax bytes
+    const int nb = precision == GXT_FP32 ? (int)sizeof(Node32) : (int)sizeof(Node64);
+    for (int l = 0; l < 2; ++l) {
+      const int bytes = hd.params.lds_nodes[l] * nb;
+      a.lds_bytes[l] = (bytes > 0 && bytes <= kLdsTablesMax && (l == 0 || transport_mode == 0)) ? bytes : 0;
+    }
     return a;
   }
 
@@ -964,6 +975,28 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
   P.trace_events = cfg->trace_tracks > 0 ? cfg->trace_tracks : 0;
   P.trace_steps = cfg->trace_steps;
   for (int i = 0; i < 6; ++i) P.seed[i] = cfg->seed;
+  // LDS layouts of the transports' tables: each slot's photon (e-/e+) tables
+  // are one contiguous node span (tables are stored in GXT_TAB order)
+  {
+    const int first[2] = {GXT_TAB_LAMBDA_COMPT, GXT_TAB_LAMBDA_BREM_EM};
+    const int last[2] = {GXT_TAB_LAMBDAPRIM_PHOT, GXT_TAB_INVRANGE_EP};
+    for (size_t s = 0; s < mats.size(); ++s)
+      for (int t = 0; t < GXT_NTAB; ++t) P.mat[s].tab[t].loff = -1;
+    for (int l = 0; l < 2; ++l) {
+      int base = 0;
+      for (size_t s = 0; s < mats.size(); ++s) {
+        DevMaterial &m = P.mat[s];
+        for (int t = first[l]; t < last[l]; ++t)
+          if (m.tab[t + 1].off != m.tab[t].off + m.tab[t].n) return set_err(GXT_EDATA, "tables not contiguous");
+        const int src = m.tab[first[l]].off;
+        P.lds_src[l][s] = src;
+        P.lds_cnt[l][s] = m.tab[last[l]].off + m.tab[last[l]].n - src;
+        for (int t = first[l]; t <= last[l]; ++t) m.tab[t].loff = base + (m.tab[t].off - src);
+        base += P.lds_cnt[l][s];
+      }
+      P.lds_nodes[l] = base;
+    }
+  }
   rc = upload(c->hd, c->dd);
   if (rc) return rc;
   // pools, queues, counters, tallies

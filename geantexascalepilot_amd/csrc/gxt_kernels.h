@@ -34,6 +34,8 @@ struct StepArgs {
                        // local helix, 3 RK45 stepper
   int inner_steps;     // steps a track may take in one transport launch while it neither
                        // interacts nor ends (>= 1)
+  int lds_bytes[2];    // > 0: the photon (0) and e-/e+ (1) transports stage their tables
+                       // (DevParams::lds_*) in that much dynamic LDS; 0: they read them in place
 };
 
 template <class R>

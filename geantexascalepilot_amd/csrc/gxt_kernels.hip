@@ -403,8 +403,8 @@ __device__ __forceinline__ void emit_at_rest(const StepArgs &a, const Pool<R> &n
 
 // PostStep GPIL of process p (ProxyEmProcess.hpp:122-151): resample the
 // number of interaction lengths when exhausted, propose nil / lambda.
-template <int p, class R>
-__device__ __forceinline__ void gpil(const DevParams *P, const Nodes<R> &N, const DevMaterial &m, int species, R e,
+template <int p, class R, class NS>
+__device__ __forceinline__ void gpil(const DevParams *P, const NS &N, const DevMaterial &m, int species, R e,
                                      R &nil, R &xs, Rng &rng, double &bestd, int &sel) {
   xs = macro_xs(P, N, m, species, p, e);
   if (nil <= R(0)) nil = -m_log(rng.template u<R>());
@@ -504,16 +504,53 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
 // 3 waves per SIMD with a 28 B/lane spill; that measured 1.6 % faster on the
 // 10 GeV e- / PbWO4 batch than 2 waves without spill.  The other modes spill
 // heavily at 3 and keep 2.
-template <class R, bool kCharged, int kMode = 0>
+// With kLds the workgroup first copies its species' physics vectors (one node
+// span per material slot, DevParams::lds_*) into dynamic LDS, where every
+// lookup of the launch is served.
+extern __shared__ uint4 tab_lds[];
+
+// The workgroup's copy of DevParams in LDS (8-byte words; every step
+// kernel's threads take part, then wait at a barrier).
+__device__ __forceinline__ const DevParams *stage_params(const DevParams *P, DevParams &sP) {
+  static_assert(sizeof(DevParams) % 8 == 0, "DevParams is copied in 8-byte words");
+  const unsigned long long *src = reinterpret_cast<const unsigned long long *>(P);
+  unsigned long long *dst = reinterpret_cast<unsigned long long *>(&sP);
+  for (int k = threadIdx.x; k < (int)(sizeof(DevParams) / 8); k += blockDim.x) dst[k] = src[k];
+  __syncthreads();
+  return &sP;
+}
+
+template <class R, bool kLds>
+__device__ __forceinline__ NodesT<R, kLds> stage_tables(const DevParams *P, int layout) {
+  using T = typename NodeType<R>::T;
+  if constexpr (kLds) {
+    constexpr int w = sizeof(T) / sizeof(uint4);
+    const uint4 *src = reinterpret_cast<const uint4 *>(NodeType<R>::global(P));
+    int dst = 0;
+    for (int s = 0; s < kMaxSlots; ++s) {
+      const int n16 = P->lds_cnt[layout][s] * w;
+      const uint4 *sp = src + (int64_t)P->lds_src[layout][s] * w;
+      for (int k = threadIdx.x; k < n16; k += blockDim.x) tab_lds[dst + k] = sp[k];
+      dst += n16;
+    }
+    __syncthreads();
+    return NodesT<R, true>(reinterpret_cast<const T *>(tab_lds), P->hints);
+  } else {
+    return NodesT<R, false>(P);
+  }
+}
+
+template <class R, bool kCharged, int kMode = 0, bool kLds = false>
 __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAVES) void k_transport(StepArgs a,
                                                                                                PoolArgs<R> pa) {
   __shared__ WaveHist whs[kBlock / 64];
+  __shared__ DevParams sP;
   WaveHist &wh = whs[threadIdx.x >> 6];
   wh_clear(wh);
-  const DevParams *P = a.P;
+  const NodesT<R, kLds> N = stage_tables<R, kLds>(a.P, kCharged ? 1 : 0);
+  const DevParams *P = stage_params(a.P, sP);
   const Pool<R> &cur = pa.cur;
   const Pool<R> &nxt = pa.nxt;
-  const Nodes<R> N(P);
   const bool trace = P->trace_events > 0;
   // counts are clamped to the stripes so a corrupted counter cannot index past them
   const int64_t sc = stripe_cap(cur);
@@ -736,7 +773,8 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 8 ? 3 : GXT_STEP_WAVES) void k
   __shared__ WaveHist whs[kBlock / 64];
   WaveHist &wh = whs[threadIdx.x >> 6];
   wh_clear(wh);
-  const DevParams *P = a.P;
+  __shared__ DevParams sP;
+  const DevParams *P = stage_params(a.P, sP);
   const Pool<R> &cur = pa.cur;
   const Pool<R> &nxt = pa.nxt;
   const bool trace = P->trace_events > 0;
@@ -1182,7 +1220,10 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
   if (e == hipSuccess) e = hipStreamWaitEvent(s.side, s.fork, 0);
   if (e != hipSuccess) return e;
   if (ev && ev[0]) (void)hipEventRecord(ev[0], s.main);
-  hipLaunchKernelGGL((k_transport<R, false>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
+  if (a.lds_bytes[0] > 0)
+    hipLaunchKernelGGL((k_transport<R, false, 0, true>), dim3(grid), dim3(kBlock), a.lds_bytes[0], s.main, a, pa);
+  else
+    hipLaunchKernelGGL((k_transport<R, false>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
   if (ev && ev[1]) (void)hipEventRecord(ev[1], s.main);
   hipLaunchKernelGGL((k_interact<R, false>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
   if (ev && ev[2]) (void)hipEventRecord(ev[2], s.main);
@@ -1195,6 +1236,8 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
     hipLaunchKernelGGL((k_transport<R, true, 2>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
   else if (a.transport_mode == 1)
     hipLaunchKernelGGL((k_transport<R, true, 1>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
+  else if (a.lds_bytes[1] > 0)
+    hipLaunchKernelGGL((k_transport<R, true, 0, true>), dim3(grid), dim3(kBlock), a.lds_bytes[1], s.side, a, pa);
   else
     hipLaunchKernelGGL((k_transport<R, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
   if (ev && ev[5]) (void)hipEventRecord(ev[5], s.side);

@@ -39,6 +39,8 @@ struct alignas(16) Node32 {
 struct DevTable {
   int off;
   int n;
+  int loff;  // first node in the LDS copy of its kernel's tables (DevParams::lds_*), -1: none
+  int lpad;
   double dl, base;  // log-binning: bin = log(E)/dl - base
   double x0, xn, y0, yn;  // first and last node (the clamps of a lookup)
   float dlf, basef;
@@ -177,6 +179,14 @@ struct DevParams {
   const double *sx;             // merged Sandia breakpoints and coefficients (DevMaterial::sx_*)
   const double *shells;         // binding energies (MeV)
   const int *hints;             // search hints of the inverse-range vectors
+  // LDS copies of the physics vectors a transport kernel reads: layout 0 the
+  // photon tables (GXT_TAB_LAMBDA_COMPT .. GXT_TAB_LAMBDAPRIM_PHOT), layout 1
+  // the e-/e+ tables (GXT_TAB_LAMBDA_BREM_EM .. GXT_TAB_INVRANGE_EP), one
+  // contiguous node span per material slot: nodes [lds_src[l][s], +
+  // lds_cnt[l][s]) of the node array, in slot order
+  int lds_src[2][kMaxSlots];
+  int lds_cnt[2][kMaxSlots];
+  int lds_nodes[2];
 };
 
 // Structure-of-arrays track pool.  R is the precision of the physics state

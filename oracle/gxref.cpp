@@ -1338,12 +1338,21 @@ static void sample_brems(const Material &m, double t0, const double d0[3], uint3
     v = sb_value(sb, k / t0, y);
   } while (v < vmax * mrg_uniform(s));
   // DipBust direction
-  const double c = 4.0 - 8.0 * mrg_uniform(s);
+  const double u0 = mrg_uniform(s);
+  const double c = 4.0 - 8.0 * u0;
   const double signc = c < 0.0 ? -1.0 : 1.0;
   const double a = std::fabs(c);
   const double delta = 0.5 * (std::sqrt(4.0 + a * a) + a);
-  const double cofA = -signc * pow_cw(delta, 1.0 / 3.0);
-  const double cost0 = cofA - 1.0 / cofA;
+  // cos = cofA - 1/cofA with cofA = -sign(c) delta^(1/3) (ProxySeltzerBerger.hpp:295-310)
+  // is -sign(c) x, where x = d - 1/d (d = delta^(1/3)) solves x^3 + 3x = a:
+  // 1 - x = (4 - a) / (4 + x + x^2) and 4 - a = 8 min(u0, 1 - u0)
+  const double d = pow_cw(delta, 1.0 / 3.0);
+  const double x = d - 1.0 / d;
+  const double omx = 8.0 * std::min(u0, 1.0 - u0) / (4.0 + x + x * x);
+  const double opx = 1.0 + x;
+  const double cost0 = -signc * x;
+  const double omc0 = signc > 0.0 ? opx : omx;
+  const double opc0 = signc > 0.0 ? omx : opx;
   // the boost to the lab (ProxySeltzerBerger.hpp:295-310 writes (cos + beta) /
   // (1 + cos beta)) in forms without cancellation near the forward
   // direction: 1 - beta = 1 / (gam (gam + sqrt(tau (tau + 2)))) and
@@ -1352,7 +1361,7 @@ static void sample_brems(const Material &m, double t0, const double d0[3], uint3
   const double gam = tau + 1.0;
   const double bgam = std::sqrt(tau * (tau + 2.0));
   const double omb = 1.0 / (gam * (gam + bgam));
-  const double omc = (1.0 - cost0) * omb / ((1.0 + cost0) - cost0 * omb);
+  const double omc = omc0 * omb / (opc0 - cost0 * omb);
   const double cost = 1.0 - omc;
   const double sint = std::sqrt(std::max(omc * (2.0 - omc), 0.0));
   const double phi = kTwoPi * mrg_uniform(s);
