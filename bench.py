@@ -261,7 +261,7 @@ def mix_gev_per_job(mix):
 def serving(g, abi, seconds, rate_jobs_s, mix="c4", warm_s=2.0, drain_s=60.0, seed=4242):
     """Poisson stream of jobs into the PbWO4 block, FP32 form, one shared pool,
     submitted for `seconds`; mix "c4": 40 % one 1 GeV gamma, 40 % 10 x 10 GeV
-    e-, 20 % 100 x 50 GeV e-; mix "e10": every job 10 x 10 GeV e-.  The
+    e-, 20 % 100 x 50 GeV e-; mix "e10": every job 100 x 10 GeV e-.  The
     throughput is the completion rate inside the submission window after
     warm_s (so it saturates at the capacity when the offered rate exceeds
     it); latencies are those of the jobs submitted after warm_s, with a
@@ -309,6 +309,12 @@ def serving(g, abi, seconds, rate_jobs_s, mix="c4", warm_s=2.0, drain_s=60.0, se
     lat = np.array([r.done_ms - r.submit_ms for r in late]) if late else np.zeros(1)
     single = np.array([r.done_ms - r.submit_ms for r in late if kinds[r.job_id] == "gamma1"])
     backlog = sum(1 for j in range(submitted) if sub_t[j] <= t_window) - sum(1 for tc, _ in done if tc <= t_window)
+    # a job's latency over the step-loop iterations it spent in flight: the
+    # host finds a finished job at the next counter snapshot, at most one
+    # round (4-8 iterations) after its last track died
+    its = np.array([max(r.iterations, 1) for r in late]) if late else np.ones(1)
+    ms_per_it = float(np.median(lat / its))
+    single_its = np.array([r.iterations for r in late if kinds[r.job_id] == "gamma1"])
     return dict(mix=mix, offered_jobs_per_s=rate_jobs_s, offered_gev_per_s=rate_jobs_s * mix_gev_per_job(mix),
                 seconds=seconds, jobs_submitted=submitted, jobs_completed=len(done), backlog_at_window_end=backlog,
                 events_per_s=events / win, track_steps_per_s=steps / win, gev_per_s=events / win * (
@@ -316,7 +322,8 @@ def serving(g, abi, seconds, rate_jobs_s, mix="c4", warm_s=2.0, drain_s=60.0, se
                 p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                 single_event_p50_ms=float(np.percentile(single, 50)) if len(single) else None,
                 single_event_p99_ms=float(np.percentile(single, 99)) if len(single) else None,
-                max_live=int(st.max_live))
+                max_live=int(st.max_live), ms_per_iteration=ms_per_it,
+                single_event_iterations_p50=float(np.percentile(single_its, 50)) if len(single_its) else None)
 
 
 def serving_sweep(g, abi, capacity_gev_s, mix, loads, seconds):

@@ -83,7 +83,8 @@ class KernelStats(C.Structure):
 class JobResult(C.Structure):
     _fields_ = [("job_id", C.c_uint64), ("n_events", C.c_int32), ("pad", C.c_int32), ("edep", C.c_double),
                 ("visible", C.c_double), ("submit_ms", C.c_double), ("admit_ms", C.c_double),
-                ("done_ms", C.c_double), ("track_steps", C.c_int64)]
+                ("done_ms", C.c_double), ("track_steps", C.c_int64),
+                ("iterations", C.c_int64)]
 
 
 class SampleIO(C.Structure):

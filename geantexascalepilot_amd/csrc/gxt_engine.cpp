@@ -252,6 +252,7 @@ struct Job {
   int ev_base = -1;                 // first event slot of the job
   bool started = false;
   double submit_ms = 0, admit_ms = 0, done_ms = 0;
+  int64_t admit_iter = 0, done_iter = 0;  // step-loop iterations launched at admission / at the finding snapshot
   std::vector<gxt_event_tally> tallies;
   int64_t steps = 0;
 };
@@ -289,6 +290,7 @@ struct gxt_ctx {
   int64_t trace_cap = 0;
   hipEvent_t snap = nullptr;      // recorded after a round's counter snapshot
   size_t ev_snap = 0;             // profiling entries recorded before that snapshot
+  int64_t snap_iter = 0;          // step-loop iterations launched before that snapshot
   // non-uniform field
   gxt_field_map fmap;
   std::string fmap_path;
@@ -429,6 +431,7 @@ int admit(gxt_ctx *c) {
         j->ev_base = eb;
         j->started = true;
         j->admit_ms = now_ms(c->t0);
+        j->admit_iter = c->iterations;
         zero_ranges.emplace_back(eb, j->n_events);
         std::fill(c->ev_counted.begin() + eb, c->ev_counted.begin() + eb + j->n_events, 0);
       }
@@ -595,6 +598,7 @@ int collect(gxt_ctx *c, bool &idle) {
       j->steps += t.n_steps;
     }
     j->done_ms = now_ms(c->t0);
+    j->done_iter = c->snap_iter;
     std::lock_guard<std::mutex> lk(c->mu);
     for (int k = 0; k < j->n_events; ++k) c->ev_used[j->ev_base + k] = 0;
     for (auto it = c->jobs.begin(); it != c->jobs.end(); ++it)
@@ -654,6 +658,7 @@ int round_trip(gxt_ctx *c, int k, bool &idle) {
                          c->stream));
   HIP_TRY(hipEventRecord(c->snap, c->stream));
   c->ev_snap = c->ev_pending.size();
+  c->snap_iter = c->iterations;
   if (any) {
     rc = iterate(c, kLookahead);
     if (rc) return rc;
@@ -1146,6 +1151,7 @@ int gxt_poll(gxt_ctx *c, gxt_job_result *out, int max, int *n_out) {
     r.submit_ms = j->submit_ms;
     r.admit_ms = j->admit_ms;
     r.done_ms = j->done_ms;
+    r.iterations = j->done_iter - j->admit_iter;
     r.track_steps = j->steps;
   }
   *n_out = n;

@@ -159,6 +159,8 @@ typedef struct gxt_job_result {
   double admit_ms;
   double done_ms;
   int64_t track_steps;
+  int64_t iterations;     /* step-loop iterations from the job's admission to the
+                             counter snapshot that found its last event finished */
 } gxt_job_result;
 
 typedef struct gxt_ctx gxt_ctx;
