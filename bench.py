@@ -274,6 +274,7 @@ def serving(g, abi, seconds, rate_jobs_s, mix="c4", warm_s=2.0, drain_s=60.0, se
         t.serve_start()
         t_start = time.perf_counter()
         next_sub, first, submitted = 0.0, 0, 0
+        next_log = 60.0
         while True:
             now = time.perf_counter() - t_start
             if now >= seconds:
@@ -293,6 +294,9 @@ def serving(g, abi, seconds, rate_jobs_s, mix="c4", warm_s=2.0, drain_s=60.0, se
                 next_sub = now + rs.exponential(1.0 / rate_jobs_s)
             for r in t.poll():
                 done.append((time.perf_counter() - t_start, r))
+            if now >= next_log:
+                log("serving %s: %.0f s, %d jobs submitted, %d completed" % (mix, now, submitted, len(done)))
+                next_log += 60.0
             time.sleep(0.0002)
         t_window = time.perf_counter() - t_start
         while len(done) < submitted and time.perf_counter() - t_start < seconds + drain_s:
@@ -439,6 +443,11 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="seconds of submission per offered load")
     ap.add_argument("--serve-loads", default="0.5,1.0,1.5,2.0,2.5",
                     help="offered loads of the serving sweep, as fractions of the FP32 batch's GeV/s")
+    ap.add_argument("--serve-mix", default="c4,e10", help="job mixes of the serving sweep (c4, e10)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving sweep (offered loads relative to --serve-capacity-gev-s)")
+    ap.add_argument("--serve-capacity-gev-s", type=float, default=0.0,
+                    help="with --serve-only: the GeV/s the loads are fractions of (0: measure the FP32 batch)")
     ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp32", action="store_true")
@@ -479,6 +488,17 @@ def main():
         print(json.dumps(dict(field_forms=field_forms(g, abi, cpu=not a.skip_cpu))), flush=True)
         return
     n = a.events
+    if a.serve_only:
+        cap = a.serve_capacity_gev_s
+        if cap <= 0:
+            r32 = run_batch(g, abi, abi.FP32, n, a.steps, a.warmup, pool=a.pool)
+            cap = a.steps * n / r32["seconds"] * 10.0
+        loads = [float(x) for x in a.serve_loads.split(",")]
+        out = dict(capacity_gev_per_s=cap)
+        for mix in a.serve_mix.split(","):
+            out["serving_" + mix] = serving_sweep(g, abi, cap, mix, loads, a.serve_seconds)
+        print(json.dumps(out), flush=True)
+        return
     r64 = run_batch(g, abi, abi.FP64, n, a.steps, a.warmup, pool=a.pool)
     value = a.steps * n / r64["seconds"]
     ks = r64["ks"]
@@ -544,10 +564,13 @@ def main():
         # same block (GeV/s of primaries)
         gev_per_s = (out.get("fp32", {}).get("value", value)) * 10.0
         loads = [float(x) for x in a.serve_loads.split(",")]
-        out["serving"] = serving_sweep(g, abi, gev_per_s, "c4", loads, a.serve_seconds)
-        e10 = serving_sweep(g, abi, gev_per_s, "e10", loads, a.serve_seconds)
-        out["serving_e10"] = e10
-        if e10["sustained"]:
+        mixes = a.serve_mix.split(",")
+        if "c4" in mixes:
+            out["serving"] = serving_sweep(g, abi, gev_per_s, "c4", loads, a.serve_seconds)
+        e10 = serving_sweep(g, abi, gev_per_s, "e10", loads, a.serve_seconds) if "e10" in mixes else None
+        if e10:
+            out["serving_e10"] = e10
+        if e10 and e10["sustained"]:
             # the serving form (FP32) on 10 GeV e- jobs against the plain FP64 batch
             out["serving_e10"]["ratio_to_plain_fp64"] = e10["sustained"]["events_per_s"] / value
     if not a.skip_c0:
