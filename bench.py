@@ -170,6 +170,44 @@ def field_forms(g, abi, n_events=20000, seed=2468, cpu=False):
     return out
 
 
+def tree_form(g, abi, n_events=10000, cpu=False):
+    """Transport through the volume tree of the reference's calorimeter (50
+    layers of 10 mm Pb + 10 mm lAr along x, 1 m square, in a vacuum world;
+    ProxyDetectorConstruction.cpp:38-60, 225-325) with the gun of its
+    transport test (10 GeV e-, position in [0, 1) mm per coordinate,
+    direction in the positive octant; test_BasicCpuTransport.cpp:556-586),
+    no field and 4 T along z, FP64 and FP32."""
+    prims = abi.reference_calorimeter_primaries(n_events)
+    out = dict(workload="10 GeV e- x %d into the reference calorimeter as a volume tree (152 touchables)" % n_events)
+    for name, kw in (("no_field_fp64", {}), ("no_field_fp32", dict(precision=abi.FP32)),
+                     ("4T_fp64", dict(field=(0.0, 0.0, 4.0))), ("4T_fp32", dict(field=(0.0, 0.0, 4.0), precision=abi.FP32))):
+        cfg = abi.config_reference_calorimeter(max_events=n_events, **kw)
+        with g.Transport(cfg) as t:
+            t0 = time.perf_counter()
+            t.submit(prims, 1)
+            t.run()
+            dt = time.perf_counter() - t0
+            ev = t.event_tallies(1, n_events)
+            st = t.stats()
+        out[name] = dict(events_per_s=n_events / dt, seconds=dt, track_steps_per_s=int(st.track_steps) / dt,
+                         visible_mean_mev=float(ev["visible"].mean()), edep_mean_mev=float(ev["edep"].mean()))
+        log("tree %s: %.2f s, %d track-steps" % (name, dt, st.track_steps))
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_harness as O
+        threads = max(1, min(16, os.cpu_count() or 1))
+        n = 32 * threads
+        o = O.Oracle(abi.config_reference_calorimeter(max_events=n))
+        t0 = time.perf_counter()
+        o.run(abi.reference_calorimeter_primaries(n), threads)
+        dt = time.perf_counter() - t0
+        steps = int(o.stats().track_steps)
+        o.close()
+        out["cpu_oracle_no_field"] = dict(events_per_s=n / dt, cores=threads, track_steps_per_s=steps / dt,
+                                          sample="first %d primaries, %d track-steps, %.1f s" % (n, steps, dt))
+    return out
+
+
 def navigation(g, abi, n=1 << 24, reps=20, cpu=False, seed=99):
     """Volume-tree navigation in the reference's calorimeter (50 layers of
     10 mm Pb + 10 mm lAr, 1 m square, 152 touchables): n points uniform in the
@@ -455,6 +493,8 @@ def main():
     ap.add_argument("--skip-field", action="store_true", help="skip the field-map and RK45 calorimeter runs")
     ap.add_argument("--skip-c0", action="store_true", help="skip the 1000 x 1 GeV gamma lead-block run")
     ap.add_argument("--skip-nav", action="store_true", help="skip the volume-tree navigation queries")
+    ap.add_argument("--skip-tree", action="store_true", help="skip the transport through the volume tree")
+    ap.add_argument("--tree-only", action="store_true", help="run only the transport through the volume tree")
     ap.add_argument("--nav-only", action="store_true", help="run only the volume-tree navigation queries")
     ap.add_argument("--field-only", action="store_true",
                     help="run only the field-map, RK45 and multiple-scattering calorimeter runs and print them")
@@ -483,6 +523,9 @@ def main():
         del src, dst
     if a.nav_only:
         print(json.dumps(dict(navigation=navigation(g, abi, cpu=not a.skip_cpu))), flush=True)
+        return
+    if a.tree_only:
+        print(json.dumps(dict(tree_form=tree_form(g, abi, cpu=not a.skip_cpu))), flush=True)
         return
     if a.field_only:
         print(json.dumps(dict(field_forms=field_forms(g, abi, cpu=not a.skip_cpu))), flush=True)
@@ -559,6 +602,8 @@ def main():
         out["field_forms"] = field_forms(g, abi, cpu=not a.skip_cpu)
     if not a.skip_nav:
         out["navigation"] = navigation(g, abi, cpu=not a.skip_cpu)
+    if not a.skip_tree:
+        out["tree_form"] = tree_form(g, abi, cpu=not a.skip_cpu)
     if not a.skip_serving:
         # offered loads relative to the FP32 batch's energy throughput on the
         # same block (GeV/s of primaries)

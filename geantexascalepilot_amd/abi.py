@@ -12,7 +12,7 @@ import numpy as np
 
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
-GEOM_BOX, GEOM_CALO = 0, 1
+GEOM_BOX, GEOM_CALO, GEOM_TREE = 0, 1, 2
 MAT_PB, MAT_LAR, MAT_PBWO4 = 0, 1, 2
 FP64, FP32 = 0, 1
 STEPPER_HELIX, STEPPER_RK45 = 0, 1
@@ -32,6 +32,14 @@ ERRORS = {-1: "GXT_EINVAL", -2: "GXT_EHIP", -3: "GXT_EDATA", -4: "GXT_ECAPACITY"
           -6: "GXT_ENOTFOUND"}
 
 
+class BoxVolume(C.Structure):
+    _fields_ = [("half", C.c_double * 3), ("material", C.c_int32), ("pad", C.c_int32)]
+
+
+class Placement(C.Structure):
+    _fields_ = [("mother", C.c_int32), ("daughter", C.c_int32), ("translation", C.c_double * 3)]
+
+
 class Config(C.Structure):
     _fields_ = [
         ("geometry", C.c_int32), ("material", C.c_int32), ("box_xy", C.c_double), ("box_depth", C.c_double),
@@ -43,7 +51,9 @@ class Config(C.Structure):
         ("max_events", C.c_int32), ("max_jobs", C.c_int32), ("trace_tracks", C.c_int32),
         ("trace_steps", C.c_int32), ("profile_kernels", C.c_int32), ("data_dir", C.c_char_p),
         ("field_map", C.c_char_p), ("field_origin", C.c_double * 3), ("field_stepper", C.c_int32),
-        ("msc", C.c_int32),
+        ("msc", C.c_int32), ("tree_volumes", C.POINTER(BoxVolume)), ("tree_placements", C.POINTER(Placement)),
+        ("n_tree_volumes", C.c_int32), ("n_tree_placements", C.c_int32), ("tree_world", C.c_int32),
+        ("pad1", C.c_int32),
     ]
 
 
@@ -93,14 +103,6 @@ class SampleIO(C.Structure):
                 ("out_dir", C.POINTER(C.c_double)), ("out_edep", C.POINTER(C.c_double)),
                 ("nsec", C.POINTER(C.c_int32)), ("sec_species", C.POINTER(C.c_int32)),
                 ("sec_ekin", C.POINTER(C.c_double)), ("sec_dir", C.POINTER(C.c_double))]
-
-
-class BoxVolume(C.Structure):
-    _fields_ = [("half", C.c_double * 3), ("material", C.c_int32), ("pad", C.c_int32)]
-
-
-class Placement(C.Structure):
-    _fields_ = [("mother", C.c_int32), ("daughter", C.c_int32), ("translation", C.c_double * 3)]
 
 
 def box_tree(volumes, placements):
@@ -219,6 +221,51 @@ def config_calorimeter_cms(precision=FP64, stepper=STEPPER_HELIX, **kw):
              field_stepper=stepper)
     d.update(kw)
     return config_calorimeter(precision=precision, **d)
+
+
+def set_tree(c, volumes, placements, world=0):
+    """Make config c a GXT_GEOM_TREE setup from [(half_xyz, material)] and
+    [(mother, daughter, translation_xyz)] (the arrays stay referenced by c)."""
+    vol, pl = box_tree(volumes, placements)
+    c._tree = (vol, pl)
+    c.geometry = GEOM_TREE
+    c.tree_volumes = C.cast(vol, C.POINTER(BoxVolume))
+    c.tree_placements = C.cast(pl, C.POINTER(Placement))
+    c.n_tree_volumes = len(volumes)
+    c.n_tree_placements = len(placements)
+    c.tree_world = world
+    return c
+
+
+def config_reference_calorimeter(precision=FP64, **kw):
+    """The reference's calorimeter as a volume tree
+    (proxy/test/BasicCpuTransport/ProxyDetectorConstruction.cpp:38-60,
+    225-325): 50 layers of 10 mm Pb + 10 mm lAr along x, 1 m square, in a
+    vacuum world 1.2 times its size; no field unless given; 0.7 mm cuts,
+    1 keV tracking cut.  lAr at the reference tables' 1.396 g/cm3."""
+    tree = kw.pop("tree", None) or reference_calorimeter_tree()
+    d = dict(absorber=MAT_PB, gap=MAT_LAR, lar_density=1.396, seed=12345, n_zbins=20, n_rbins=20,
+             zbin_mm=50.0, rbin_mm=25.0, max_events=1000, precision=precision)
+    d.update(kw)
+    return set_tree(base_config(**d), *tree)
+
+
+def reference_calorimeter_primaries(n, species=ELECTRON, ekin=10000.0, seed=0):
+    """Primaries of the reference's transport test (test_BasicCpuTransport.cpp:
+    556-586, 677-698): 10 GeV, position with each coordinate uniform in
+    [0, 1) mm, direction (u1, u2, u3) normalised with u uniform in [0, 1);
+    here from numpy's generator with the given seed, one event per primary,
+    substream = event index."""
+    rs = np.random.default_rng(seed)
+    p = primaries_array(n)
+    for i in range(n):
+        u = rs.random(6)
+        d = u[3:] / np.linalg.norm(u[3:])
+        p[i].species, p[i].event, p[i].ekin = species, i, ekin
+        p[i].pos[:] = [float(x) for x in u[:3]]
+        p[i].dir[:] = [float(x) for x in d]
+        p[i].substream = i
+    return p
 
 
 def primaries_array(n):

@@ -358,6 +358,7 @@ int load_material(const std::string &data, int which, double lar_density, HostDa
 
 void build_geometry(const gxt_config &c, DevGeometry &g) {
   std::memset(&g, 0, sizeof(g));
+  if (c.geometry == GXT_GEOM_TREE) return;  // built from the navigator by the engine
   g.hx = g.hy = 0.5 * c.box_xy;
   if (c.geometry == GXT_GEOM_BOX) {
     g.nvol = 1;

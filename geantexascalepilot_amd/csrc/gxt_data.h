@@ -24,6 +24,11 @@ int load_material(const std::string &data_dir, int which, double lar_density, Ho
 void build_geometry(const gxt_config &c, DevGeometry &g);
 void build_jump_matrices(uint32_t out[2][64][9]);
 
+// A navigator built by gxt_nav_create: its device view and its touchables
+// (host copy; n = their number).
+DevNav nav_device_view(const gxt_navigator *nav);
+const NavNode *nav_host_nodes(const gxt_navigator *nav, int *n);
+
 // Record msg as the library's last error (gxt_last_error) and return code.
 int set_error(int code, const std::string &msg);
 

@@ -1228,26 +1228,35 @@ __device__ __forceinline__ double nav_dist_out(const NavNode &b, const double p[
   return s < 0.0 ? 0.0 : s;
 }
 
-// The boundary step from p in touchable t (FindNextBoundaryAndStep,
-// NavigationInterface.hpp:64-98): distance to leaving t or entering one of its
+// Distance along v from p in touchable t to leaving t or entering one of its
 // daughters, whichever is first (ties leave t, then take the lowest
-// daughter), and the touchable holding p + s v.  Also the isotropic safety.
-__device__ __forceinline__ int nav_step(const DevNav &g, int t, const double p[3], const double v[3],
-                                        double &snext, double &safety) {
+// daughter); enter = that daughter, or -1.
+__device__ __forceinline__ double nav_boundary(const DevNav &g, int t, const double p[3], const double v[3],
+                                               int &enter) {
   const NavNode *node = g.node;
   const NavNode &m = node[t];
   double s = nav_dist_out(m, p, v);
-  int enter = -1;
+  enter = -1;
+  for (int d = m.first; d < m.first + m.count; ++d) {
+    const double sd = nav_dist_in(node[d], p, v);
+    if (sd < s) {
+      s = sd;
+      enter = d;
+    }
+  }
+  return s;
+}
+
+// Isotropic safety of p in touchable t: the distance to t's faces and to the
+// nearest daughter box.
+__device__ __forceinline__ double nav_safety(const DevNav &g, int t, const double p[3]) {
+  const NavNode *node = g.node;
+  const NavNode &m = node[t];
   double sm = kHuge;
   for (int k = 0; k < 3; ++k) sm = fmin(sm, m.h[k] - fabs(p[k] - m.c[k]));
   double dmin = kHuge;
   for (int d = m.first; d < m.first + m.count; ++d) {
     const NavNode &b = node[d];
-    const double sd = nav_dist_in(b, p, v);
-    if (sd < s) {
-      s = sd;
-      enter = d;
-    }
     double q = 0.0;
     for (int k = 0; k < 3; ++k) {
       const double e = fabs(p[k] - b.c[k]) - b.h[k];
@@ -1256,7 +1265,17 @@ __device__ __forceinline__ int nav_step(const DevNav &g, int t, const double p[3
     dmin = fmin(dmin, q);
   }
   if (dmin < kHuge) sm = fmin(sm, sqrt(dmin));
-  safety = sm < 0.0 ? 0.0 : sm;
+  return sm < 0.0 ? 0.0 : sm;
+}
+
+// The boundary step from p in touchable t (FindNextBoundaryAndStep,
+// NavigationInterface.hpp:64-98): nav_boundary's distance, the touchable
+// holding p + s v, and the isotropic safety.
+__device__ __forceinline__ int nav_step(const DevNav &g, int t, const double p[3], const double v[3],
+                                        double &snext, double &safety) {
+  int enter;
+  const double s = nav_boundary(g, t, p, v, enter);
+  safety = nav_safety(g, t, p);
   snext = s;
   const double q[3] = {p[0] + s * v[0], p[1] + s * v[1], p[2] + s * v[2]};
   return nav_relocate(g, enter >= 0 ? enter : t, q, v);
@@ -1383,6 +1402,89 @@ __device__ __forceinline__ int propagate(const DevParams *P, int species, R e, d
       if (v2 < 0) return kEscaped;
       vol = v2;
       return kBoundary;
+    }
+    if (remaining <= 0.0) return kPhysics;
+  }
+  return kLimit;
+}
+
+// Volume of a starting point (-1: outside the setup), in either form.
+__device__ __forceinline__ int geo_locate(const DevGeometry &g, const double p[3], const double d[3]) {
+  if (g.tree) return nav_locate(g.nav, p, d);
+  return locate(g, p, d[0], d[1], d[2]);
+}
+
+// Tally cell of volume v (DevGeometry): v itself in the slab form, cell[v]
+// for a touchable of the tree form.  kGeo 0: slab, 1: tree, 2: read g.tree.
+template <int kGeo>
+__device__ __forceinline__ int geo_cell(const DevGeometry &g, int v) {
+  if constexpr (kGeo == 0)
+    return v;
+  else if constexpr (kGeo == 1)
+    return g.cell[v];
+  else
+    return g.tree ? g.cell[v] : v;
+}
+
+// Propagation in the tree form: along the straight line to pstep or the next
+// boundary (the touchable beyond it found by relocation from the daughter
+// entered, or from the touchable left); in a uniform field the helix in
+// sub-steps chosen as in propagate, from the navigator's safety and
+// straight-line distance, a sub-step ending the step when the point it
+// reaches lies in another touchable.  A track in vacuum passes pstep = kHuge.
+template <class R>
+__device__ __forceinline__ int propagate_tree(const DevParams *P, int species, R e, double pos[3], R dir[3], int &vol,
+                                              double pstep, double &step) {
+  const DevNav &g = P->geo.nav;
+  if (species == 0 || P->bmag == 0.0) {
+    const double v[3] = {(double)dir[0], (double)dir[1], (double)dir[2]};
+    int enter;
+    const double sn = nav_boundary(g, vol, pos, v, enter);
+    if (pstep < sn) {
+      pos[0] += pstep * v[0];
+      pos[1] += pstep * v[1];
+      pos[2] += pstep * v[2];
+      step = pstep;
+      return kPhysics;
+    }
+    pos[0] += sn * v[0];
+    pos[1] += sn * v[1];
+    pos[2] += sn * v[2];
+    step = sn;
+    vol = nav_relocate(g, enter >= 0 ? enter : vol, pos, v);
+    return vol < 0 ? kEscaped : kBoundary;
+  }
+  const R q = species == 1 ? R(-1) : R(1);
+  const R mom = m_sqrt(e * (e + R(2) * R(kMe)));
+  const R ub[3] = {(R)(P->B[0] / P->bmag), (R)(P->B[1] / P->bmag), (R)(P->B[2] / P->bmag)};
+  double remaining = pstep;
+  step = 0.0;
+  for (int it = 0; it < 64; ++it) {
+    const double saf = nav_safety(g, vol, pos);
+    if (remaining <= saf) {
+      helix_step<R>(P->B, P->bmag, q, mom, (R)remaining, pos, dir);
+      step += remaining;
+      return kPhysics;
+    }
+    int enter;
+    const double v[3] = {(double)dir[0], (double)dir[1], (double)dir[2]};
+    const double slin = nav_boundary(g, vol, pos, v, enter);
+    const R pl = mom * (dir[0] * ub[0] + dir[1] * ub[1] + dir[2] * ub[2]);
+    R pt2 = mom * mom - pl * pl;
+    if (pt2 < R(0)) pt2 = R(0);
+    const R pt = m_sqrt(pt2);
+    const double sl = pt > R(0) ? (double)(R(2) * m_sqrt(R(0.1) / (R(0.299792458) * (R)P->bmag / pt))) : kHuge;
+    // aimed at the boundary when the straight line reaches it within the
+    // sagitta-limited length
+    const double s = (slin <= sl && slin <= remaining) ? fmax(slin, 1.0e-6) : fmin(remaining, fmax(sl, saf));
+    helix_step<R>(P->B, P->bmag, q, mom, (R)s, pos, dir);
+    step += s;
+    remaining -= s;
+    const double w[3] = {(double)dir[0], (double)dir[1], (double)dir[2]};
+    const int v2 = nav_relocate(g, vol, pos, w);
+    if (v2 != vol) {
+      vol = v2;
+      return v2 < 0 ? kEscaped : kBoundary;
     }
     if (remaining <= 0.0) return kPhysics;
   }

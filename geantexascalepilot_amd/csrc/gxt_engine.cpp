@@ -294,6 +294,12 @@ struct gxt_ctx {
   // non-uniform field
   gxt_field_map fmap;
   std::string fmap_path;
+  // tree form of the geometry: the caller's volumes and placements (copied),
+  // the navigator and the touchable -> tally cell map on the device
+  std::vector<gxt_box_volume> tree_vol;
+  std::vector<gxt_placement> tree_pl;
+  gxt_navigator *nav = nullptr;
+  int32_t *cell = nullptr;
   int transport_mode = 0;  // e-/e+ transport kernel (StepArgs::transport_mode)
   int inner_steps = 4;     // StepArgs::inner_steps
   // jobs
@@ -340,12 +346,14 @@ struct gxt_ctx {
     a.trace_cap = trace_cap;
     a.transport_mode = transport_mode;
     a.inner_steps = inner_steps;
-    // the mode-0 transports stage their tables in LDS when they take at most
+    // the transports stage their tables in LDS when they take at most
     // kLdsTablesMax bytes
     const int nb = precision == GXT_FP32 ? (int)sizeof(Node32) : (int)sizeof(Node64);
     for (int l = 0; l < 2; ++l) {
       const int bytes = hd.params.lds_nodes[l] * nb;
-      a.lds_bytes[l] = (bytes > 0 && bytes <= kLdsTablesMax && (l == 0 || transport_mode == 0)) ? bytes : 0;
+      // (the e-/e+ tables in LDS do not include the multiple-scattering ones)
+      const bool lds_ok = l == 0 || transport_mode == 0 || (transport_mode == 4 && !hd.params.msc);
+      a.lds_bytes[l] = (bytes > 0 && bytes <= kLdsTablesMax && lds_ok) ? bytes : 0;
     }
     return a;
   }
@@ -703,6 +711,8 @@ void destroy_ctx(gxt_ctx *c) {
       if (x) (void)hipEventDestroy(x);
   c->dd.release();
   release_field_map(c->fmap);
+  if (c->nav) (void)gxt_nav_free(c->nav);
+  (void)hipFree(c->cell);
   if (c->snap) (void)hipEventDestroy(c->snap);
   for (auto &jp : c->jobs) {
     if (jp->dprims) (void)hipFree(jp->dprims);
@@ -726,9 +736,66 @@ void destroy_ctx(gxt_ctx *c) {
   delete c;
 }
 
+// The tree form of the geometry (GXT_GEOM_TREE): the navigator on the
+// context's device, and the tally cells: the touchables with a material in
+// touchable order (slot 0 absorber, 1 gap, which is visible), then one
+// vacuum cell (slot -1).
+int build_tree_geometry(gxt_ctx *c, const gxt_config &cfg, DevGeometry &g) {
+  c->tree_vol.assign(cfg.tree_volumes, cfg.tree_volumes + cfg.n_tree_volumes);
+  c->tree_pl.assign(cfg.tree_placements, cfg.tree_placements + cfg.n_tree_placements);
+  c->cfg.tree_volumes = c->tree_vol.data();
+  c->cfg.tree_placements = c->tree_pl.empty() ? nullptr : c->tree_pl.data();
+  int rc = gxt_nav_create(c->tree_vol.data(), (int)c->tree_vol.size(), c->cfg.tree_placements,
+                          (int)c->tree_pl.size(), cfg.tree_world, &c->nav);
+  if (rc) return rc;
+  int nt = 0;
+  const NavNode *node = nav_host_nodes(c->nav, &nt);
+  std::vector<int32_t> cell(nt);
+  std::memset(g.z, 0, sizeof(g.z));
+  g.hx = g.hy = 0.0;
+  int k = 0;
+  for (int t = 0; t < nt; ++t) {
+    const int m = c->tree_vol[node[t].volume].material;
+    if (m < 0) continue;
+    if (k >= kMaxVol - 1)
+      return set_err(GXT_EINVAL, "tree geometry: more than " + std::to_string(kMaxVol - 1) +
+                                     " touchables with a material (one tally cell each)");
+    cell[t] = k;
+    // a material that is both absorber and gap takes the gap's slot
+    g.slot[k] = m == cfg.gap ? 1 : 0;
+    g.visible[k] = m == cfg.gap ? 1 : 0;
+    ++k;
+  }
+  for (int t = 0; t < nt; ++t)
+    if (c->tree_vol[node[t].volume].material < 0) cell[t] = k;
+  g.slot[k] = -1;
+  g.visible[k] = 0;
+  g.nvol = k + 1;
+  g.tree = 1;
+  HIP_TRY(dalloc(&c->cell, (size_t)nt));
+  HIP_TRY(hipMemcpy(c->cell, cell.data(), (size_t)nt * sizeof(int32_t), hipMemcpyHostToDevice));
+  g.cell = c->cell;
+  g.nav = nav_device_view(c->nav);
+  return GXT_OK;
+}
+
 int check_config(const gxt_config *c) {
   if (!c) return set_err(GXT_EINVAL, "null config");
-  if (c->geometry != GXT_GEOM_BOX && c->geometry != GXT_GEOM_CALO) return set_err(GXT_EINVAL, "bad geometry");
+  if (c->geometry != GXT_GEOM_BOX && c->geometry != GXT_GEOM_CALO && c->geometry != GXT_GEOM_TREE)
+    return set_err(GXT_EINVAL, "bad geometry");
+  if (c->geometry == GXT_GEOM_TREE) {
+    if (!c->tree_volumes || c->n_tree_volumes <= 0 || c->n_tree_placements < 0 ||
+        (c->n_tree_placements > 0 && !c->tree_placements) || c->tree_world < 0 ||
+        c->tree_world >= c->n_tree_volumes)
+      return set_err(GXT_EINVAL, "tree geometry: bad volume or placement arrays");
+    for (int i = 0; i < c->n_tree_volumes; ++i) {
+      const int m = c->tree_volumes[i].material;
+      if (m != -1 && m != c->absorber && m != c->gap)
+        return set_err(GXT_EINVAL, "tree geometry: a volume's material is neither the absorber, the gap nor vacuum");
+    }
+    if (c->field_map || c->field_stepper != GXT_STEPPER_HELIX)
+      return set_err(GXT_EINVAL, "tree geometry: uniform field only (no field map, HELIX stepper)");
+  }
   if (c->geometry == GXT_GEOM_CALO && (c->n_layers <= 0 || 2 * c->n_layers > kMaxVol))
     return set_err(GXT_EINVAL, "n_layers out of range");
   if (c->n_zbins <= 0 || c->n_zbins > kMaxZBins || c->n_rbins <= 0 || c->n_rbins > kMaxRBins)
@@ -946,6 +1013,10 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
     if (rc) return set_err(rc, err);
   }
   build_geometry(*cfg, P.geo);
+  if (cfg->geometry == GXT_GEOM_TREE) {
+    rc = build_tree_geometry(c.get(), *cfg, P.geo);
+    if (rc) return rc;
+  }
   P.B[0] = cfg->field[0];
   P.B[1] = cfg->field[1];
   P.B[2] = cfg->field[2];
@@ -963,8 +1034,9 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
     for (size_t s = 0; s < mats.size(); ++s)
       if (P.mat[s].tab[GXT_TAB_MSC_EM].n == 0 || P.mat[s].tab[GXT_TAB_MSC_EP].n == 0)
         return set_err(GXT_EINVAL, "multiple scattering needs the LambdaMod1.msc tables (none for PbWO4)");
-  c->transport_mode =
-      P.field.general ? (P.field.stepper == GXT_STEPPER_RK45 ? 3 : 2) : (P.msc ? 1 : 0);
+  c->transport_mode = cfg->geometry == GXT_GEOM_TREE
+                           ? 4
+                           : (P.field.general ? (P.field.stepper == GXT_STEPPER_RK45 ? 3 : 2) : (P.msc ? 1 : 0));
   // Steps per transport launch for tracks that neither interact nor end (a
   // track's results do not depend on it).  Measured: in the block 2 is
   // fastest (10 GeV e- in PbWO4; 1: -2 %, 3: -1 %, 4: -2 %, 6: -4 %); in the
@@ -1093,7 +1165,18 @@ int gxt_submit_tracks(gxt_ctx *c, const gxt_track *trk, int n, int n_events, uin
     const gxt_track &t = trk[i];
     std::string bad;
     if (t.nsteps >= c->cfg.max_steps) bad = "nsteps >= max_steps";
-    if (bad.empty() && t.volume != -1) {
+    if (bad.empty() && t.volume != -1 && g.tree) {
+      int nt = 0;
+      const NavNode *node = nav_host_nodes(c->nav, &nt);
+      if (t.volume < 0 || t.volume >= nt) {
+        bad = "volume out of range";
+      } else {
+        // inside the touchable's box, within the navigator's surface tolerance
+        for (int k = 0; k < 3; ++k)
+          if (!(std::fabs(t.pos[k] - node[t.volume].c[k]) <= node[t.volume].h[k] + 1e-9))
+            bad = "position outside its volume";
+      }
+    } else if (bad.empty() && t.volume != -1) {
       if (t.volume < 0 || t.volume >= g.nvol)
         bad = "volume out of range";
       else if (!(std::fabs(t.pos[0]) <= g.hx && std::fabs(t.pos[1]) <= g.hy && t.pos[2] >= g.z[t.volume] &&
@@ -1188,6 +1271,17 @@ int gxt_get_profiles(gxt_ctx *c, double *ez, double *er, double *ev) {
       if (er) er[i] += r[nz + i];
     for (int i = 0; i < nv; ++i)
       if (ev) ev[i] += r[nz + nr + i];
+  }
+  return GXT_OK;
+}
+
+int gxt_volume_bins(gxt_ctx *c, int *n_bins, int32_t *cell) {
+  if (!c || !n_bins) return set_err(GXT_EINVAL, "bad arguments");
+  *n_bins = c->hd.params.geo.nvol;
+  if (cell && c->cell) {
+    int nt = 0;
+    (void)nav_host_nodes(c->nav, &nt);
+    HIP_TRY(hipMemcpy(cell, c->cell, (size_t)nt * sizeof(int32_t), hipMemcpyDeviceToHost));
   }
   return GXT_OK;
 }
@@ -1428,16 +1522,18 @@ int gxt_slab_distance(gxt_ctx *c, const double *pos, const double *dir, const do
                       double *snext, int32_t *next, double *safety, int32_t *boundary, int n) {
   if (!c || !pos || !dir || !vol || !snext || !next || !safety || n <= 0 || n > kMaxSoaPoints)
     return set_err(GXT_EINVAL, "bad slab_distance arguments (1 <= n <= 2^29)");
+  if (c->hd.params.geo.tree)
+    return c->fail(GXT_EINVAL, "slab_distance: the context's geometry is a volume tree (use the gxt_nav_* calls)");
   const int nvol = c->hd.params.geo.nvol;
   for (int i = 0; i < n; ++i) {
     const double d[3] = {dir[i], dir[n + i], dir[2 * n + i]};
     const double nn = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
     if (!std::isfinite(pos[i]) || !std::isfinite(pos[n + i]) || !std::isfinite(pos[2 * n + i]) ||
         !(std::fabs(nn - 1.0) <= 2e-6))
-      return set_err(GXT_EINVAL, "slab_distance: non-finite position or non-unit direction at " + std::to_string(i));
-    if (vol[i] >= nvol) return set_err(GXT_EINVAL, "slab_distance: volume out of range at " + std::to_string(i));
+      return c->fail(GXT_EINVAL, "slab_distance: non-finite position or non-unit direction at " + std::to_string(i));
+    if (vol[i] >= nvol) return c->fail(GXT_EINVAL, "slab_distance: volume out of range at " + std::to_string(i));
     if (pstep && !(pstep[i] >= 0.0 && std::isfinite(pstep[i])))
-      return set_err(GXT_EINVAL, "slab_distance: bad pstep at " + std::to_string(i));
+      return c->fail(GXT_EINVAL, "slab_distance: bad pstep at " + std::to_string(i));
   }
   DevBuf<double> dp(3 * (size_t)n), dd(3 * (size_t)n), ds(n), dsaf(n), dps(pstep ? n : 0);
   DevBuf<int32_t> dv(n), dn(n), db(boundary ? n : 0);

@@ -309,9 +309,11 @@ __device__ __forceinline__ int64_t stripe_cap(const Pool<R> &p) {
 struct Dep {
   double e = 0.0;
   int iz = 0, ir = 0, vol = 0;
-  template <class R>
+  // v: the track's volume (kGeo as geo_cell)
+  template <int kGeo = 2, class R>
   __device__ __forceinline__ void add(const DevParams *P, TrackRegs<R> &t, double de, const double p[3], int v) {
     if (de <= 0.0) return;
+    v = geo_cell<kGeo>(P->geo, v);
     t.aed += de;
     if (P->geo.visible[v]) t.avis += de;
     if (e == 0.0) {
@@ -423,7 +425,7 @@ __device__ __forceinline__ void inject_state(const DevParams *P, const StepArgs 
                                              TrackRegs<R> &t) {
   t.nil0 = t.nil1 = t.nil2 = R(-1);
   t.nsteps = 0;
-  t.vol = locate(P->geo, t.pos, p.dir[0], p.dir[1], p.dir[2]);
+  t.vol = geo_locate(P->geo, t.pos, p.dir);
   if (t.vol >= 0) substream_state(P, t.uid, t.rng.s);
 }
 
@@ -438,7 +440,7 @@ __device__ __forceinline__ void inject_state(const DevParams *P, const StepArgs 
   t.nil1 = (R)p.nil[1];
   t.nil2 = (R)p.nil[2];
   t.nsteps = p.nsteps;
-  t.vol = p.volume >= 0 ? p.volume : locate(P->geo, t.pos, p.dir[0], p.dir[1], p.dir[2]);
+  t.vol = p.volume >= 0 ? p.volume : geo_locate(P->geo, t.pos, p.dir);
   if (t.vol >= 0) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) t.rng.s[k] = (double)p.rng[k];
@@ -498,8 +500,10 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
 // scattering; 1: the same propagation with e-/e+ multiple scattering; 2: e-/e+
 // in a field map with the local helix, 3: with the RK45 stepper (map or
 // uniform field), both through propagate_general and with or without
-// multiple scattering.  Each mode is its own kernel, so the simpler ones keep
-// their register budgets.
+// multiple scattering; 4: the tree form of the geometry (volume-tree
+// navigation, vacuum volumes), straight lines and the uniform-field helix,
+// with or without multiple scattering.  Each mode is its own kernel, so the
+// simpler ones keep their register budgets.
 // The plain e-/e+ transport (no field map, RK45 or multiple scattering) fits
 // 3 waves per SIMD with a 28 B/lane spill; that measured 1.6 % faster on the
 // 10 GeV e- / PbWO4 batch than 2 waves without spill.  The other modes spill
@@ -570,6 +574,7 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
   if (bad && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.cnt->overflow, 1);
   const int total = span.total();
   const R mine = (R)P->tracking_cut;
+  constexpr int kG = kMode == 4 ? 1 : 0;  // geo_cell form
   for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < total; base += gridDim.x * blockDim.x) {
     const int i = base + lane();
     const bool active = i < total;
@@ -594,10 +599,16 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
       survive = false;
       edep = 0.0;
       Dep dA, dB;  // along-step loss (mid-step, pre-step volume); local deposit at the end point
-      const DevMaterial &m = P->mat[P->geo.slot[t.vol]];
+      // material slot of the track's volume; -1 (tree form only): vacuum, no
+      // physics, the track flies to the next boundary
+      const int mslot = P->geo.slot[geo_cell<kG>(P->geo, t.vol)];
+      const bool vac = kMode == 4 && mslot < 0;
+      const DevMaterial &m = P->mat[vac ? 0 : mslot];
       R xs0 = R(0), xs1 = R(0), xs2 = R(0);
       double bestd = kHuge;
       int sel = -1;
+      double lam1 = kMscNone;
+      if (!vac) {
       gpil<0>(P, N, m, sp, t.e, t.nil0, xs0, t.rng, bestd, sel);
       gpil<1>(P, N, m, sp, t.e, t.nil1, xs1, t.rng, bestd, sel);
       if (sp != 1) gpil<2>(P, N, m, sp, t.e, t.nil2, xs2, t.rng, bestd, sel);
@@ -611,20 +622,20 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
       // multiple scattering: its step limit; then the geometric length of the
       // true path is propagated (the step is the true path for interaction
       // lengths and loss)
-      double lam1 = kMscNone;
       if constexpr (kMode != 0) {
         if (charged && P->msc) {
           lam1 = msc_lambda1<R>(N, m, sp == 2, t.e);
           if (lam1 < kMscNone) {
             const double range = (double)pvec_value(N, m.tab[sp == 2 ? GXT_TAB_RANGE_EP : GXT_TAB_RANGE_EM], t.e);
-            const double tlim =
-                fmax(fmax(kMscFacRange * fmax(range, lam1), kMscFacSafety * safety(P->geo, t.vol, t.pos)), kMscTmin);
+            const double saf = kMode == 4 ? nav_safety(P->geo.nav, t.vol, t.pos) : safety(P->geo, t.vol, t.pos);
+            const double tlim = fmax(fmax(kMscFacRange * fmax(range, lam1), kMscFacSafety * saf), kMscTmin);
             if (tlim < bestd) {
               bestd = tlim;
               sel = -1;
             }
           }
         }
+      }
       }
       const double pre[3] = {t.pos[0], t.pos[1], t.pos[2]};
       const int prevol = t.vol;
@@ -634,7 +645,9 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
         const bool msc_on = lam1 < kMscNone;
         const double request = msc_on ? msc_geom(bestd, lam1) : bestd;
         double gstep = 0.0;
-        if constexpr (kMode == 2)
+        if constexpr (kMode == 4)
+          status = propagate_tree<R>(P, sp, t.e, t.pos, t.dir, t.vol, request, gstep);
+        else if constexpr (kMode == 2)
           status = propagate_general<R, GXT_STEPPER_HELIX>(P, sp, t.e, t.pos, t.dir, t.vol, request, gstep);
         else if constexpr (kMode == 3)
           status = propagate_general<R, GXT_STEPPER_RK45>(P, sp, t.e, t.pos, t.dir, t.vol, request, gstep);
@@ -648,12 +661,12 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
       if (xs1 > R(0)) t.nil1 -= (R)step * xs1;
       if (xs2 > R(0)) t.nil2 -= (R)step * xs2;
       t.nsteps++;
-      if (charged) {
+      if (charged && !vac) {
         t.alen += step;
         R loss = R(0);
         t.e = along_doit(N, m, t.species == 2, t.e, (R)step, mine, loss);
         const double mid[3] = {0.5 * (pre[0] + t.pos[0]), 0.5 * (pre[1] + t.pos[1]), 0.5 * (pre[2] + t.pos[2])};
-        dA.add(P, t, (double)loss, mid, prevol);
+        dA.add<kG>(P, t, (double)loss, mid, prevol);
         edep += (double)loss;
         if constexpr (kMode != 0) {
           if (lam1 < kMscNone && status != kEscaped && t.e > R(0) && step > 0.0)
@@ -676,11 +689,11 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
       } else {
         if (trace) trace_rec(a, t, tproc, edep);
         if (t.e < mine) {
-          dB.add(P, t, (double)t.e, t.pos, t.vol);
+          dB.add<kG>(P, t, (double)t.e, t.pos, t.vol);
           annihilate = t.species == 2;
           dies = !annihilate;
         } else if (t.nsteps >= P->max_steps) {
-          dB.add(P, t, (double)t.e, t.pos, t.vol);
+          dB.add<kG>(P, t, (double)t.e, t.pos, t.vol);
           killed = true;
           dies = true;
         } else {
@@ -831,7 +844,8 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 8 ? 3 : GXT_STEP_WAVES) void k
         outN = 2;
         t.e = R(0);
       } else {
-        const DevMaterial &m = P->mat[P->geo.slot[t.vol]];
+        // an interacting track is in a material (the tree form's vacuum has no physics)
+        const DevMaterial &m = P->mat[max(P->geo.slot[geo_cell<2>(P->geo, t.vol)], 0)];
         if (kCharged)
           sample_charged<R>(P, m, q, t.e, t.dir, t.rng, o);
         else
@@ -1220,7 +1234,11 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
   if (e == hipSuccess) e = hipStreamWaitEvent(s.side, s.fork, 0);
   if (e != hipSuccess) return e;
   if (ev && ev[0]) (void)hipEventRecord(ev[0], s.main);
-  if (a.lds_bytes[0] > 0)
+  if (a.transport_mode == 4 && a.lds_bytes[0] > 0)
+    hipLaunchKernelGGL((k_transport<R, false, 4, true>), dim3(grid), dim3(kBlock), a.lds_bytes[0], s.main, a, pa);
+  else if (a.transport_mode == 4)
+    hipLaunchKernelGGL((k_transport<R, false, 4>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
+  else if (a.lds_bytes[0] > 0)
     hipLaunchKernelGGL((k_transport<R, false, 0, true>), dim3(grid), dim3(kBlock), a.lds_bytes[0], s.main, a, pa);
   else
     hipLaunchKernelGGL((k_transport<R, false>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
@@ -1230,7 +1248,11 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
   hipLaunchKernelGGL((k_derive<R, 0>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
   if (ev && ev[3]) (void)hipEventRecord(ev[3], s.main);
   if (ev && ev[4]) (void)hipEventRecord(ev[4], s.side);
-  if (a.transport_mode == 3)
+  if (a.transport_mode == 4 && a.lds_bytes[1] > 0)
+    hipLaunchKernelGGL((k_transport<R, true, 4, true>), dim3(grid), dim3(kBlock), a.lds_bytes[1], s.side, a, pa);
+  else if (a.transport_mode == 4)
+    hipLaunchKernelGGL((k_transport<R, true, 4>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
+  else if (a.transport_mode == 3)
     hipLaunchKernelGGL((k_transport<R, true, 3>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
   else if (a.transport_mode == 2)
     hipLaunchKernelGGL((k_transport<R, true, 2>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);

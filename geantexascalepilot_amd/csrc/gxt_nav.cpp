@@ -39,6 +39,14 @@ struct gxt_navigator {
 
 using namespace gxt;
 
+namespace gxt {
+DevNav nav_device_view(const gxt_navigator *nav) { return nav->view(); }
+const NavNode *nav_host_nodes(const gxt_navigator *nav, int *n) {
+  *n = (int)nav->node.size();
+  return nav->node.data();
+}
+}  // namespace gxt
+
 namespace {
 
 constexpr int kMaxTouchables = 65535;

@@ -85,13 +85,33 @@ struct DevMaterial {
   float sx_hlog0, sx_hscale;
 };
 
+struct NavNode;
+struct NavGrid;
+
+// A volume tree on the device.
+struct DevNav {
+  const NavNode *node;
+  const NavGrid *grid;   // per volume
+  const int32_t *idx;
+  int ntouch;
+  int pad;
+};
+
+// The transport geometry.  Slab form (tree = 0): nvol volumes between the
+// planes z[v] and z[v + 1], |x| <= hx, |y| <= hy; a track's volume is v.
+// Tree form (tree = 1): a track's volume is a touchable of nav, and
+// cell[touchable] its tally cell; cells [0, nvol - 1) are the touchables
+// with a material, in touchable order, and cell nvol - 1 is vacuum (slot -1).
+// slot[] and visible[] are per cell (in the slab form cells are volumes).
 struct DevGeometry {
   int nvol;
-  int pad;
+  int tree;
   double hx, hy;
   double z[kMaxVol + 1];
   int slot[kMaxVol];
   int visible[kMaxVol];
+  const int32_t *cell;
+  DevNav nav;
 };
 
 // Magnetic field seen by the general (non-uniform or Runge-Kutta)
@@ -144,15 +164,6 @@ constexpr int kMaxSoaPoints = 1 << 29;
 
 // Largest tree (nodes, grids and index) the navigation kernels copy into LDS.
 constexpr int kNavLdsMax = 48 * 1024;
-
-// A volume tree on the device.
-struct DevNav {
-  const NavNode *node;
-  const NavGrid *grid;   // per volume
-  const int32_t *idx;
-  int ntouch;
-  int pad;
-};
 
 // Everything a kernel needs besides the pool: lives in one device buffer.
 struct DevParams {

@@ -274,7 +274,9 @@ class Transport:
         self.cfg = cfg
         self.ctx = C.c_void_p()
         _check(lib.gxt_create(C.byref(cfg), C.byref(self.ctx)))
-        self.nvol = 1 if cfg.geometry == abi.GEOM_BOX else 2 * cfg.n_layers
+        nb = C.c_int()
+        _check(lib.gxt_volume_bins(self.ctx, C.byref(nb), None))
+        self.nvol = nb.value
 
     def close(self):
         if self.ctx:

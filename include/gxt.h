@@ -34,8 +34,22 @@ extern "C" {
 #define GXT_ENOTFOUND (-6) /* unknown job id                            */
 
 /* ---- configuration ----------------------------------------------------- */
+/* A volume of a box geometry (GXT_GEOM_TREE, and the navigator entry points
+ * below) and the placement of one volume inside another. */
+typedef struct gxt_box_volume {
+  double half[3];      /* half-lengths (mm)                         */
+  int32_t material;    /* GXT_MAT_* or -1 (vacuum)                  */
+  int32_t pad;
+} gxt_box_volume;
+typedef struct gxt_placement {
+  int32_t mother;      /* index into the volume array               */
+  int32_t daughter;    /* index into the volume array               */
+  double translation[3]; /* daughter centre in the mother frame (mm) */
+} gxt_placement;
+
 #define GXT_GEOM_BOX 0  /* homogeneous block: |x|,|y| <= xy/2, 0 <= z <= depth */
 #define GXT_GEOM_CALO 1 /* z-stacked layers of absorber + gap                  */
+#define GXT_GEOM_TREE 2 /* boxes placed inside each other (gxt_config::tree_*)  */
 
 #define GXT_MAT_PB 0    /* G4_Pb 11.35 g/cm3 (reference tables, couple 1)    */
 #define GXT_MAT_LAR 1   /* G4_lAr (reference tables, couple 2, density-scaled) */
@@ -94,6 +108,22 @@ typedef struct gxt_config {
                                takes the exact constant-field path)                  */
   int32_t msc;              /* 1: multiple scattering of e-/e+ (see GXT_TAB_MSC_EM);
                                0: none, as in the reference's physics list           */
+  /* GXT_GEOM_TREE: the volumes and placements of a box geometry, as
+   * gxt_nav_create takes them (the reference's calorimeter is built this way,
+   * proxy/test/BasicCpuTransport/ProxyDetectorConstruction.cpp:225-325).  A
+   * volume's material is the absorber, the gap or -1 (vacuum: no physics, a
+   * track flies to the next boundary; its interaction lengths wait for the
+   * next material).  Deposits in gap volumes count as visible.  Per-volume
+   * tallies: one bin per touchable with a material, in touchable order (at
+   * most 127), then one bin for vacuum.  Uniform field only (no field_map,
+   * HELIX stepper); multiple scattering allowed.  Arrays are copied by
+   * gxt_create. */
+  const gxt_box_volume *tree_volumes;
+  const gxt_placement *tree_placements;
+  int32_t n_tree_volumes;
+  int32_t n_tree_placements;
+  int32_t tree_world;       /* index of the world volume                           */
+  int32_t pad1;
 } gxt_config;
 
 /* One primary particle. */
@@ -111,7 +141,7 @@ typedef struct gxt_event_tally {
   double edep;          /* total energy deposited in the setup (MeV)  */
   double visible;       /* deposited in the gap (calorimeter) or the block (box) */
   double leak;          /* kinetic energy leaving the setup           */
-  double charged_length;/* summed e-/e+ step length (mm)              */
+  double charged_length;/* summed e-/e+ step length in material (mm)  */
   int64_t n_secondaries[3]; /* secondaries created per species        */
   int64_t n_steps;      /* track-steps of this event                  */
   int64_t n_pairs;      /* gamma conversions                          */
@@ -268,8 +298,12 @@ int gxt_run(gxt_ctx *ctx, int64_t max_iterations);
 int gxt_poll(gxt_ctx *ctx, gxt_job_result *out, int max, int *n_out);
 /* Tallies (replaces ProxyDeviceManager::DownloadTrackData,
  * ProxyDeviceManager.cpp:206-259).  Arrays sized n_zbins, n_rbins and the
- * number of volumes (1 for the box, 2*n_layers for the calorimeter). */
+ * number of volume bins (gxt_volume_bins). */
 int gxt_get_profiles(gxt_ctx *ctx, double *edep_z, double *edep_r, double *edep_volume);
+/* Volume bins of gxt_get_profiles: 1 for the box, 2*n_layers for the
+ * calorimeter, the tree form's cells (touchables with a material, then
+ * vacuum).  cell (optional, one entry per touchable): each touchable's bin. */
+int gxt_volume_bins(gxt_ctx *ctx, int *n_bins, int32_t *cell);
 int gxt_get_event_tallies(gxt_ctx *ctx, uint64_t job_id, gxt_event_tally *out, int n);
 /* Energy histogram of created secondaries: 3 species x 64 bins of log10(E/MeV)
  * over [-3, 5). */
@@ -450,17 +484,9 @@ int gxt_field_step(const gxt_field_map *map, const double field[3], const double
  * IsSameLocation) and VecGeom's locate.  A volume's daughters must lie inside
  * it and must not overlap each other (checked when the navigator is built).
  * Touchables (placed instances) are numbered breadth-first from the world (0).
- * The transport loop itself uses the analytic slab navigator. */
-typedef struct gxt_box_volume {
-  double half[3];      /* half-lengths (mm)                         */
-  int32_t material;    /* GXT_MAT_* or -1 (vacuum)                  */
-  int32_t pad;
-} gxt_box_volume;
-typedef struct gxt_placement {
-  int32_t mother;      /* index into the volume array               */
-  int32_t daughter;    /* index into the volume array               */
-  double translation[3]; /* daughter centre in the mother frame (mm) */
-} gxt_placement;
+ * The transport loop uses the same navigator for GXT_GEOM_TREE, and the
+ * analytic slab navigator for the block and the z-stacked calorimeter. */
+/* gxt_box_volume and gxt_placement: see the configuration section. */
 typedef struct gxt_navigator gxt_navigator;
 /* world: the volume placed at the origin; at most 65535 touchables, 32
  * levels.  GXT_EINVAL for half-lengths below 1e-6 mm, a daughter outside its

@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <mutex>
 #include <sstream>
 #include <string>
@@ -601,12 +602,19 @@ static double heitler_xs(double ekin) {
 }
 
 // ---------------------------------------------------------------- geometry
+// Slab form: volumes v between the planes z[v] and z[v + 1].  Tree form
+// (GXT_GEOM_TREE): a track's volume is a touchable of the context's
+// navigator and cell[touchable] its tally cell (the touchables with a
+// material in touchable order, then one vacuum cell, slot -1); slot and
+// visible are per cell.
 struct Geometry {
   int type;
   double hx, hy;
   std::vector<double> z;      // plane positions, z[0] .. z[nvol]
-  std::vector<int> slot;      // material slot per volume
+  std::vector<int> slot;      // material slot per volume (cell)
   std::vector<int> visible;   // volume counts toward visible energy
+  bool tree = false;
+  std::vector<int> cell;      // tree form: touchable -> cell
   int nvol() const { return (int)slot.size(); }
 };
 
@@ -616,6 +624,9 @@ static void build_geometry(const gxt_config &c, Geometry &g) {
   g.z.clear();
   g.slot.clear();
   g.visible.clear();
+  g.tree = false;
+  g.cell.clear();
+  if (c.geometry == GXT_GEOM_TREE) return;  // cells built with the navigator
   if (c.geometry == GXT_GEOM_BOX) {
     g.z = {0.0, c.box_depth};
     g.slot = {0};
@@ -1668,6 +1679,89 @@ enum { kPhysics = 0, kBoundary = 1, kEscaped = 2, kLimit = 3 };
 
 using namespace gxref;
 
+struct gxref_nav {
+  struct Touchable {
+    double c[3], h[3];
+    int parent, volume;
+    std::vector<int> daughters;
+  };
+  std::vector<Touchable> t;
+};
+
+namespace gxref {
+namespace nav {
+
+constexpr double kTol = 1e-9;
+
+static bool inside(const gxref_nav::Touchable &b, const double p[3], const double v[3]) {
+  for (int k = 0; k < 3; ++k) {
+    const double x = p[k] - b.c[k];
+    if (std::fabs(x) > b.h[k] + kTol) return false;                  // outside
+    if (std::fabs(x) >= b.h[k] - kTol && !(x * v[k] < 0.0)) return false;  // on the face, not entering
+  }
+  return true;
+}
+
+static int down(const gxref_nav &g, int cur, const double p[3], const double v[3]) {
+  for (int d : g.t[cur].daughters)
+    if (inside(g.t[d], p, v)) return down(g, d, p, v);
+  return cur;
+}
+
+static int relocate(const gxref_nav &g, int cur, const double p[3], const double v[3]) {
+  while (cur >= 0 && !inside(g.t[cur], p, v)) cur = g.t[cur].parent;
+  return cur < 0 ? -1 : down(g, cur, p, v);
+}
+
+// UnplacedBox DistanceToIn: entry into the slab intersection, if the ray
+// runs more than the tolerance inside the box.
+static double to_in(const gxref_nav::Touchable &b, const double p[3], const double v[3]) {
+  double enter = -kHuge, leave = kHuge;
+  for (int k = 0; k < 3; ++k) {
+    const double x = p[k] - b.c[k];
+    if (v[k] == 0.0) {
+      if (std::fabs(x) >= b.h[k] - kTol) return kHuge;  // parallel, outside the slab or on its face
+      continue;
+    }
+    const double a = (-b.h[k] - x) / v[k], c = (b.h[k] - x) / v[k];
+    enter = std::max(enter, std::min(a, c));
+    leave = std::min(leave, std::max(a, c));
+  }
+  enter = std::max(enter, 0.0);
+  return leave > enter + kTol ? enter : kHuge;
+}
+
+// UnplacedBox DistanceToOut from inside.
+static double to_out(const gxref_nav::Touchable &b, const double p[3], const double v[3]) {
+  double s = kHuge;
+  for (int k = 0; k < 3; ++k) {
+    if (v[k] == 0.0) continue;
+    const double x = p[k] - b.c[k];
+    s = std::min(s, ((v[k] > 0.0 ? b.h[k] : -b.h[k]) - x) / v[k]);
+  }
+  return std::max(s, 0.0);
+}
+
+static double safety(const gxref_nav &g, int cur, const double p[3]) {
+  const gxref_nav::Touchable &m = g.t[cur];
+  double s = kHuge;
+  for (int k = 0; k < 3; ++k) s = std::min(s, m.h[k] - std::fabs(p[k] - m.c[k]));
+  double q2 = kHuge;
+  for (int d : m.daughters) {
+    double q = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double e = std::fabs(p[k] - g.t[d].c[k]) - g.t[d].h[k];
+      if (e > 0.0) q += e * e;
+    }
+    q2 = std::min(q2, q);
+  }
+  if (q2 < kHuge) s = std::min(s, std::sqrt(q2));
+  return std::max(s, 0.0);
+}
+
+}  // namespace nav
+}  // namespace gxref
+
 struct gxref_ctx {
   gxt_config cfg;
   std::string data;
@@ -1678,6 +1772,7 @@ struct gxref_ctx {
   Field field;          // map or uniform field seen by e-/e+ on the general path
   bool general = false; // e-/e+ propagate through propagate_general (a map, or RK45)
   bool msc = false;     // e-/e+ multiple scattering
+  std::unique_ptr<gxref_nav> nav;  // tree form of the geometry
   uint32_t seed[6];
   std::vector<EventTally> events;
   Worker tot;
@@ -1685,8 +1780,18 @@ struct gxref_ctx {
 
 namespace gxref {
 
+// Tally cell of a track's volume.
+static int cell_of(const gxref_ctx &c, int vol) { return c.geo.tree ? c.geo.cell[vol] : vol; }
+
+// Volume of a starting point, -1 outside the setup.
+static int geo_locate(const gxref_ctx &c, const double p[3], const double d[3]) {
+  if (!c.geo.tree) return locate(c.geo, p, d);
+  return nav::inside(c.nav->t[0], p, d) ? nav::down(*c.nav, 0, p, d) : -1;
+}
+
 static void deposit(const gxref_ctx &c, Worker &w, EventTally &ev, double e, const double p[3], int vol) {
   if (e <= 0.0) return;
+  vol = cell_of(c, vol);
   ev.edep += e;
   if (c.geo.visible[vol]) ev.visible += e;
   w.ev[vol] += e;
@@ -1830,9 +1935,78 @@ static int propagate_general(const gxref_ctx &c, Track &t, double pstep, double 
   return kLimit;
 }
 
+// Tree form: the straight line to pstep or the next boundary (the distance
+// to leaving the touchable or entering a daughter, the first daughter on a
+// tie; the touchable beyond found by relocation from the daughter entered or
+// the touchable left), or in a uniform field the helix in sub-steps chosen as
+// in the slab form below from the navigator's safety and straight-line
+// distance, a sub-step ending the step when its end point relocates to
+// another touchable (FieldPropagationHandler.cpp:93-111 with
+// NavigationInterface.hpp:64-98).
+static int propagate_tree(const gxref_ctx &c, Track &t, double pstep, double &step) {
+  const gxref_nav &g = *c.nav;
+  auto boundary = [&](int &target) {
+    double s = nav::to_out(g.t[t.vol], t.pos, t.dir);
+    target = t.vol;
+    for (int d : g.t[t.vol].daughters) {
+      const double sd = nav::to_in(g.t[d], t.pos, t.dir);
+      if (sd < s) {
+        s = sd;
+        target = d;
+      }
+    }
+    return s;
+  };
+  if (t.species == 0 || c.bmag == 0.0) {
+    int target;
+    const double sn = boundary(target);
+    if (pstep < sn) {
+      for (int i = 0; i < 3; ++i) t.pos[i] += pstep * t.dir[i];
+      step = pstep;
+      return kPhysics;
+    }
+    for (int i = 0; i < 3; ++i) t.pos[i] += sn * t.dir[i];
+    step = sn;
+    t.vol = nav::relocate(g, target, t.pos, t.dir);
+    return t.vol < 0 ? kEscaped : kBoundary;
+  }
+  const double q = t.species == 1 ? -1.0 : 1.0;
+  const double mom = std::sqrt(t.e * (t.e + 2.0 * kMe));
+  const double ub[3] = {c.bfield[0] / c.bmag, c.bfield[1] / c.bmag, c.bfield[2] / c.bmag};
+  double remaining = pstep;
+  step = 0.0;
+  for (int it = 0; it < 64; ++it) {
+    const double saf = nav::safety(g, t.vol, t.pos);
+    if (remaining <= saf) {
+      helix_step(c.bfield, c.bmag, q, mom, remaining, t.pos, t.dir);
+      step += remaining;
+      return kPhysics;
+    }
+    int target;
+    const double slin = boundary(target);
+    const double pl = mom * (t.dir[0] * ub[0] + t.dir[1] * ub[1] + t.dir[2] * ub[2]);
+    double pt2 = mom * mom - pl * pl;
+    if (pt2 < 0.0) pt2 = 0.0;
+    const double pt = std::sqrt(pt2);
+    const double sl = pt > 0.0 ? 2.0 * std::sqrt(0.1 / (0.299792458 * c.bmag / pt)) : kHuge;
+    const double s = slin <= sl && slin <= remaining ? std::max(slin, 1.0e-6) : std::min(remaining, std::max(sl, saf));
+    helix_step(c.bfield, c.bmag, q, mom, s, t.pos, t.dir);
+    step += s;
+    remaining -= s;
+    const int v2 = nav::relocate(g, t.vol, t.pos, t.dir);
+    if (v2 != t.vol) {
+      t.vol = v2;
+      return v2 < 0 ? kEscaped : kBoundary;
+    }
+    if (remaining <= 0.0) return kPhysics;
+  }
+  return kLimit;
+}
+
 static int propagate(const gxref_ctx &c, Track &t, double pstep, double &step) {
   const Geometry &g = c.geo;
   const bool charged = t.species != 0;
+  if (g.tree) return propagate_tree(c, t, pstep, step);
   if (charged && c.general) return propagate_general(c, t, pstep, step);
   if (!charged || c.bmag == 0.0) {
     int face;
@@ -1908,9 +2082,13 @@ static int propagate(const gxref_ctx &c, Track &t, double pstep, double &step) {
 
 // One step of one track (OneStep/InnerStep, proxy/test/test_BasicCpuTransport.cpp:358-497).
 static void one_step(gxref_ctx &c, Worker &w, EventTally &ev, std::vector<Track> &stack, Track &t, bool &alive) {
-  const Material &m = c.phys.mat[c.geo.slot[t.vol]];
+  // tree form: a vacuum volume (slot -1) has no physics; the track flies to
+  // the next boundary and keeps its interaction lengths
+  const int slot = c.geo.slot[cell_of(c, t.vol)];
+  const bool vac = slot < 0;
+  const Material &m = c.phys.mat[vac ? 0 : slot];
   const double mine = c.phys.tracking_cut;
-  const int np = nproc(t.species);
+  const int np = vac ? 0 : nproc(t.species);
   double xs[3] = {0, 0, 0};
   double best = kHuge;
   int sel = -1;
@@ -1925,7 +2103,7 @@ static void one_step(gxref_ctx &c, Worker &w, EventTally &ev, std::vector<Track>
     }
   }
   const bool charged = t.species != 0;
-  if (charged) {
+  if (charged && !vac) {
     const double al = along_limit(m, t.species == 2, t.e, mine);
     if (al < best) {
       best = al;
@@ -1935,12 +2113,12 @@ static void one_step(gxref_ctx &c, Worker &w, EventTally &ev, std::vector<Track>
   // multiple scattering: its step limit, then the geometric length of the
   // true path (the step is the true path for interaction lengths and loss)
   double lam1 = kMscNone;
-  if (charged && c.msc) {
+  if (charged && c.msc && !vac) {
     lam1 = msc_lambda1(m, t.species == 2, t.e);
     if (lam1 < kMscNone) {
       const double range = pvec_value(m.tab[t.species == 2 ? GXT_TAB_RANGE_EP : GXT_TAB_RANGE_EM], t.e);
-      const double tlim = std::max(
-          std::max(kMscFacRange * std::max(range, lam1), kMscFacSafety * safety(c.geo, t.vol, t.pos)), kMscTmin);
+      const double saf = c.geo.tree ? nav::safety(*c.nav, t.vol, t.pos) : safety(c.geo, t.vol, t.pos);
+      const double tlim = std::max(std::max(kMscFacRange * std::max(range, lam1), kMscFacSafety * saf), kMscTmin);
       if (tlim < best) {
         best = tlim;
         sel = -1;
@@ -1959,13 +2137,15 @@ static void one_step(gxref_ctx &c, Worker &w, EventTally &ev, std::vector<Track>
   ev.nsteps++;
   w.steps++;
   double edep = 0.0;
-  if (charged) {
+  if (charged && !vac) {
     ev.clen += step;
-    double loss = 0.0;
-    t.e = along_doit(m, t.species == 2, t.e, step, mine, loss);
-    const double mid[3] = {0.5 * (pre[0] + t.pos[0]), 0.5 * (pre[1] + t.pos[1]), 0.5 * (pre[2] + t.pos[2])};
-    deposit(c, w, ev, loss, mid, prevol);
-    edep += loss;
+    {
+      double loss = 0.0;
+      t.e = along_doit(m, t.species == 2, t.e, step, mine, loss);
+      const double mid[3] = {0.5 * (pre[0] + t.pos[0]), 0.5 * (pre[1] + t.pos[1]), 0.5 * (pre[2] + t.pos[2])};
+      deposit(c, w, ev, loss, mid, prevol);
+      edep += loss;
+    }
     if (msc_on && status != kEscaped && t.e > 0.0 && step > 0.0) msc_deflect(step / lam1, t.dir, t.rng);
   }
   int tproc = status == kPhysics ? sel : (status == kBoundary ? -2 : (status == kEscaped ? -3 : -1));
@@ -2044,7 +2224,7 @@ static Track primary_track(const gxref_ctx &c, const gxt_primary &p) {
   }
   t.uid = p.substream;
   substream_state(c.seed, t.uid, t.rng);
-  t.vol = locate(c.geo, t.pos, t.dir);
+  t.vol = geo_locate(c, t.pos, t.dir);
   return t;
 }
 
@@ -2075,7 +2255,7 @@ static Track basket_track(const gxref_ctx &c, const gxt_track &in) {
   }
   for (int k = 0; k < 6; ++k) t.rng[k] = in.rng[k];
   t.uid = in.substream;
-  t.vol = in.volume >= 0 ? in.volume : locate(c.geo, t.pos, t.dir);
+  t.vol = in.volume >= 0 ? in.volume : geo_locate(c, t.pos, t.dir);
   return t;
 }
 
@@ -2098,6 +2278,49 @@ static gxt_track to_basket(const Track &t) {
 
 }  // namespace gxref
 
+namespace gxref {
+
+// The tree form (GXT_GEOM_TREE): the navigator, and the tally cells of the
+// touchables (with a material: in touchable order, slot 0 absorber, 1 gap,
+// visible; vacuum: the last cell, slot -1).  At most 127 cells with a material.
+static int build_tree(const gxt_config &cfg, gxref_ctx &c) {
+  if (!cfg.tree_volumes || cfg.n_tree_volumes <= 0)
+    return fail(GXT_EINVAL, "tree geometry: bad volume or placement arrays");
+  for (int i = 0; i < cfg.n_tree_volumes; ++i) {
+    const int m = cfg.tree_volumes[i].material;
+    if (m != -1 && m != cfg.absorber && m != cfg.gap)
+      return fail(GXT_EINVAL, "tree geometry: a volume's material is neither the absorber, the gap nor vacuum");
+  }
+  if (cfg.field_map || cfg.field_stepper != GXT_STEPPER_HELIX)
+    return fail(GXT_EINVAL, "tree geometry: uniform field only (no field map, HELIX stepper)");
+  gxref_nav *g = nullptr;
+  const int rc = gxref_nav_create(cfg.tree_volumes, cfg.n_tree_volumes, cfg.tree_placements, cfg.n_tree_placements,
+                                  cfg.tree_world, &g);
+  if (rc) return rc;
+  c.nav.reset(g);
+  Geometry &geo = c.geo;
+  geo.tree = true;
+  geo.cell.assign(g->t.size(), -1);
+  int k = 0;
+  for (size_t t = 0; t < g->t.size(); ++t) {
+    const int m = cfg.tree_volumes[g->t[t].volume].material;
+    if (m < 0) continue;
+    if (k >= 127) return fail(GXT_EINVAL, "tree geometry: more than 127 touchables with a material");
+    geo.cell[t] = k++;
+    geo.slot.push_back(m == cfg.gap ? 1 : 0);
+    geo.visible.push_back(m == cfg.gap ? 1 : 0);
+  }
+  for (size_t t = 0; t < g->t.size(); ++t)
+    if (geo.cell[t] < 0) geo.cell[t] = k;
+  geo.slot.push_back(-1);
+  geo.visible.push_back(0);
+  c.cfg.tree_volumes = nullptr;  // the caller's arrays are not kept
+  c.cfg.tree_placements = nullptr;
+  return GXT_OK;
+}
+
+}  // namespace gxref
+
 // ---------------------------------------------------------------- C API
 extern "C" {
 
@@ -2111,6 +2334,13 @@ int gxref_create(const gxt_config *cfg, gxref_ctx **out) {
   for (int i = 0; i < 6; ++i) c->seed[i] = cfg->seed;
   c->phys.tracking_cut = cfg->tracking_cut;
   build_geometry(*cfg, c->geo);
+  if (cfg->geometry == GXT_GEOM_TREE) {
+    const int rc = build_tree(*cfg, *c);
+    if (rc) {
+      delete c;
+      return rc;
+    }
+  }
   std::vector<int> mats;
   if (cfg->geometry == GXT_GEOM_BOX)
     mats = {cfg->material};
@@ -2471,6 +2701,14 @@ int gxref_run_partial(gxref_ctx *c, const gxt_primary *prim, int n, int64_t step
   return GXT_OK;
 }
 
+int gxref_volume_bins(gxref_ctx *c, int *n_bins, int32_t *cell) {
+  if (!c || !n_bins) return fail(GXT_EINVAL, "bad arguments");
+  *n_bins = c->geo.nvol();
+  if (cell)
+    for (size_t t = 0; t < c->geo.cell.size(); ++t) cell[t] = c->geo.cell[t];
+  return GXT_OK;
+}
+
 int gxref_get_profiles(gxref_ctx *c, double *ez, double *er, double *ev) {
   if (!c) return fail(GXT_EINVAL, "null ctx");
   if (ez) std::copy(c->tot.ez.begin(), c->tot.ez.end(), ez);
@@ -2627,6 +2865,7 @@ int gxref_rotate_to_lab(double *u, double *v, double *w, const double *u1, const
 int gxref_slab_distance(gxref_ctx *c, const double *pos, const double *dir, const double *pstep, int32_t *vol,
                         double *snext, int32_t *next, double *saf, int32_t *boundary, int n) {
   if (!c || !pos || !dir || !vol || !snext || !next || !saf || n < 0) return fail(GXT_EINVAL, "bad slab request");
+  if (c->geo.tree) return fail(GXT_EINVAL, "slab request on a volume-tree geometry");
   const Geometry &g = c->geo;
   for (int i = 0; i < n; ++i)
     if (vol[i] >= g.nvol()) return fail(GXT_EINVAL, "volume out of range");
@@ -2806,88 +3045,6 @@ int gxref_msc_screening(const double *tau, int n, double *u) {
 // The surface tolerance is VecGeom's kTolerance, 1e-9 mm
 // (NavigationInterface.hpp:66); a point within it of a face belongs to the box
 // only if its direction enters through that face.
-struct gxref_nav {
-  struct Touchable {
-    double c[3], h[3];
-    int parent, volume;
-    std::vector<int> daughters;
-  };
-  std::vector<Touchable> t;
-};
-
-namespace gxref {
-namespace nav {
-
-constexpr double kTol = 1e-9;
-
-static bool inside(const gxref_nav::Touchable &b, const double p[3], const double v[3]) {
-  for (int k = 0; k < 3; ++k) {
-    const double x = p[k] - b.c[k];
-    if (std::fabs(x) > b.h[k] + kTol) return false;                  // outside
-    if (std::fabs(x) >= b.h[k] - kTol && !(x * v[k] < 0.0)) return false;  // on the face, not entering
-  }
-  return true;
-}
-
-static int down(const gxref_nav &g, int cur, const double p[3], const double v[3]) {
-  for (int d : g.t[cur].daughters)
-    if (inside(g.t[d], p, v)) return down(g, d, p, v);
-  return cur;
-}
-
-static int relocate(const gxref_nav &g, int cur, const double p[3], const double v[3]) {
-  while (cur >= 0 && !inside(g.t[cur], p, v)) cur = g.t[cur].parent;
-  return cur < 0 ? -1 : down(g, cur, p, v);
-}
-
-// UnplacedBox DistanceToIn: entry into the slab intersection, if the ray
-// runs more than the tolerance inside the box.
-static double to_in(const gxref_nav::Touchable &b, const double p[3], const double v[3]) {
-  double enter = -kHuge, leave = kHuge;
-  for (int k = 0; k < 3; ++k) {
-    const double x = p[k] - b.c[k];
-    if (v[k] == 0.0) {
-      if (std::fabs(x) >= b.h[k] - kTol) return kHuge;  // parallel, outside the slab or on its face
-      continue;
-    }
-    const double a = (-b.h[k] - x) / v[k], c = (b.h[k] - x) / v[k];
-    enter = std::max(enter, std::min(a, c));
-    leave = std::min(leave, std::max(a, c));
-  }
-  enter = std::max(enter, 0.0);
-  return leave > enter + kTol ? enter : kHuge;
-}
-
-// UnplacedBox DistanceToOut from inside.
-static double to_out(const gxref_nav::Touchable &b, const double p[3], const double v[3]) {
-  double s = kHuge;
-  for (int k = 0; k < 3; ++k) {
-    if (v[k] == 0.0) continue;
-    const double x = p[k] - b.c[k];
-    s = std::min(s, ((v[k] > 0.0 ? b.h[k] : -b.h[k]) - x) / v[k]);
-  }
-  return std::max(s, 0.0);
-}
-
-static double safety(const gxref_nav &g, int cur, const double p[3]) {
-  const gxref_nav::Touchable &m = g.t[cur];
-  double s = kHuge;
-  for (int k = 0; k < 3; ++k) s = std::min(s, m.h[k] - std::fabs(p[k] - m.c[k]));
-  double q2 = kHuge;
-  for (int d : m.daughters) {
-    double q = 0.0;
-    for (int k = 0; k < 3; ++k) {
-      const double e = std::fabs(p[k] - g.t[d].c[k]) - g.t[d].h[k];
-      if (e > 0.0) q += e * e;
-    }
-    q2 = std::min(q2, q);
-  }
-  if (q2 < kHuge) s = std::min(s, std::sqrt(q2));
-  return std::max(s, 0.0);
-}
-
-}  // namespace nav
-}  // namespace gxref
 
 extern "C" {
 

@@ -46,6 +46,8 @@ int gxref_run_tracks(gxref_ctx *ctx, const gxt_track *tracks, int n, int nthread
 int gxref_run_partial(gxref_ctx *ctx, const gxt_primary *prim, int n, int64_t steps_per_event, gxt_track *out,
                       int capacity, int *n_out);
 int gxref_get_profiles(gxref_ctx *ctx, double *edep_z, double *edep_r, double *edep_volume);
+/* Volume bins of gxref_get_profiles (gxt_volume_bins); cell optional, one per touchable. */
+int gxref_volume_bins(gxref_ctx *ctx, int *n_bins, int32_t *cell);
 int gxref_get_event_tallies(gxref_ctx *ctx, gxt_event_tally *out, int n);
 int gxref_get_spectra(gxref_ctx *ctx, int64_t *counts);
 int gxref_get_stats(gxref_ctx *ctx, gxt_stats *out);

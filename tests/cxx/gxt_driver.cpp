@@ -12,6 +12,9 @@
 //                                 per event
 //   gxt_driver DATA_DIR slab      gxt_slab_distance of fixed points in the
 //                                 PbWO4 block; one line per point
+//   gxt_driver DATA_DIR tree N    10 GeV e- x N through the reference
+//                                 calorimeter as the transport's volume
+//                                 tree; one line per event
 //
 // Test infrastructure: tests/test_gpu_cxx_driver.py compares its output with
 // the Python binding and the oracle.
@@ -129,10 +132,10 @@ static int run_slab(const char *data_dir) {
 // ProxyDetectorConstruction::CreateGeometry's calorimeter
 // (proxy/test/BasicCpuTransport/ProxyDetectorConstruction.cpp:225-325):
 // world (vacuum) > calo > 50 layers along x > 10 mm Pb + 10 mm lAr, 1 m square.
-static int run_nav() {
+static void reference_calorimeter(std::vector<gxt_box_volume> &vol, std::vector<gxt_placement> &pl) {
   const int nl = 50;
   const double th[2] = {10.0, 10.0}, yz = 1000.0, layer = th[0] + th[1], calo = nl * layer;
-  std::vector<gxt_box_volume> vol(5);
+  vol.resize(5);
   const double halves[5][3] = {{0.6 * calo, 0.6 * yz, 0.6 * yz},
                                {0.5 * layer, 0.5 * yz, 0.5 * yz},
                                {0.5 * calo, 0.5 * yz, 0.5 * yz},
@@ -144,7 +147,7 @@ static int run_nav() {
     vol[v].material = mats[v];
     vol[v].pad = 0;
   }
-  std::vector<gxt_placement> pl;
+  pl.clear();
   double start = -0.5 * layer;
   for (int i = 0; i < 2; ++i) {
     pl.push_back(gxt_placement{1, 3 + i, {start + 0.5 * th[i], 0.0, 0.0}});
@@ -156,6 +159,12 @@ static int run_nav() {
     start += layer;
   }
   pl.push_back(gxt_placement{0, 2, {0.0, 0.0, 0.0}});
+}
+
+static int run_nav() {
+  std::vector<gxt_box_volume> vol;
+  std::vector<gxt_placement> pl;
+  reference_calorimeter(vol, pl);
   gxt_navigator *nav = nullptr;
   if (gxt_nav_create(vol.data(), (int)vol.size(), pl.data(), (int)pl.size(), 0, &nav) != GXT_OK)
     return fail("gxt_nav_create");
@@ -188,15 +197,61 @@ static int run_nav() {
   return 0;
 }
 
+// Transport through that calorimeter as the context's geometry
+// (GXT_GEOM_TREE): n events of one 10 GeV e- each from (0.5, 0.25, 0.75) mm
+// along (2, 1, 2)/3, substream = event; one line per event.
+static int run_tree(const char *data_dir, int n) {
+  std::vector<gxt_box_volume> vol;
+  std::vector<gxt_placement> pl;
+  reference_calorimeter(vol, pl);
+  gxt_config cfg;
+  if (gxt_default_config(&cfg) != GXT_OK) return fail("gxt_default_config");
+  cfg.geometry = GXT_GEOM_TREE;
+  cfg.tree_volumes = vol.data();
+  cfg.tree_placements = pl.data();
+  cfg.n_tree_volumes = (int)vol.size();
+  cfg.n_tree_placements = (int)pl.size();
+  cfg.tree_world = 0;
+  cfg.absorber = GXT_MAT_PB;
+  cfg.gap = GXT_MAT_LAR;
+  cfg.max_events = n;
+  cfg.data_dir = data_dir;
+  gxt_ctx *ctx = nullptr;
+  if (gxt_create(&cfg, &ctx) != GXT_OK) return fail("gxt_create");
+  int nbins = 0;
+  if (gxt_volume_bins(ctx, &nbins, nullptr) != GXT_OK) return fail("gxt_volume_bins");
+  std::printf("bins %d\n", nbins);
+  std::vector<gxt_primary> prim(n);
+  for (int i = 0; i < n; ++i) {
+    prim[i].species = 1;
+    prim[i].event = i;
+    prim[i].ekin = 10000.0;
+    prim[i].pos[0] = 0.5, prim[i].pos[1] = 0.25, prim[i].pos[2] = 0.75;
+    prim[i].dir[0] = 2.0 / 3.0, prim[i].dir[1] = 1.0 / 3.0, prim[i].dir[2] = 2.0 / 3.0;
+    prim[i].substream = (uint64_t)i;
+  }
+  std::vector<gxt_event_tally> t(n);
+  if (gxt_submit(ctx, prim.data(), n, 1) != GXT_OK || gxt_run(ctx, 0) != GXT_OK ||
+      gxt_get_event_tallies(ctx, 1, t.data(), n) != GXT_OK) {
+    std::fprintf(stderr, "transport: %s\n", gxt_ctx_error(ctx));
+    gxt_destroy(ctx);
+    return 1;
+  }
+  print_events(t);
+  gxt_destroy(ctx);
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s DATA_DIR block N | nav | basket FILE N | slab\n", argv[0]);
+    std::fprintf(stderr, "usage: %s DATA_DIR block N | nav | basket FILE N | slab | tree N\n", argv[0]);
     return 2;
   }
   if (!std::strcmp(argv[2], "block")) return run_block(argv[1], argc > 3 ? std::atoi(argv[3]) : 16);
   if (!std::strcmp(argv[2], "nav")) return run_nav();
   if (!std::strcmp(argv[2], "basket") && argc > 4) return run_basket(argv[1], argv[3], std::atoi(argv[4]));
   if (!std::strcmp(argv[2], "slab")) return run_slab(argv[1]);
+  if (!std::strcmp(argv[2], "tree")) return run_tree(argv[1], argc > 3 ? std::atoi(argv[3]) : 8);
   std::fprintf(stderr, "unknown mode %s\n", argv[2]);
   return 2;
 }

@@ -69,7 +69,13 @@ class Oracle:
         check(lib().gxref_get_event_tallies(self.ctx, t, n))
         return abi.tallies_to_numpy(t)
 
-    def profiles(self, nvol):
+    def volume_bins(self):
+        n = C.c_int()
+        check(lib().gxref_volume_bins(self.ctx, C.byref(n), None))
+        return n.value
+
+    def profiles(self, nvol=None):
+        nvol = self.volume_bins() if nvol is None else nvol
         ez = np.zeros(self.cfg.n_zbins)
         er = np.zeros(self.cfg.n_rbins)
         ev = np.zeros(nvol)

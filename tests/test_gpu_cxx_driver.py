@@ -107,3 +107,30 @@ def test_cxx_slab_distance_matches_oracle():
     for i, r in enumerate(rows):
         assert int(r[1]) == i and int(r[2]) == vol[i] and int(r[4]) == nxt[i] and int(r[6]) == b[i], (i, r)
         assert float(r[3]) == s[i] and float(r[5]) == saf[i], (i, r)
+
+
+def test_cxx_tree_transport_matches_oracle():
+    """The reference calorimeter handed to gxt_create as a volume tree from
+    C++ (gxt_box_volume / gxt_placement as ProxyDetectorConstruction builds
+    it): the device's events equal the oracle's, counts exact, energies 1e-9."""
+    n = 6
+    _flush("C++ driver: %d x 10 GeV e- through the reference calorimeter tree" % n)
+    out = _driver("tree", str(n))
+    assert out[0] == "bins 101"
+    rows = [line.split() for line in out if line.startswith("event")]
+    assert len(rows) == n
+    cfg = abi.config_reference_calorimeter(max_events=n)
+    p = abi.primaries_array(n)
+    for i in range(n):
+        p[i].species, p[i].event, p[i].ekin, p[i].substream = abi.ELECTRON, i, 10000.0, i
+        p[i].pos[:] = (0.5, 0.25, 0.75)
+        p[i].dir[:] = (2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0)
+    o = O.Oracle(cfg)
+    o.run(p, 16)
+    ev = o.event_tallies(n)
+    for i, r in enumerate(rows):
+        assert int(r[1]) == i
+        np.testing.assert_allclose([float(x) for x in r[2:5]], [ev["edep"][i], ev["visible"][i], ev["leak"][i]],
+                                   rtol=1e-9, atol=1e-9)
+        assert int(r[5]) == ev["n_steps"][i]
+        assert [int(x) for x in r[6:9]] == list(ev["n_secondaries"][i])
