@@ -273,6 +273,20 @@ int load_material(const std::string &data, int which, double lar_density, HostDa
       err = "cannot read brem_SB/br" + std::to_string(c.Z);
       return GXT_EDATA;
     }
+    // the kernels keep one copy of the (k / T, ln T) grid, which the
+    // Seltzer-Berger files share
+    DevParams &P = hd.params;
+    if (!hd.sb_grid) {
+      for (int k = 0; k < 32; ++k) P.sbx[k] = grid[k], P.sbxf[k] = (float)grid[k];
+      for (int k = 0; k < 57; ++k) P.sby[k] = grid[32 + k], P.sbyf[k] = (float)grid[32 + k];
+      hd.sb_grid = true;
+    } else {
+      for (int k = 0; k < 32 + 57; ++k)
+        if (grid[k] != (k < 32 ? P.sbx[k] : P.sby[k - 32])) {
+          err = "brem_SB/br" + std::to_string(c.Z) + ": its grid differs from the other elements'";
+          return GXT_EDATA;
+        }
+    }
     e.sandia = (int)hd.sandia.size();
     e.nint = (int)rows.size() / 5;
     hd.sandia.insert(hd.sandia.end(), rows.begin(), rows.end());

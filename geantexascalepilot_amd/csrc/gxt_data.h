@@ -16,6 +16,7 @@ struct HostData {
   std::vector<double> sx;      // merged Sandia intervals of the materials
   std::vector<double> shells;
   std::vector<int> hints;
+  bool sb_grid = false;        // params.sbx / sby set from the first element read
 };
 
 // Load material `which` (GXT_MAT_*) into params.mat[slot].

@@ -439,20 +439,55 @@ __device__ __forceinline__ int sb_bin(const R *a, int n, R z) {
   return hi;
 }
 
-// Proxy2DVector::Value (Proxy2DVector.cpp:37-67).
+template <class R> __device__ __forceinline__ const R *sb_xgrid(const DevParams *P);
+template <> __device__ __forceinline__ const double *sb_xgrid<double>(const DevParams *P) { return P->sbx; }
+template <> __device__ __forceinline__ const float *sb_xgrid<float>(const DevParams *P) { return P->sbxf; }
+template <class R> __device__ __forceinline__ const R *sb_ygrid(const DevParams *P);
+template <> __device__ __forceinline__ const double *sb_ygrid<double>(const DevParams *P) { return P->sby; }
+template <> __device__ __forceinline__ const float *sb_ygrid<float>(const DevParams *P) { return P->sbyf; }
+
+// Proxy2DVector::Value (Proxy2DVector.cpp:37-67), split in two: the part that
+// depends only on y (its bin and the two value rows), which a rejection loop
+// over x at fixed electron energy computes once, and the lookup at x.  The
+// grid comes from DevParams (an LDS copy in the step kernels); the values
+// from the element's table g (x[32], y[57], v[57][32]).
 template <class R>
-__device__ __forceinline__ R sb_value(const R *g, R x, R y) {
-  const R *gx = g, *gy = g + 32, *gv = g + 32 + 57;
-  if (x < gx[0]) x = gx[0];
-  else if (x > gx[31]) x = gx[31];
+struct SbRow {
+  const R *v1, *v2;  // value rows iy and iy + 1
+  R y, y1, y2;
+};
+
+template <class R>
+__device__ __forceinline__ SbRow<R> sb_row(const DevParams *P, const R *g, R y) {
+  const R *gy = sb_ygrid<R>(P);
   if (y < gy[0]) y = gy[0];
   else if (y > gy[56]) y = gy[56];
-  const int ix = sb_bin(gx, 32, x), iy = sb_bin(gy, 57, y);
-  const R x1 = gx[ix], x2 = gx[ix + 1], y1 = gy[iy], y2 = gy[iy + 1];
-  const R v11 = gv[iy * 32 + ix], v12 = gv[iy * 32 + ix + 1];
-  const R v21 = gv[(iy + 1) * 32 + ix], v22 = gv[(iy + 1) * 32 + ix + 1];
+  const int iy = sb_bin(gy, 57, y);
+  SbRow<R> r;
+  r.v1 = g + 32 + 57 + iy * 32;
+  r.v2 = r.v1 + 32;
+  r.y = y;
+  r.y1 = gy[iy];
+  r.y2 = gy[iy + 1];
+  return r;
+}
+
+template <class R>
+__device__ __forceinline__ R sb_at(const DevParams *P, const SbRow<R> &r, R x) {
+  const R *gx = sb_xgrid<R>(P);
+  if (x < gx[0]) x = gx[0];
+  else if (x > gx[31]) x = gx[31];
+  const int ix = sb_bin(gx, 32, x);
+  const R x1 = gx[ix], x2 = gx[ix + 1], y = r.y, y1 = r.y1, y2 = r.y2;
+  const R v11 = r.v1[ix], v12 = r.v1[ix + 1];
+  const R v21 = r.v2[ix], v22 = r.v2[ix + 1];
   return ((y2 - y) * (v11 * (x2 - x) + v12 * (x - x1)) + ((y - y1) * (v21 * (x2 - x) + v22 * (x - x1)))) /
          ((x2 - x1) * (y2 - y1));
+}
+
+template <class R>
+__device__ __forceinline__ R sb_value(const DevParams *P, const R *g, R x, R y) {
+  return sb_at(P, sb_row(P, g, y), x);
 }
 
 // Merged Sandia interval of material m at ek (keV): j with B[j-1] <= ek <
@@ -753,7 +788,7 @@ __device__ __forceinline__ void sample_photo(const DevParams *P, const DevMateri
   const DevElement &el = m.el[ie];
   R bind = R(0);
   for (int i = 0; i < el.nshell; ++i) {
-    const R b = (R)P->shells[el.shell + i];
+    const R b = (R)P->shell_e[el.shell + i];
     if (e0 >= b) {
       bind = b;
       break;
@@ -809,10 +844,11 @@ __device__ __forceinline__ void sample_brems(const DevParams *P, const DevMateri
   const R xmax = m_log(emax * emax + dcorr);
   const R y = m_log(t0);
   const R x0 = cut / t0;
-  R vmax = sb_value(sb, x0, y) * R(1.02);
+  const SbRow<R> row = sb_row(P, sb, y);
+  R vmax = sb_at(P, row, x0) * R(1.02);
   const R epeak = R(300.0), elow = R(10.0e-3);
   if (x0 < R(0.97) && (t0 > epeak || t0 < elow)) {
-    const R ylim = rmin(sb_value(sb, R(0.97), R(4.0) * m_log(R(10.0))), R(1.1) * sb_value(sb, R(0.97), y));
+    const R ylim = rmin(sb_value(P, sb, R(0.97), R(4.0) * m_log(R(10.0))), R(1.1) * sb_at(P, row, R(0.97)));
     if (ylim > vmax) vmax = ylim;
   }
   if (x0 < R(0.05)) vmax *= R(1.2);
@@ -822,7 +858,7 @@ __device__ __forceinline__ void sample_brems(const DevParams *P, const DevMateri
     R x = m_exp(xmin + r * (xmax - xmin)) - dcorr;
     if (x < R(0)) x = R(0);
     k = m_sqrt(x);
-    v = sb_value(sb, k / t0, y);
+    v = sb_at(P, row, k / t0);
   } while (v < vmax * rng.template u<R>());
   const R u0 = rng.template u<R>();
   const R c = R(4.0) - R(8.0) * u0;

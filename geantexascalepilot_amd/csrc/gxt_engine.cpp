@@ -128,7 +128,7 @@ struct DeviceData {
   DevParams *P = nullptr;
   Node64 *nd = nullptr;
   Node32 *nf = nullptr;
-  double *sbd = nullptr, *sandia = nullptr, *shells = nullptr, *sx = nullptr;
+  double *sbd = nullptr, *sandia = nullptr, *sx = nullptr;
   float *sbf = nullptr;
   uint32_t *jump8 = nullptr;
   int *hints = nullptr;
@@ -141,7 +141,6 @@ struct DeviceData {
     (void)hipFree(sbd);
     (void)hipFree(sandia);
     (void)hipFree(sx);
-    (void)hipFree(shells);
     (void)hipFree(sbf);
     *this = DeviceData();
   }
@@ -171,7 +170,6 @@ int upload(HostData &hd, DeviceData &dd) {
   HIP_TRY(put(&dd.sbf, tof(hd.sb)));
   HIP_TRY(put(&dd.sandia, hd.sandia));
   HIP_TRY(put(&dd.sx, hd.sx));
-  HIP_TRY(put(&dd.shells, hd.shells));
   HIP_TRY(put(&dd.hints, hd.hints));
   DevParams &p = hd.params;
   p.hints = dd.hints;
@@ -182,6 +180,8 @@ int upload(HostData &hd, DeviceData &dd) {
   HIP_TRY(put(&dd.jump8, j8));
   p.nsandia = (int)hd.sandia.size();
   p.nshells = (int)hd.shells.size();
+  if (hd.shells.size() > (size_t)kMaxShells) return set_err(GXT_EDATA, "more shell energies than DevParams holds");
+  for (size_t i = 0; i < hd.shells.size(); ++i) p.shell_e[i] = hd.shells[i];
   p.jump8 = dd.jump8;
   p.nd = dd.nd;
   p.nf = dd.nf;
@@ -189,7 +189,6 @@ int upload(HostData &hd, DeviceData &dd) {
   p.sbf = dd.sbf;
   p.sandia = dd.sandia;
   p.sx = dd.sx;
-  p.shells = dd.shells;
   HIP_TRY(dalloc(&dd.P, 1));
   HIP_TRY(hipMemcpy(dd.P, &p, sizeof(p), hipMemcpyHostToDevice));
   return GXT_OK;

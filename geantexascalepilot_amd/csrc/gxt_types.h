@@ -17,6 +17,7 @@ constexpr int kNumQueues = 9;     // (species, process) interaction queues + pos
 constexpr int kQueueAtRest = 8;
 constexpr int kSpecBins = 64;     // secondary spectrum bins per species
 constexpr int kInvRangeHints = 256;  // search-hint bins per inverse-range vector
+constexpr int kMaxShells = 128;   // shell binding energies over the elements of the slots
 // The pool, the interaction queues and the new-track list are split into
 // kStripes stripes of cap / kStripes slots, each with its own counters;
 // workgroup b appends to stripe b % kStripes, so each counter takes 1/kStripes
@@ -188,8 +189,15 @@ struct DevParams {
   const float *sbf;             // FP32 mirror
   const double *sandia;         // Sandia rows (edge keV, a1..a4)
   const double *sx;             // merged Sandia breakpoints and coefficients (DevMaterial::sx_*)
-  const double *shells;         // binding energies (MeV)
   const int *hints;             // search hints of the inverse-range vectors
+  // the Seltzer-Berger grid every element shares: reduced photon energy k / T
+  // (32 nodes) and ln(T / MeV) (57 nodes), FP64 and FP32; the step kernels
+  // read them from their LDS copy of DevParams
+  double sbx[32], sby[57];
+  float sbxf[32], sbyf[57];
+  // shell binding energies (MeV) of every element of the slots, innermost
+  // first (DevElement::shell, nshell), read where DevParams is
+  double shell_e[kMaxShells];
   // LDS copies of the physics vectors a transport kernel reads: layout 0 the
   // photon tables (GXT_TAB_LAMBDA_COMPT .. GXT_TAB_LAMBDAPRIM_PHOT), layout 1
   // the e-/e+ tables (GXT_TAB_LAMBDA_BREM_EM .. GXT_TAB_INVRANGE_EP), one
