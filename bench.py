@@ -317,7 +317,9 @@ def serving(g, abi, seconds, rate_jobs_s, mix="c4", warm_s=2.0, drain_s=60.0, se
             now = time.perf_counter() - t_start
             if now >= seconds:
                 break
-            if now >= next_sub:
+            # every job whose arrival time has passed is submitted now, so the
+            # arrivals keep the offered rate when a pass of this loop is slow
+            while now >= next_sub:
                 u, acc = rs.random(), 0.0
                 for kind, p, n, sp, e in MIXES[mix]:
                     acc += p
@@ -329,7 +331,7 @@ def serving(g, abi, seconds, rate_jobs_s, mix="c4", warm_s=2.0, drain_s=60.0, se
                 kinds.append(kind)
                 sub_t.append(now)
                 submitted += 1
-                next_sub = now + rs.exponential(1.0 / rate_jobs_s)
+                next_sub += rs.exponential(1.0 / rate_jobs_s)
             for r in t.poll():
                 done.append((time.perf_counter() - t_start, r))
             if now >= next_log:

@@ -547,15 +547,8 @@ __device__ __forceinline__ NodesT<R, kLds> stage_tables(const DevParams *P, int 
 template <class R, bool kCharged, int kMode = 0, bool kLds = false>
 __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAVES) void k_transport(StepArgs a,
                                                                                                PoolArgs<R> pa) {
-  __shared__ WaveHist whs[kBlock / 64];
-  __shared__ DevParams sP;
-  WaveHist &wh = whs[threadIdx.x >> 6];
-  wh_clear(wh);
-  const NodesT<R, kLds> N = stage_tables<R, kLds>(a.P, kCharged ? 1 : 0);
-  const DevParams *P = stage_params(a.P, sP);
   const Pool<R> &cur = pa.cur;
   const Pool<R> &nxt = pa.nxt;
-  const bool trace = P->trace_events > 0;
   // counts are clamped to the stripes so a corrupted counter cannot index past them
   const int64_t sc = stripe_cap(cur);
   const int st = my_stripe();
@@ -573,6 +566,16 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
   // injected primaries can fill a stripe from both ends: report, never overlap
   if (bad && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.cnt->overflow, 1);
   const int total = span.total();
+  // a workgroup without tracks (the grid is fixed, the count is on the
+  // device) leaves before it stages anything
+  if ((int64_t)blockIdx.x * blockDim.x >= total) return;
+  __shared__ WaveHist whs[kBlock / 64];
+  __shared__ DevParams sP;
+  WaveHist &wh = whs[threadIdx.x >> 6];
+  wh_clear(wh);
+  const NodesT<R, kLds> N = stage_tables<R, kLds>(a.P, kCharged ? 1 : 0);
+  const DevParams *P = stage_params(a.P, sP);
+  const bool trace = P->trace_events > 0;
   const R mine = (R)P->tracking_cut;
   constexpr int kG = kMode == 4 ? 1 : 0;  // geo_cell form
   for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < total; base += gridDim.x * blockDim.x) {
@@ -783,14 +786,8 @@ __device__ __forceinline__ int sec_fate(const DevParams *P, TrackRegs<R> &t, Dep
 // 3 and keeps 2.
 template <class R, bool kCharged>
 __global__ __launch_bounds__(kBlock, sizeof(R) == 8 ? 3 : GXT_STEP_WAVES) void k_interact(StepArgs a, PoolArgs<R> pa) {
-  __shared__ WaveHist whs[kBlock / 64];
-  WaveHist &wh = whs[threadIdx.x >> 6];
-  wh_clear(wh);
-  __shared__ DevParams sP;
-  const DevParams *P = stage_params(a.P, sP);
   const Pool<R> &cur = pa.cur;
   const Pool<R> &nxt = pa.nxt;
-  const bool trace = P->trace_events > 0;
   const int64_t sc = stripe_cap(cur);
   const int st = my_stripe();
   // photon queues 0-2 in one variant, e-/e+ queues 3-8 in the other; the
@@ -809,6 +806,14 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 8 ? 3 : GXT_STEP_WAVES) void k
   }
   __syncthreads();
   const int total = seg[kNQ * kStripes];
+  // a workgroup without items leaves before it stages anything
+  if ((int64_t)blockIdx.x * blockDim.x >= total) return;
+  __shared__ WaveHist whs[kBlock / 64];
+  WaveHist &wh = whs[threadIdx.x >> 6];
+  wh_clear(wh);
+  __shared__ DevParams sP;
+  const DevParams *P = stage_params(a.P, sP);
+  const bool trace = P->trace_events > 0;
   const R mine = (R)P->tracking_cut;
   // this variant's new-track list
   a.newlist += kCharged ? cur.cap : 0;
