@@ -370,7 +370,7 @@ def serving(g, abi, seconds, rate_jobs_s, mix="c4", warm_s=2.0, drain_s=60.0, se
                 single_event_iterations_p50=float(np.percentile(single_its, 50)) if len(single_its) else None)
 
 
-def serving_sweep(g, abi, capacity_gev_s, mix, loads, seconds):
+def serving_sweep(g, abi, capacity_gev_s, mix, loads, seconds, drain_s=15.0):
     """The serving loop at offered loads given as fractions of capacity_gev_s
     (GeV/s of primaries); the sustained capacity is the highest completion
     rate among the loads the loop kept up with (backlog at the window's end
@@ -378,7 +378,7 @@ def serving_sweep(g, abi, capacity_gev_s, mix, loads, seconds):
     runs = []
     for f in loads:
         rate = f * capacity_gev_s / mix_gev_per_job(mix)
-        r = serving(g, abi, seconds, rate, mix=mix)
+        r = serving(g, abi, seconds, rate, mix=mix, drain_s=drain_s if len(loads) > 1 else 60.0)
         r["load"] = f
         runs.append(r)
         log("serving %s at %.2f x: %.0f events/s, backlog %d of %d, p99 %.0f ms"
@@ -481,7 +481,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--events", type=int, default=10000)
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="seconds of submission per offered load")
-    ap.add_argument("--serve-loads", default="0.5,1.0,1.5,2.0,2.5",
+    ap.add_argument("--serve-loads", default="0.6,0.9,1.1,1.4,1.8",
                     help="offered loads of the serving sweep, as fractions of the FP32 batch's GeV/s")
     ap.add_argument("--serve-mix", default="c4,e10", help="job mixes of the serving sweep (c4, e10)")
     ap.add_argument("--serve-only", action="store_true",
