@@ -513,17 +513,55 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
 // lookup of the launch is served.
 extern __shared__ uint4 tab_lds[];
 
-// The workgroup's copy of DevParams in LDS (8-byte words; every step
-// kernel's threads take part, then wait at a barrier).
-__device__ __forceinline__ const DevParams *stage_params(const DevParams *P, DevParams &sP) {
+// The workgroup's copy of DevParams in LDS, in 8-byte words: every thread
+// of the (kBlock-thread) workgroup issues all its loads before its stores, so
+// the copy costs one memory latency rather than one per pass of a strided
+// loop.  params_load, then params_store, then a barrier (after any other
+// staging, so that one barrier serves all).
+constexpr int kParamWords = (int)(sizeof(DevParams) / 8);
+constexpr int kParamPer = (kParamWords + kBlock - 1) / kBlock;
+struct ParamWords {
+  unsigned long long v[kParamPer];
+};
+
+__device__ __forceinline__ void params_load(const DevParams *P, ParamWords &w) {
   static_assert(sizeof(DevParams) % 8 == 0, "DevParams is copied in 8-byte words");
   const unsigned long long *src = reinterpret_cast<const unsigned long long *>(P);
+#pragma unroll
+  for (int j = 0; j < kParamPer; ++j) {
+    const int k = j * kBlock + (int)threadIdx.x;
+    if (k < kParamWords) w.v[j] = src[k];
+  }
+}
+
+__device__ __forceinline__ const DevParams *params_store(DevParams &sP, const ParamWords &w) {
   unsigned long long *dst = reinterpret_cast<unsigned long long *>(&sP);
-  for (int k = threadIdx.x; k < (int)(sizeof(DevParams) / 8); k += blockDim.x) dst[k] = src[k];
-  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kParamPer; ++j) {
+    const int k = j * kBlock + (int)threadIdx.x;
+    if (k < kParamWords) dst[k] = w.v[j];
+  }
   return &sP;
 }
 
+// n 16-byte words from src to LDS dst by the workgroup's threads, four loads
+// in flight per thread before their stores.
+__device__ __forceinline__ void copy_words16(uint4 *dst, const uint4 *src, int n) {
+  for (int k = (int)threadIdx.x; k < n; k += 4 * kBlock) {
+    const int k1 = k + kBlock, k2 = k + 2 * kBlock, k3 = k + 3 * kBlock;
+    const uint4 v0 = src[k];
+    const uint4 v1 = k1 < n ? src[k1] : v0;
+    const uint4 v2 = k2 < n ? src[k2] : v0;
+    const uint4 v3 = k3 < n ? src[k3] : v0;
+    dst[k] = v0;
+    if (k1 < n) dst[k1] = v1;
+    if (k2 < n) dst[k2] = v2;
+    if (k3 < n) dst[k3] = v3;
+  }
+}
+
+// The transport's species tables into dynamic LDS (no barrier: the caller's
+// barrier after params_store covers both).
 template <class R, bool kLds>
 __device__ __forceinline__ NodesT<R, kLds> stage_tables(const DevParams *P, int layout) {
   using T = typename NodeType<R>::T;
@@ -533,11 +571,9 @@ __device__ __forceinline__ NodesT<R, kLds> stage_tables(const DevParams *P, int 
     int dst = 0;
     for (int s = 0; s < kMaxSlots; ++s) {
       const int n16 = P->lds_cnt[layout][s] * w;
-      const uint4 *sp = src + (int64_t)P->lds_src[layout][s] * w;
-      for (int k = threadIdx.x; k < n16; k += blockDim.x) tab_lds[dst + k] = sp[k];
+      copy_words16(tab_lds + dst, src + (int64_t)P->lds_src[layout][s] * w, n16);
       dst += n16;
     }
-    __syncthreads();
     return NodesT<R, true>(reinterpret_cast<const T *>(tab_lds), P->hints);
   } else {
     return NodesT<R, false>(P);
@@ -573,8 +609,11 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
   __shared__ DevParams sP;
   WaveHist &wh = whs[threadIdx.x >> 6];
   wh_clear(wh);
+  ParamWords pw;
+  params_load(a.P, pw);
   const NodesT<R, kLds> N = stage_tables<R, kLds>(a.P, kCharged ? 1 : 0);
-  const DevParams *P = stage_params(a.P, sP);
+  const DevParams *P = params_store(sP, pw);
+  __syncthreads();
   const bool trace = P->trace_events > 0;
   const R mine = (R)P->tracking_cut;
   constexpr int kG = kMode == 4 ? 1 : 0;  // geo_cell form
@@ -812,7 +851,10 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 8 ? 3 : GXT_STEP_WAVES) void k
   WaveHist &wh = whs[threadIdx.x >> 6];
   wh_clear(wh);
   __shared__ DevParams sP;
-  const DevParams *P = stage_params(a.P, sP);
+  ParamWords pw;
+  params_load(a.P, pw);
+  const DevParams *P = params_store(sP, pw);
+  __syncthreads();
   const bool trace = P->trace_events > 0;
   const R mine = (R)P->tracking_cut;
   // this variant's new-track list
