@@ -302,7 +302,9 @@ int gxt_poll(gxt_ctx *ctx, gxt_job_result *out, int max, int *n_out);
 int gxt_get_profiles(gxt_ctx *ctx, double *edep_z, double *edep_r, double *edep_volume);
 /* Volume bins of gxt_get_profiles: 1 for the box, 2*n_layers for the
  * calorimeter, the tree form's cells (touchables with a material, then
- * vacuum).  cell (optional, one entry per touchable): each touchable's bin. */
+ * vacuum).  cell (optional, one entry per touchable): each touchable's bin.
+ * No counterpart in the reference, whose scoring is a placeholder
+ * (proxy/test/test_BasicCpuTransport.cpp:479-483). */
 int gxt_volume_bins(gxt_ctx *ctx, int *n_bins, int32_t *cell);
 int gxt_get_event_tallies(gxt_ctx *ctx, uint64_t job_id, gxt_event_tally *out, int n);
 /* Energy histogram of created secondaries: 3 species x 64 bins of log10(E/MeV)
