@@ -15,6 +15,6 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def gpu_lib():
-    """The product library; on a GPU box a missing library is a failure, not a skip."""
+    """The product library; where a GPU is present a missing library is a failure, not a skip."""
     import geantexascalepilot_amd.gxt as g
     return g
