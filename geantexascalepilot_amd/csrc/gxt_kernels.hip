@@ -990,9 +990,10 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 8 ? 3 : GXT_STEP_WAVES) void k
 }
 
 // RNG state of the tracks created this iteration: substream uid of the seed,
-// one track per lane (the jump is 16 table-driven matrix-vector products, so
-// it runs here, without divergence and at full occupancy, rather than in the
-// interaction kernels).
+// one MRG32k3a component per lane, two lanes per track (the jump is 8
+// table-driven matrix-vector products per component, so it runs here,
+// without divergence and at full occupancy, rather than in the interaction
+// kernels).
 template <class R, int kList>
 __global__ __launch_bounds__(kBlock) void k_derive(StepArgs a, PoolArgs<R> pa) {
   const Pool<R> &nxt = pa.nxt;
@@ -1004,15 +1005,28 @@ __global__ __launch_bounds__(kBlock) void k_derive(StepArgs a, PoolArgs<R> pa) {
   for (int k = 0; k < kStripes; ++k)
     span.pre[k + 1] = span.pre[k] + (int)min((int64_t)max(a.cnt->ctr(kNew0 + kList, k), 0), sc);
   const int n = span.total();
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  // two adjacent lanes of a wave per track, one MRG32k3a component each;
+  // both read the substream index (RNG words 1 and 2) before either writes
+  // its words: the loads complete before the products that use them
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < 2 * n; j += gridDim.x * blockDim.x) {
+    const int i = j >> 1, c = j & 1;
     int is, off;
     span.locate(i, is, off);
     const int64_t s = a.newlist[(int64_t)is * sc + off];
     if (s < 0 || s >= nxt.cap) continue;
     const uint64_t uid = (uint64_t)nxt.rng[nxt.cap + s] | ((uint64_t)nxt.rng[2 * nxt.cap + s] << 32);
-    Rng r;
-    substream_state(a.P, uid, r.s);
-    r.store(nxt.rng, nxt.cap, s);
+    const uint32_t m = c ? 4294944443u : 4294967087u;
+    const double invm = c ? 1.0 / 4294944443.0 : 1.0 / 4294967087.0;
+    uint32_t v[3] = {a.P->seed[3 * c], a.P->seed[3 * c + 1], a.P->seed[3 * c + 2]};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      uint32_t M[9];
+      jump_row(a.P->jump8, c, k, (int)((uid >> (8 * k)) & 255u), M);
+      matvec_mod(M, v, m, invm);
+    }
+    nxt.rng[(int64_t)(3 * c) * nxt.cap + s] = v[0];
+    nxt.rng[(int64_t)(3 * c + 1) * nxt.cap + s] = v[1];
+    nxt.rng[(int64_t)(3 * c + 2) * nxt.cap + s] = v[2];
   }
 }
 
