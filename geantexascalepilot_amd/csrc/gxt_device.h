@@ -379,13 +379,13 @@ __device__ __forceinline__ R spline(const V &lo, const V &hi, R e) {
 
 // ProxyPhysicsVector::Value on a log-binned vector (ProxyPhysicsVector.cu:388-394,469-483).
 template <class R, class NS>
-__device__ __forceinline__ R pvec_value(const NS &N, const DevTable &t, R e) {
+__device__ __forceinline__ R pvec_value(const NS &N, const DevTable &t, R e, R le) {
   const int n = t.n;
   R x0, y0, xn, yn;
   tends<R>(t, x0, y0, xn, yn);
   if (e <= x0) return y0;
   if (e >= xn) return yn;
-  int bin = (int)(m_log(e) / tdl<R>(t) - tbase<R>(t));
+  int bin = (int)(le / tdl<R>(t) - tbase<R>(t));
   if (bin > n - 2) bin = n - 2;
   if (bin < 0) bin = 0;
   auto lo = N.at(t, bin);
@@ -395,6 +395,17 @@ __device__ __forceinline__ R pvec_value(const NS &N, const DevTable &t, R e) {
     lo = N.at(t, bin - 1);
   }
   return spline<R>(lo, hi, e);
+}
+
+// The same with log(e) computed here (le: m_log(e), which a step's lookups
+// at one energy share).
+template <class R, class NS>
+__device__ __forceinline__ R pvec_value(const NS &N, const DevTable &t, R e) {
+  R x0, y0, xn, yn;
+  tends<R>(t, x0, y0, xn, yn);
+  if (e <= x0) return y0;
+  if (e >= xn) return yn;
+  return pvec_value(N, t, e, m_log(e));
 }
 
 // InvRangeValue (ProxyPhysicsVector.cu:491-524): binary search (the reference calls the
@@ -543,20 +554,25 @@ __device__ __forceinline__ R heitler_xs(R ekin) {
 // (ProxyEmProcess::GetLambda and the processes' GetLambda) with the track's
 // couple, Compton above 1 MeV and photoelectric from LambdaPrim/E.
 template <class R, class NS>
-__device__ __forceinline__ R macro_xs(const DevParams *P, const NS &N, const DevMaterial &m, int species, int p, R e) {
+__device__ __forceinline__ R macro_xs(const DevParams *P, const NS &N, const DevMaterial &m, int species, int p, R e,
+                                      R le) {
   if (species == 0) {
     if (p == 0) {
-      if ((double)e < m.compt_split) return pvec_value(N, m.tab[GXT_TAB_LAMBDA_COMPT], e);
-      return pvec_value(N, m.tab[GXT_TAB_LAMBDAPRIM_COMPT], e) / e;
+      if ((double)e < m.compt_split) return pvec_value(N, m.tab[GXT_TAB_LAMBDA_COMPT], e, le);
+      return pvec_value(N, m.tab[GXT_TAB_LAMBDAPRIM_COMPT], e, le) / e;
     }
-    if (p == 1) return pvec_value(N, m.tab[GXT_TAB_LAMBDA_CONV], e);
-    if ((double)e >= m.phot_emin) return pvec_value(N, m.tab[GXT_TAB_LAMBDAPRIM_PHOT], e) / e;
+    if (p == 1) return pvec_value(N, m.tab[GXT_TAB_LAMBDA_CONV], e, le);
+    if ((double)e >= m.phot_emin) return pvec_value(N, m.tab[GXT_TAB_LAMBDAPRIM_PHOT], e, le) / e;
     return (R)sandia_macro(P, m, (double)e);
   }
   const bool pos = species == 2;
-  if (p == 0) return pvec_value(N, m.tab[pos ? GXT_TAB_LAMBDA_BREM_EP : GXT_TAB_LAMBDA_BREM_EM], e);
-  if (p == 1) return pvec_value(N, m.tab[pos ? GXT_TAB_LAMBDA_IONI_EP : GXT_TAB_LAMBDA_IONI_EM], e);
+  if (p == 0) return pvec_value(N, m.tab[pos ? GXT_TAB_LAMBDA_BREM_EP : GXT_TAB_LAMBDA_BREM_EM], e, le);
+  if (p == 1) return pvec_value(N, m.tab[pos ? GXT_TAB_LAMBDA_IONI_EP : GXT_TAB_LAMBDA_IONI_EM], e, le);
   return R(m.ne) * heitler_xs(e);
+}
+template <class R, class NS>
+__device__ __forceinline__ R macro_xs(const DevParams *P, const NS &N, const DevMaterial &m, int species, int p, R e) {
+  return macro_xs(P, N, m, species, p, e, m_log(e));
 }
 
 __device__ __forceinline__ int nproc(int species) { return species == 1 ? 2 : 3; }
@@ -1073,8 +1089,9 @@ __device__ __forceinline__ int queue_model(int q) {
 // ProxyIonization::AlongStepGPIL (ProxyIonization.hpp:108-128): production-cut
 // length for finR and Geant4's (2 - finR/range) form.
 template <class R, class NS>
-__device__ __forceinline__ R along_limit(const NS &N, const DevMaterial &m, bool pos, R e, R mine) {
-  R range = pvec_value(N, m.tab[pos ? GXT_TAB_RANGE_EP : GXT_TAB_RANGE_EM], e);
+__device__ __forceinline__ R along_limit(const NS &N, const DevMaterial &m, bool pos, R e, R le, R mine, R &range0) {
+  R range = pvec_value(N, m.tab[pos ? GXT_TAB_RANGE_EP : GXT_TAB_RANGE_EM], e, le);
+  range0 = range;
   if (e < mine) range *= e / mine;
   const R finR = rmin(R(10.0), (R)m.cut_length);
   return range > finR ? range * R(0.2) + finR * R(0.8) * (R(2.0) - finR / range) : range;
@@ -1083,14 +1100,13 @@ __device__ __forceinline__ R along_limit(const NS &N, const DevMaterial &m, bool
 // ProxyIonization::AlongStepDoIt (ProxyIonization.hpp:130-168): binary-search
 // inverse range.
 template <class R, class NS>
-__device__ __forceinline__ R along_doit(const NS &N, const DevMaterial &m, bool pos, R e, R step, R mine,
-                                        R &eloss) {
-  const R range = pvec_value(N, m.tab[pos ? GXT_TAB_RANGE_EP : GXT_TAB_RANGE_EM], e);
+__device__ __forceinline__ R along_doit(const NS &N, const DevMaterial &m, bool pos, R e, R le, R range, R step,
+                                        R mine, R &eloss) {
   if (step >= range || e <= mine) {
     eloss = e;
     return R(0);
   }
-  const R dedx = pvec_value(N, m.tab[pos ? GXT_TAB_DEDX_EP : GXT_TAB_DEDX_EM], e);
+  const R dedx = pvec_value(N, m.tab[pos ? GXT_TAB_DEDX_EP : GXT_TAB_DEDX_EM], e, le);
   R loss = step * dedx;
   if (loss > e * R(0.01))
     loss = e - pvec_inv_value(N, m.tab[pos ? GXT_TAB_INVRANGE_EP : GXT_TAB_INVRANGE_EM], range - step);
@@ -1542,10 +1558,10 @@ constexpr double kMscFacRange = 0.04, kMscFacSafety = 0.6, kMscTmin = 1.0e-5;
 constexpr double kMscNone = 1.0e30;  // lambda1 where a material has no msc table
 
 template <class R, class NS>
-__device__ __forceinline__ double msc_lambda1(const NS &N, const DevMaterial &m, bool pos, R e) {
+__device__ __forceinline__ double msc_lambda1(const NS &N, const DevMaterial &m, bool pos, R e, R le) {
   const DevTable &t = m.tab[pos ? GXT_TAB_MSC_EP : GXT_TAB_MSC_EM];
   if (t.n == 0) return kMscNone;
-  const R v = pvec_value(N, t, e);
+  const R v = pvec_value(N, t, e, le);
   return v > R(0) ? (double)(e * e / v) : kMscNone;
 }
 

@@ -407,8 +407,8 @@ __device__ __forceinline__ void emit_at_rest(const StepArgs &a, const Pool<R> &n
 // number of interaction lengths when exhausted, propose nil / lambda.
 template <int p, class R, class NS>
 __device__ __forceinline__ void gpil(const DevParams *P, const NS &N, const DevMaterial &m, int species, R e,
-                                     R &nil, R &xs, Rng &rng, double &bestd, int &sel) {
-  xs = macro_xs(P, N, m, species, p, e);
+                                     R le, R &nil, R &xs, Rng &rng, double &bestd, int &sel) {
+  xs = macro_xs(P, N, m, species, p, e, le);
   if (nil <= R(0)) nil = -m_log(rng.template u<R>());
   const double sp = xs > R(0) ? (double)(nil / xs) : kHuge;
   if (sp < bestd) {
@@ -650,12 +650,15 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
       double bestd = kHuge;
       int sel = -1;
       double lam1 = kMscNone;
+      // the step's table lookups share log(E) and the e-/e+ range at E
+      const R le = m_log(t.e);
+      R range = R(0);
       if (!vac) {
-      gpil<0>(P, N, m, sp, t.e, t.nil0, xs0, t.rng, bestd, sel);
-      gpil<1>(P, N, m, sp, t.e, t.nil1, xs1, t.rng, bestd, sel);
-      if (sp != 1) gpil<2>(P, N, m, sp, t.e, t.nil2, xs2, t.rng, bestd, sel);
+      gpil<0>(P, N, m, sp, t.e, le, t.nil0, xs0, t.rng, bestd, sel);
+      gpil<1>(P, N, m, sp, t.e, le, t.nil1, xs1, t.rng, bestd, sel);
+      if (sp != 1) gpil<2>(P, N, m, sp, t.e, le, t.nil2, xs2, t.rng, bestd, sel);
       if (charged) {
-        const double al = (double)along_limit(N, m, sp == 2, t.e, mine);
+        const double al = (double)along_limit(N, m, sp == 2, t.e, le, mine, range);
         if (al < bestd) {
           bestd = al;
           sel = -1;
@@ -666,11 +669,10 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
       // lengths and loss)
       if constexpr (kMode != 0) {
         if (charged && P->msc) {
-          lam1 = msc_lambda1<R>(N, m, sp == 2, t.e);
+          lam1 = msc_lambda1<R>(N, m, sp == 2, t.e, le);
           if (lam1 < kMscNone) {
-            const double range = (double)pvec_value(N, m.tab[sp == 2 ? GXT_TAB_RANGE_EP : GXT_TAB_RANGE_EM], t.e);
             const double saf = kMode == 4 ? nav_safety(P->geo.nav, t.vol, t.pos) : safety(P->geo, t.vol, t.pos);
-            const double tlim = fmax(fmax(kMscFacRange * fmax(range, lam1), kMscFacSafety * saf), kMscTmin);
+            const double tlim = fmax(fmax(kMscFacRange * fmax((double)range, lam1), kMscFacSafety * saf), kMscTmin);
             if (tlim < bestd) {
               bestd = tlim;
               sel = -1;
@@ -706,7 +708,7 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
       if (charged && !vac) {
         t.alen += step;
         R loss = R(0);
-        t.e = along_doit(N, m, t.species == 2, t.e, (R)step, mine, loss);
+        t.e = along_doit(N, m, t.species == 2, t.e, le, range, (R)step, mine, loss);
         const double mid[3] = {0.5 * (pre[0] + t.pos[0]), 0.5 * (pre[1] + t.pos[1]), 0.5 * (pre[2] + t.pos[2])};
         dA.add<kG>(P, t, (double)loss, mid, prevol);
         edep += (double)loss;
