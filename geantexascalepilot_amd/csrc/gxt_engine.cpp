@@ -280,8 +280,9 @@ struct gxt_ctx {
   Counters *hcnt = nullptr;  // pinned
   DevEvent *ev = nullptr;
   int *ev_live = nullptr;    // live tracks of event slot e at ev_live[e * kLiveStride]
-  int *ev_live_packed = nullptr;  // ev_live[e * kLiveStride] at [e], for the host's snapshot
-  int *hev_live = nullptr;   // pinned mirror
+  int *hev_live = nullptr;   // pinned: ev_live[e * kLiveStride] at [e], the host's snapshot
+  Counters *dhcnt = nullptr;  // device addresses of hcnt and hev_live
+  int *dhev_live = nullptr;
   double *rows = nullptr;
   unsigned long long *spec = nullptr;
   int64_t row_stride = 0;
@@ -290,6 +291,8 @@ struct gxt_ctx {
   hipEvent_t snap = nullptr;      // recorded after a round's counter snapshot
   size_t ev_snap = 0;             // profiling entries recorded before that snapshot
   int64_t snap_iter = 0;          // step-loop iterations launched before that snapshot
+  int gen = 0;                    // counter generation of the next iteration's current pool
+  int snap_gen = 0;               // the same when the snapshot was queued
   // non-uniform field
   gxt_field_map fmap;
   std::string fmap_path;
@@ -345,6 +348,7 @@ struct gxt_ctx {
     a.trace_cap = trace_cap;
     a.transport_mode = transport_mode;
     a.inner_steps = inner_steps;
+    a.gen = gen;
     // the transports stage their tables in LDS when they take at most
     // kLdsTablesMax bytes
     const int nb = precision == GXT_FP32 ? (int)sizeof(Node32) : (int)sizeof(Node64);
@@ -373,9 +377,10 @@ struct gxt_ctx {
 
 namespace {
 
-int64_t live_tracks(const Counters &h) {
+// Live tracks of the pool of counter generation g.
+int64_t live_tracks(const Counters &h, int g) {
   int64_t n = 0;
-  for (int s = 0; s < kStripes; ++s) n += (int64_t)h.ctr(kCur0, s) + h.ctr(kCur1, s);
+  for (int s = 0; s < kStripes; ++s) n += (int64_t)h.ctr(g, kPool0, s) + h.ctr(g, kPool1, s);
   return n;
 }
 
@@ -491,16 +496,22 @@ int admit(gxt_ctx *c) {
 }
 
 int iterate(gxt_ctx *c, int k) {
-  const StepArgs a = c->args();
+  StepArgs a = c->args();
+  // the e-/e+ chain follows what main holds so far (injections, cleared
+  // event slots; a snapshot, which must see no iteration in progress)
+  hipError_t se = launch_side_sync(c->ss);
+  if (se != hipSuccess) return c->fail(GXT_EHIP, std::string("step: ") + hipGetErrorString(se));
   for (int i = 0; i < k; ++i) {
+    a.gen = c->gen;
     hipError_t e;
     hipEvent_t *evs = nullptr;
     if (c->prof) {
       if (c->ev_free.empty()) {
-        // level 2 records only the events around the e-/e+ transport (4, 5)
+        // level 2 times only the e-/e+ transport (kernel 3: events 6, 7);
+        // the events only time kernels, so they need no system-scope fence
         std::array<hipEvent_t, kStepEvents> q{};
         for (int k = 0; k < kStepEvents; ++k)
-          if (c->prof_level == 1 || k == 4 || k == 5) HIP_TRY(hipEventCreate(&q[k]));
+          if (c->prof_level == 1 || k / 2 == 3) HIP_TRY(hipEventCreateWithFlags(&q[k], hipEventDisableSystemFence));
         c->ev_free.push_back(q);
       }
       c->ev_pending.push_back(c->ev_free.back());
@@ -516,6 +527,7 @@ int iterate(gxt_ctx *c, int k) {
     }
     if (e != hipSuccess) return c->fail(GXT_EHIP, std::string("step: ") + hipGetErrorString(e));
     c->parity = 1 - c->parity;
+    c->gen = gen_next(c->gen);
     ++c->iterations;
   }
   return GXT_OK;
@@ -526,15 +538,15 @@ int iterate(gxt_ctx *c, int k) {
 int collect(gxt_ctx *c, bool &idle) {
   for (size_t qi = 0; qi < c->ev_snap; ++qi) {
     auto &q = c->ev_pending[qi];
-    float t[kStepEvents - 1] = {};
-    for (int k = 0; k + 1 < kStepEvents; ++k)
-      if (q[k] && q[k + 1]) HIP_TRY(hipEventElapsedTime(&t[k], q[k], q[k + 1]));
+    // kernel j ran between events 2j and 2j + 1
+    float t[kStepKernels] = {};
+    for (int j = 0; j < kStepKernels; ++j)
+      if (q[2 * j] && q[2 * j + 1]) HIP_TRY(hipEventElapsedTime(&t[j], q[2 * j], q[2 * j + 1]));
     c->kst.transport_neutral_ms += t[0];
     c->kst.interact_neutral_ms += t[1];
-    c->kst.transport_charged_ms += t[4];
-    c->kst.interact_charged_ms += t[5];
-    c->kst.derive_ms += t[2] + t[6];
-    c->kst.advance_ms += t[8];
+    c->kst.transport_charged_ms += t[3];
+    c->kst.interact_charged_ms += t[4];
+    c->kst.derive_ms += t[2] + t[5];
     c->kst.launches++;
     c->ev_free.push_back(q);
   }
@@ -542,7 +554,7 @@ int collect(gxt_ctx *c, bool &idle) {
   c->ev_snap = 0;
   const Counters &h = *c->hcnt;
   if (h.overflow) return c->fail(GXT_ECAPACITY, "track pool or queue overflow (increase pool_capacity)");
-  const int64_t live = live_tracks(h);
+  const int64_t live = live_tracks(h, c->snap_gen);
   c->max_live = std::max<int64_t>(c->max_live, live);
   std::vector<Job *> fin;
   {
@@ -659,13 +671,13 @@ int round_trip(gxt_ctx *c, int k, bool &idle) {
     rc = iterate(c, std::max(k - kLookahead, 1));
     if (rc) return rc;
   }
-  HIP_TRY(hipMemcpyAsync(c->hcnt, c->cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(launch_pack_live(c->ev_live, c->ev_live_packed, c->cfg.max_events, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->hev_live, c->ev_live_packed, c->cfg.max_events * sizeof(int), hipMemcpyDeviceToHost,
-                         c->stream));
+  // one kernel writes the snapshot into pinned memory; the event's
+  // system-scope release makes it visible to the host
+  HIP_TRY(launch_snapshot(c->cnt, c->gen, c->ev_live, c->cfg.max_events, c->dhcnt, c->dhev_live, c->stream));
   HIP_TRY(hipEventRecord(c->snap, c->stream));
   c->ev_snap = c->ev_pending.size();
   c->snap_iter = c->iterations;
+  c->snap_gen = c->gen;
   if (any) {
     rc = iterate(c, kLookahead);
     if (rc) return rc;
@@ -675,7 +687,7 @@ int round_trip(gxt_ctx *c, int k, bool &idle) {
 }
 
 int iters_per_sync(gxt_ctx *c) {
-  return live_tracks(*c->hcnt) > 100000 ? 8 : 4;
+  return live_tracks(*c->hcnt, c->snap_gen) > 100000 ? 8 : 4;
 }
 
 void serve_loop(gxt_ctx *c) {
@@ -723,13 +735,13 @@ void destroy_ctx(gxt_ctx *c) {
   (void)hipHostFree(c->hcnt);
   (void)hipFree(c->ev);
   (void)hipFree(c->ev_live);
-  (void)hipFree(c->ev_live_packed);
   (void)hipHostFree(c->hev_live);
   (void)hipFree(c->rows);
   (void)hipFree(c->spec);
   (void)hipFree(c->trace_buf);
   if (c->ss.fork) (void)hipEventDestroy(c->ss.fork);
   if (c->ss.join) (void)hipEventDestroy(c->ss.join);
+  if (c->ss.entry) (void)hipEventDestroy(c->ss.entry);
   if (c->ss.side) (void)hipStreamDestroy(c->ss.side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -994,8 +1006,13 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
   if (rc) return rc;
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_TRY(hipStreamCreateWithFlags(&c->ss.side, hipStreamNonBlocking));
-  HIP_TRY(hipEventCreateWithFlags(&c->ss.fork, hipEventDisableTiming));
-  HIP_TRY(hipEventCreateWithFlags(&c->ss.join, hipEventDisableTiming));
+  // the chains' events order kernels of this device only: the end of a
+  // kernel already makes its writes visible to the device's later kernels,
+  // so they skip the system-scope fence (a cache write-back each)
+  const unsigned sync_flags = hipEventDisableTiming | hipEventDisableSystemFence;
+  HIP_TRY(hipEventCreateWithFlags(&c->ss.fork, sync_flags));
+  HIP_TRY(hipEventCreateWithFlags(&c->ss.join, sync_flags));
+  HIP_TRY(hipEventCreateWithFlags(&c->ss.entry, sync_flags));
   HIP_TRY(hipEventCreateWithFlags(&c->snap, hipEventDisableTiming));
   c->ss.main = c->stream;
   int cus = 256;
@@ -1090,10 +1107,10 @@ int gxt_create(const gxt_config *cfg, gxt_ctx **out) {
   HIP_TRY(hipMemset(c->ev, 0, sizeof(DevEvent) * cfg->max_events));
   HIP_TRY(dalloc(&c->ev_live, (size_t)cfg->max_events * kLiveStride));
   HIP_TRY(hipMemset(c->ev_live, 0, sizeof(int) * (size_t)cfg->max_events * kLiveStride));
-  HIP_TRY(dalloc(&c->ev_live_packed, cfg->max_events));
-  HIP_TRY(hipMemset(c->ev_live_packed, 0, sizeof(int) * cfg->max_events));
   HIP_TRY(hipHostMalloc((void **)&c->hev_live, sizeof(int) * cfg->max_events, hipHostMallocDefault));
   std::memset(c->hev_live, 0, sizeof(int) * cfg->max_events);
+  HIP_TRY(hipHostGetDevicePointer((void **)&c->dhcnt, c->hcnt, 0));
+  HIP_TRY(hipHostGetDevicePointer((void **)&c->dhev_live, c->hev_live, 0));
   c->row_stride = cfg->n_zbins + cfg->n_rbins + P.geo.nvol;
   HIP_TRY(dalloc(&c->rows, (size_t)c->grid * c->row_stride));
   HIP_TRY(hipMemset(c->rows, 0, sizeof(double) * c->grid * c->row_stride));

@@ -36,6 +36,7 @@ struct StepArgs {
                        // interacts nor ends (>= 1)
   int lds_bytes[2];    // > 0: the photon (0) and e-/e+ (1) transports stage their tables
                        // (DevParams::lds_*) in that much dynamic LDS; 0: they read them in place
+  int gen;             // counter generation of the current pool (Counters)
 };
 
 template <class R>
@@ -58,30 +59,38 @@ struct SampleArgs {
   double *sec_dir;
 };
 
-// Streams of one step-loop iteration: main carries the photon chain, the
-// RNG derivation and the counter advance; side the e-/e+ chain, forked from
-// and joined back into main with the two events.
+// Streams of the step loop: main carries the photon chain, side the e-/e+
+// chain.  After its chain of an iteration each stream records its event
+// (main: fork, side: join) and waits for the other's, so an iteration's
+// chains both follow the previous iteration's, and the two waits run side
+// by side.  launch_side_sync makes side follow all work queued on main so
+// far (injections, snapshots) through the event entry.
 struct StepStreams {
   hipStream_t main, side;
-  hipEvent_t fork, join;
+  hipEvent_t fork, join, entry;
 };
 
-// One step-loop iteration.  With ev (kStepEvents events, null entries not
-// recorded) every kernel is bracketed by events on its stream: ev[0..3] photon transport,
-// interactions and RNG derivation of their new tracks (main), ev[4..7] the
-// same for e-/e+ (side), ev[8..9] the counter advance (main).
-constexpr int kStepEvents = 10;
+// One step-loop iteration of kStepKernels kernels: photon transport,
+// interactions and RNG derivation of their new tracks (main), then the same
+// for e-/e+ (side).  With ev (kStepEvents events, null entries not
+// recorded) kernel j takes its start and end times into ev[2j], ev[2j + 1].
+constexpr int kStepKernels = 6;
+constexpr int kStepEvents = 2 * kStepKernels;
 template <class R>
 hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const StepStreams &s,
                        hipEvent_t *ev = nullptr);
+hipError_t launch_side_sync(const StepStreams &s);
 // Injects n primaries or tracks in flight (In = gxt_primary, gxt_track) from
 // device memory; event slots are prim[i].event + event_base.
 template <class R, class In>
 hipError_t launch_inject(const StepArgs &a, const PoolArgs<R> &pa, const In *prim, int n, int event_base,
                          hipStream_t st);
 
-// out[e] = ev_live[e * kLiveStride] for e < n (the host's snapshot of the live counts).
-hipError_t launch_pack_live(const int *ev_live, int *out, int n, hipStream_t st);
+// The host's snapshot, into pinned host memory (device addresses out_cnt,
+// out_live): out_live[e] = ev_live[e * kLiveStride] for e < n, the pool
+// counts of counter generation g and the overflow flag into out_cnt.
+hipError_t launch_snapshot(const Counters *cnt, int g, const int *ev_live, int n, Counters *out_cnt, int *out_live,
+                           hipStream_t st);
 hipError_t launch_rng_init(const DevParams *P, const uint32_t *seed, const uint64_t *idx, int n, uint32_t *out,
                            hipStream_t st);
 hipError_t launch_rng_draw(uint32_t *states, int n, int k, double *out, hipStream_t st);

@@ -11,13 +11,16 @@
 //   k_interact  : the concatenated queues are sampled, so a wave runs one
 //                 sampler; survivors and secondaries are appended to the next
 //                 pool by wave prefix sums (one atomic per wave and region);
-//   k_derive    : the RNG states of the tracks created this iteration;
-//   k_advance   : the next pool's counts become the current ones.
+//   k_derive    : the RNG states of the tracks created this iteration.
+// The counters rotate over three generations (Counters): the next pool's
+// counts are the following iteration's current ones as they stand, and the
+// photon transport clears the generation after next.
 // The pool, the queues and the new-track list are striped (kStripes stripes
 // with their own counters; workgroup b appends to stripe b % kStripes) so no
 // counter takes the atomics of the whole grid.  Histogram tallies are
 // no-return FP64 atomics into per-workgroup rows; per-event sums are FP64
 // atomics on the event's record.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "gxt_device.h"
@@ -112,7 +115,7 @@ struct WaveReserve {
 // before any result is used.  Returns the entry's index in queues[], -1 for
 // none, -2 when the stripe of the queue is full.
 template <int Q0, int NQ>
-__device__ __forceinline__ int64_t queue_reserve(int q, Counters *cnt, int64_t cap, int64_t sc) {
+__device__ __forceinline__ int64_t queue_reserve(int q, Counters *cnt, int g, int64_t cap, int64_t sc) {
   const unsigned long long lt = (lane() == 0) ? 0ULL : ((~0ULL) >> (64 - lane()));
   const int st = my_stripe();
   unsigned long long m[NQ];
@@ -121,7 +124,7 @@ __device__ __forceinline__ int64_t queue_reserve(int q, Counters *cnt, int64_t c
   for (int k = 0; k < NQ; ++k) {
     m[k] = __ballot(q == Q0 + k);
     base[k] = 0;
-    if (m[k] && lane() == __ffsll((long long)m[k]) - 1) base[k] = atomicAdd(&cnt->ctr(kQ0 + Q0 + k, st), (int)__popcll(m[k]));
+    if (m[k] && lane() == __ffsll((long long)m[k]) - 1) base[k] = atomicAdd(&cnt->ctr(g, kQ0 + Q0 + k, st), (int)__popcll(m[k]));
   }
   int64_t pos = -1;
 #pragma unroll
@@ -481,8 +484,8 @@ __global__ __launch_bounds__(kBlock) void k_inject(StepArgs a, PoolArgs<R> pa, c
       }
     }
     const int st = my_stripe();
-    const int b0 = wave_reserve(region == 0 ? 1 : 0, &a.cnt->ctr(kCur0, st));
-    const int b1 = wave_reserve(region == 1 ? 1 : 0, &a.cnt->ctr(kCur1, st));
+    const int b0 = wave_reserve(region == 0 ? 1 : 0, &a.cnt->ctr(a.gen, kPool0, st));
+    const int b1 = wave_reserve(region == 1 ? 1 : 0, &a.cnt->ctr(a.gen, kPool1, st));
     if (region >= 0) {
       const int64_t s = out_slot(region, st, region == 0 ? b0 : b1, stripe_cap(cur));
       if (s >= 0)
@@ -588,12 +591,17 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
   // counts are clamped to the stripes so a corrupted counter cannot index past them
   const int64_t sc = stripe_cap(cur);
   const int st = my_stripe();
+  const int g1 = gen_next(a.gen);
+  // the photon transport clears the generation the next iteration fills (no
+  // kernel of this iteration uses it; the next starts after this one)
+  if (!kCharged && blockIdx.x == 0 && threadIdx.x < kStripes * kNumKinds)
+    a.cnt->ctr(gen_next(g1), threadIdx.x % kNumKinds, threadIdx.x / kNumKinds) = 0;
   StripeSpan span;
   span.pre[0] = 0;
   bool bad = false;
 #pragma unroll
   for (int k = 0; k < kStripes; ++k) {
-    const int64_t r0 = a.cnt->ctr(kCur0, k), r1 = a.cnt->ctr(kCur1, k);
+    const int64_t r0 = a.cnt->ctr(a.gen, kPool0, k), r1 = a.cnt->ctr(a.gen, kPool1, k);
     bad = bad || r0 < 0 || r1 < 0 || r0 + r1 > sc;
     const int64_t n0 = min(max(r0, (int64_t)0), sc);
     const int64_t n1 = min(max(r1, (int64_t)0), sc - n0);
@@ -601,7 +609,10 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
   }
   // injected primaries can fill a stripe from both ends: report, never overlap
   if (bad && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.cnt->overflow, 1);
-  const int total = span.total();
+  // after an overflow (set by an earlier iteration, which ended before this
+  // one began) the pool is dropped whole: a slot whose store was skipped is
+  // never read; the host reports the overflow at its next snapshot
+  const int total = a.cnt->overflow ? 0 : span.total();
   // a workgroup without tracks (the grid is fixed, the count is on the
   // device) leaves before it stages anything
   if ((int64_t)blockIdx.x * blockDim.x >= total) return;
@@ -757,8 +768,8 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
     // returning atomics first, all in flight together: the survivor's slot in
     // the next pool and the queue entry
     WaveReserve rs;
-    rs.issue(survive ? 1 : 0, &a.cnt->ctr(kCharged ? kNxt1 : kNxt0, st));
-    const int64_t qpos = queue_reserve<kCharged ? 3 : 0, kCharged ? kNumQueues - 3 : 3>(q, a.cnt, cur.cap, sc);
+    rs.issue(survive ? 1 : 0, &a.cnt->ctr(g1, kCharged ? kPool1 : kPool0, st));
+    const int64_t qpos = queue_reserve<kCharged ? 3 : 0, kCharged ? kNumQueues - 3 : 3>(q, a.cnt, g1, cur.cap, sc);
     const int bs = rs.get();
     // then stores and no-return atomics
     if (active) {
@@ -841,7 +852,7 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 8 ? 3 : GXT_STEP_WAVES) void k
     int acc = 0;
     seg[0] = 0;
     for (int k = 0; k < kNQ * kStripes; ++k) {
-      acc += (int)min((int64_t)max(a.cnt->ctr(kQ0 + kQ0q + k / kStripes, k % kStripes), 0), sc);
+      acc += (int)min((int64_t)max(a.cnt->ctr(gen_next(a.gen), kQ0 + kQ0q + k / kStripes, k % kStripes), 0), sc);
       seg[k + 1] = acc;
     }
   }
@@ -933,9 +944,10 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 8 ? 3 : GXT_STEP_WAVES) void k
     // returning atomics first, all in flight together; then the stores
     const int made = outN + outC - (survive ? 1 : 0);
     WaveReserve rN, rC, rK;
-    rN.issue(outN, &a.cnt->ctr(kNxt0, st));
-    rC.issue(outC, &a.cnt->ctr(kNxt1, st));
-    rK.issue(made, &a.cnt->ctr(kCharged ? kNew1 : kNew0, st));
+    const int g1 = gen_next(a.gen);
+    rN.issue(outN, &a.cnt->ctr(g1, kPool0, st));
+    rC.issue(outC, &a.cnt->ctr(g1, kPool1, st));
+    rK.issue(made, &a.cnt->ctr(g1, kCharged ? kNew1 : kNew0, st));
     const int bN = rN.get(), bC = rC.get();
     int nk = rK.get();
     if (active) {
@@ -1005,7 +1017,7 @@ __global__ __launch_bounds__(kBlock) void k_derive(StepArgs a, PoolArgs<R> pa) {
   span.pre[0] = 0;
 #pragma unroll
   for (int k = 0; k < kStripes; ++k)
-    span.pre[k + 1] = span.pre[k] + (int)min((int64_t)max(a.cnt->ctr(kNew0 + kList, k), 0), sc);
+    span.pre[k + 1] = span.pre[k] + (int)min((int64_t)max(a.cnt->ctr(gen_next(a.gen), kNew0 + kList, k), 0), sc);
   const int n = span.total();
   // two adjacent lanes of a wave per track, one MRG32k3a component each;
   // both read the substream index (RNG words 1 and 2) before either writes
@@ -1029,28 +1041,6 @@ __global__ __launch_bounds__(kBlock) void k_derive(StepArgs a, PoolArgs<R> pa) {
     nxt.rng[(int64_t)(3 * c) * nxt.cap + s] = v[0];
     nxt.rng[(int64_t)(3 * c + 1) * nxt.cap + s] = v[1];
     nxt.rng[(int64_t)(3 * c + 2) * nxt.cap + s] = v[2];
-  }
-}
-
-__global__ void k_advance(Counters *c, int64_t cap) {
-  if (threadIdx.x == 0) {
-    const int64_t sc = cap / kStripes;
-    bool bad = c->overflow != 0;
-    for (int s = 0; s < kStripes; ++s) {
-      const int64_t n0 = c->ctr(kNxt0, s), n1 = c->ctr(kNxt1, s);
-      bad = bad || n0 < 0 || n1 < 0 || n0 + n1 > sc;
-    }
-    // an overflowing pool is dropped whole: the host reports GXT_ECAPACITY
-    if (bad) c->overflow = 1;
-    for (int s = 0; s < kStripes; ++s) {
-      c->ctr(kCur0, s) = bad ? 0 : c->ctr(kNxt0, s);
-      c->ctr(kCur1, s) = bad ? 0 : c->ctr(kNxt1, s);
-      c->ctr(kNxt0, s) = 0;
-      c->ctr(kNxt1, s) = 0;
-      c->ctr(kNew0, s) = 0;
-      c->ctr(kNew1, s) = 0;
-      for (int k = 0; k < kNumQueues; ++k) c->ctr(kQ0 + k, s) = 0;
-    }
   }
 }
 
@@ -1286,57 +1276,72 @@ __global__ void k_field_step(DevField F, const double *charge, const double *mom
 // ------------------------------------------------------------------ launchers
 static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
 
+// A step kernel launch; with events (timing only) they take the kernel's own
+// start and end times from its dispatch, so no marker packet sits between
+// two kernels of a chain.
+template <class K, class... Args>
+static inline void launch_k(K kernel, int grid, size_t lds, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+                            Args... args) {
+  if (e0 || e1)
+    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), (uint32_t)lds, st, e0, e1, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), lds, st, args...);
+}
+
 template <class R>
 hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const StepStreams &s, hipEvent_t *ev) {
-  // The photon chain (transport, then interactions) runs on s.main and the
-  // e-/e+ chain on s.side: they read disjoint regions and queues and write
-  // the next pool, queues and tallies only through atomics, so they overlap
-  // and each fills the other's tail.  RNG derivation and the counter advance
-  // follow the join.
+  // The photon chain (transport, interactions, RNG derivation) runs on
+  // s.main and the e-/e+ chain on s.side: they read disjoint regions and
+  // queues and write the next pool, queues and tallies only through atomics,
+  // so they overlap and each fills the other's tail.  Both chains of this
+  // iteration follow both of the previous one: each stream waits (queued at
+  // the end of the previous launch) for the event the other recorded after
+  // its chain.
+  hipEvent_t none[2 * kStepKernels] = {};
+  hipEvent_t *E = ev ? ev : none;
+  const size_t l0 = a.lds_bytes[0], l1 = a.lds_bytes[1];
+  if (a.transport_mode == 4 && l0 > 0)
+    launch_k(k_transport<R, false, 4, true>, grid, l0, s.main, E[0], E[1], a, pa);
+  else if (a.transport_mode == 4)
+    launch_k(k_transport<R, false, 4>, grid, 0, s.main, E[0], E[1], a, pa);
+  else if (l0 > 0)
+    launch_k(k_transport<R, false, 0, true>, grid, l0, s.main, E[0], E[1], a, pa);
+  else
+    launch_k(k_transport<R, false>, grid, 0, s.main, E[0], E[1], a, pa);
+  launch_k(k_interact<R, false>, grid, 0, s.main, E[2], E[3], a, pa);
+  launch_k(k_derive<R, 0>, grid, 0, s.main, E[4], E[5], a, pa);
   hipError_t e = hipEventRecord(s.fork, s.main);
+  if (e != hipSuccess) return e;
+  if (a.transport_mode == 4 && l1 > 0)
+    launch_k(k_transport<R, true, 4, true>, grid, l1, s.side, E[6], E[7], a, pa);
+  else if (a.transport_mode == 4)
+    launch_k(k_transport<R, true, 4>, grid, 0, s.side, E[6], E[7], a, pa);
+  else if (a.transport_mode == 3)
+    launch_k(k_transport<R, true, 3>, grid, 0, s.side, E[6], E[7], a, pa);
+  else if (a.transport_mode == 2)
+    launch_k(k_transport<R, true, 2>, grid, 0, s.side, E[6], E[7], a, pa);
+  else if (a.transport_mode == 1)
+    launch_k(k_transport<R, true, 1>, grid, 0, s.side, E[6], E[7], a, pa);
+  else if (l1 > 0)
+    launch_k(k_transport<R, true, 0, true>, grid, l1, s.side, E[6], E[7], a, pa);
+  else
+    launch_k(k_transport<R, true>, grid, 0, s.side, E[6], E[7], a, pa);
+  launch_k(k_interact<R, true>, grid, 0, s.side, E[8], E[9], a, pa);
+  launch_k(k_derive<R, 1>, grid, 0, s.side, E[10], E[11], a, pa);
+  e = hipEventRecord(s.join, s.side);
+  // each stream's wait is queued after its own chain: main's later work
+  // (the next photon chain, snapshots, injections, synchronisation) follows
+  // the e-/e+ chain, the next e-/e+ chain the photon chain
+  if (e == hipSuccess) e = hipStreamWaitEvent(s.main, s.join, 0);
   if (e == hipSuccess) e = hipStreamWaitEvent(s.side, s.fork, 0);
   if (e != hipSuccess) return e;
-  if (ev && ev[0]) (void)hipEventRecord(ev[0], s.main);
-  if (a.transport_mode == 4 && a.lds_bytes[0] > 0)
-    hipLaunchKernelGGL((k_transport<R, false, 4, true>), dim3(grid), dim3(kBlock), a.lds_bytes[0], s.main, a, pa);
-  else if (a.transport_mode == 4)
-    hipLaunchKernelGGL((k_transport<R, false, 4>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
-  else if (a.lds_bytes[0] > 0)
-    hipLaunchKernelGGL((k_transport<R, false, 0, true>), dim3(grid), dim3(kBlock), a.lds_bytes[0], s.main, a, pa);
-  else
-    hipLaunchKernelGGL((k_transport<R, false>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
-  if (ev && ev[1]) (void)hipEventRecord(ev[1], s.main);
-  hipLaunchKernelGGL((k_interact<R, false>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
-  if (ev && ev[2]) (void)hipEventRecord(ev[2], s.main);
-  hipLaunchKernelGGL((k_derive<R, 0>), dim3(grid), dim3(kBlock), 0, s.main, a, pa);
-  if (ev && ev[3]) (void)hipEventRecord(ev[3], s.main);
-  if (ev && ev[4]) (void)hipEventRecord(ev[4], s.side);
-  if (a.transport_mode == 4 && a.lds_bytes[1] > 0)
-    hipLaunchKernelGGL((k_transport<R, true, 4, true>), dim3(grid), dim3(kBlock), a.lds_bytes[1], s.side, a, pa);
-  else if (a.transport_mode == 4)
-    hipLaunchKernelGGL((k_transport<R, true, 4>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
-  else if (a.transport_mode == 3)
-    hipLaunchKernelGGL((k_transport<R, true, 3>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
-  else if (a.transport_mode == 2)
-    hipLaunchKernelGGL((k_transport<R, true, 2>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
-  else if (a.transport_mode == 1)
-    hipLaunchKernelGGL((k_transport<R, true, 1>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
-  else if (a.lds_bytes[1] > 0)
-    hipLaunchKernelGGL((k_transport<R, true, 0, true>), dim3(grid), dim3(kBlock), a.lds_bytes[1], s.side, a, pa);
-  else
-    hipLaunchKernelGGL((k_transport<R, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
-  if (ev && ev[5]) (void)hipEventRecord(ev[5], s.side);
-  hipLaunchKernelGGL((k_interact<R, true>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
-  if (ev && ev[6]) (void)hipEventRecord(ev[6], s.side);
-  hipLaunchKernelGGL((k_derive<R, 1>), dim3(grid), dim3(kBlock), 0, s.side, a, pa);
-  if (ev && ev[7]) (void)hipEventRecord(ev[7], s.side);
-  e = hipEventRecord(s.join, s.side);
-  if (e == hipSuccess) e = hipStreamWaitEvent(s.main, s.join, 0);
-  if (e != hipSuccess) return e;
-  if (ev && ev[8]) (void)hipEventRecord(ev[8], s.main);
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s.main, a.cnt, pa.cur.cap);
-  if (ev && ev[9]) (void)hipEventRecord(ev[9], s.main);
   return hipGetLastError();
+}
+
+hipError_t launch_side_sync(const StepStreams &s) {
+  hipError_t e = hipEventRecord(s.entry, s.main);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s.side, s.entry, 0);
+  return e;
 }
 
 template <class R, class In>
@@ -1362,14 +1367,25 @@ template hipError_t launch_inject<double, gxt_track>(const StepArgs &, const Poo
 template hipError_t launch_inject<float, gxt_track>(const StepArgs &, const PoolArgs<float> &, const gxt_track *, int,
                                                     int, hipStream_t);
 
-__global__ void k_pack_live(const int *ev_live, int *out, int n) {
+// The host's snapshot of the step loop, written straight into its pinned
+// buffers: the per-event live counts packed (ev_live[e * kLiveStride] at
+// out_live[e]), the track counts of the pool of generation g and the
+// overflow flag (the rest of out_cnt is left as it is).
+__global__ void k_snapshot(const Counters *cnt, int g, const int *ev_live, int n, Counters *out_cnt,
+                           int *out_live) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n) out[e] = ev_live[(int64_t)e * kLiveStride];
+  if (e < n) out_live[e] = ev_live[(int64_t)e * kLiveStride];
+  if (blockIdx.x == 0 && threadIdx.x < 2 * kStripes) {
+    const int k = kPool0 + (threadIdx.x & 1), s = threadIdx.x >> 1;
+    out_cnt->ctr(g, k, s) = cnt->ctr(g, k, s);
+    if (threadIdx.x == 0) out_cnt->overflow = cnt->overflow;
+  }
 }
 
-hipError_t launch_pack_live(const int *ev_live, int *out, int n, hipStream_t st) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pack_live, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, ev_live, out, n);
+hipError_t launch_snapshot(const Counters *cnt, int g, const int *ev_live, int n, Counters *out_cnt, int *out_live,
+                           hipStream_t st) {
+  const int grid = n > kBlock ? (n + kBlock - 1) / kBlock : 1;
+  hipLaunchKernelGGL(k_snapshot, dim3(grid), dim3(kBlock), 0, st, cnt, g, ev_live, n, out_cnt, out_live);
   return hipGetLastError();
 }
 

@@ -227,25 +227,38 @@ struct Pool {
   double *tedep;      // step deposit carried to the interaction kernel (tracing only)
 };
 
-// Device counters of the step loop.  The stripe counters (kinds below) of
-// stripe s are addressed ctr(kind, s); each stripe's counters sit on cache
-// lines no other stripe uses, so the returning atomics of one stripe's
-// workgroups never wait behind another stripe's.  Statistics are per stripe
-// as well (summed by the host).
-enum CounterKind { kCur0 = 0, kCur1 = 1, kNxt0 = 2, kNxt1 = 3, kQ0 = 4, kNew0 = 4 + kNumQueues, kNew1 = 5 + kNumQueues,
-                   kNumKinds = 6 + kNumQueues };
+// Device counters of the step loop, in three generations.  Generation g
+// holds, per stripe, the photon and e-/e+ track counts of a pool (kPool0,
+// kPool1) and the queue and new-track-list counts of the iteration that
+// fills that pool.  Iteration i reads the pool of generation g = i mod 3,
+// fills generation g + 1 (mod 3) and clears generation g + 2, which the
+// next iteration fills: no kernel between iterations moves counts from one
+// set to another, so each chain of the loop waits only for the other
+// chain's last kernel.  The counters of kind k of stripe s in generation g
+// are addressed ctr(g, k, s); each stripe's counters sit on cache lines no
+// other stripe uses, so the returning atomics of one stripe's workgroups
+// never wait behind another stripe's.  Statistics are per stripe as well
+// (summed by the host).
+enum CounterKind { kPool0 = 0, kPool1 = 1, kQ0 = 2, kNew0 = 2 + kNumQueues, kNew1 = 3 + kNumQueues,
+                   kNumKinds = 4 + kNumQueues };
+constexpr int kGens = 3;
 constexpr int kKindStride = 32;      // ints between the counters of one stripe
 constexpr int kStripeStride = 512;  // ints between stripes
+__host__ __device__ constexpr int gen_next(int g) { return g == kGens - 1 ? 0 : g + 1; }
 enum StatKind { kStatSteps = 0, kStatStepsCharged, kStatKilled, kStatInteractions, kStatCreated, kStatLoaded0,
                 kStatLoaded1, kNumStats };
 struct alignas(256) Counters {
-  int32_t c[kStripes * kStripeStride];
+  int32_t c[kGens * kStripes * kStripeStride];
   unsigned long long stat[kStripes][16];  // 128 B per stripe
   int32_t overflow;        // a pool or queue write would have gone out of range
   int32_t pad[31];
   unsigned long long trace_n; // trace records written
-  __host__ __device__ int32_t &ctr(int kind, int s) { return c[s * kStripeStride + kind * kKindStride]; }
-  __host__ __device__ int32_t ctr(int kind, int s) const { return c[s * kStripeStride + kind * kKindStride]; }
+  __host__ __device__ int32_t &ctr(int g, int kind, int s) {
+    return c[(g * kStripes + s) * kStripeStride + kind * kKindStride];
+  }
+  __host__ __device__ int32_t ctr(int g, int kind, int s) const {
+    return c[(g * kStripes + s) * kStripeStride + kind * kKindStride];
+  }
   __host__ __device__ unsigned long long total(int k) const {
     unsigned long long t = 0;
     for (int s = 0; s < kStripes; ++s) t += stat[s][k];

@@ -164,7 +164,8 @@ typedef struct gxt_kernel_stats {
   double interact_neutral_ms;  /* summed durations of the photon interaction launches  */
   double interact_charged_ms;  /* summed durations of the e-/e+ interaction launches   */
   double derive_ms;       /* summed durations of the new-track RNG derivation       */
-  double advance_ms;      /* summed duration of the counter-advance launches        */
+  double advance_ms;      /* 0: the step loop has no counter-advance launch (its
+                             counters rotate over generations); kept for layout */
   int64_t launches;       /* step-loop iterations timed (one launch of each kernel) */
   int64_t track_steps;    /* tracks stepped (one per live track per iteration)     */
   int64_t neutral_steps;  /* of which photon steps                                 */

@@ -36,6 +36,20 @@ def test_rng_states_and_draws_bit_exact(gpu_lib, seed):
     assert gd.min() > 0.0 and gd.max() < 1.0
 
 
+def test_rng_substream_jumps_many_bit_exact(gpu_lib):
+    """The substream jump (eight modular 3x3 products per component) over
+    random 64-bit indices, with a seed whose residues sit at the moduli's
+    edges (m - 1, 0, 1) so that the products reach their largest values."""
+    _flush("rng substream jumps: 20000 random indices")
+    idx = np.random.default_rng(9).integers(0, 2**64 - 1, 20000, dtype=np.uint64, endpoint=True)
+    idx[:4] = [0, 2**64 - 1, 2**63, 255]
+    for s6 in (np.array([4294967086, 1, 0, 4294944442, 4294944441, 1], dtype=np.uint32),
+               np.array([4294967086, 4294967085, 4294967000, 4294944442, 4294944442, 4294944000], dtype=np.uint32)):
+        g = gpu_lib.rng_init(s6, idx)
+        c = O.rng_init(s6, idx)
+        assert np.array_equal(g, c)
+
+
 def _ctx_pair(cfg):
     return gpu_lib_transport(cfg), O.Oracle(cfg)
 
