@@ -293,6 +293,7 @@ struct gxt_ctx {
   int64_t snap_iter = 0;          // step-loop iterations launched before that snapshot
   int gen = 0;                    // counter generation of the next iteration's current pool
   int snap_gen = 0;               // the same when the snapshot was queued
+  bool side_behind = true;        // main holds work the e-/e+ chain must follow (launch_side_sync)
   // non-uniform field
   gxt_field_map fmap;
   std::string fmap_path;
@@ -473,6 +474,7 @@ int admit(gxt_ctx *c) {
   // jobs' primaries were copied to the device at submission, earlier on the
   // same stream); the slots of a new job belong to events without tracks, so
   // the look-ahead iterations still running do not touch them
+  if (!zero_ranges.empty() || !ranges.empty()) c->side_behind = true;
   for (auto &r : zero_ranges) {
     HIP_TRY(hipMemsetAsync(c->ev + r.first, 0, r.second * sizeof(DevEvent), c->stream));
     HIP_TRY(hipMemsetAsync(c->ev_live + (size_t)r.first * kLiveStride, 0, (size_t)r.second * kLiveStride * sizeof(int),
@@ -497,10 +499,14 @@ int admit(gxt_ctx *c) {
 
 int iterate(gxt_ctx *c, int k) {
   StepArgs a = c->args();
-  // the e-/e+ chain follows what main holds so far (injections, cleared
-  // event slots; a snapshot, which must see no iteration in progress)
-  hipError_t se = launch_side_sync(c->ss);
-  if (se != hipSuccess) return c->fail(GXT_EHIP, std::string("step: ") + hipGetErrorString(se));
+  // the e-/e+ chain follows what main holds that it must see (injections,
+  // cleared event slots; a snapshot, which must see no iteration in
+  // progress); otherwise it follows only the photon chains
+  if (c->side_behind) {
+    const hipError_t se = launch_side_sync(c->ss);
+    if (se != hipSuccess) return c->fail(GXT_EHIP, std::string("step: ") + hipGetErrorString(se));
+    c->side_behind = false;
+  }
   for (int i = 0; i < k; ++i) {
     a.gen = c->gen;
     hipError_t e;
@@ -675,6 +681,7 @@ int round_trip(gxt_ctx *c, int k, bool &idle) {
   // system-scope release makes it visible to the host
   HIP_TRY(launch_snapshot(c->cnt, c->gen, c->ev_live, c->cfg.max_events, c->dhcnt, c->dhev_live, c->stream));
   HIP_TRY(hipEventRecord(c->snap, c->stream));
+  c->side_behind = true;
   c->ev_snap = c->ev_pending.size();
   c->snap_iter = c->iterations;
   c->snap_gen = c->gen;
