@@ -592,10 +592,6 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
   const int64_t sc = stripe_cap(cur);
   const int st = my_stripe();
   const int g1 = gen_next(a.gen);
-  // the photon transport clears the generation the next iteration fills (no
-  // kernel of this iteration uses it; the next starts after this one)
-  if (!kCharged && blockIdx.x == 0 && threadIdx.x < kStripes * kNumKinds)
-    a.cnt->ctr(gen_next(g1), threadIdx.x % kNumKinds, threadIdx.x / kNumKinds) = 0;
   StripeSpan span;
   span.pre[0] = 0;
   bool bad = false;
@@ -613,6 +609,11 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAV
   // one began) the pool is dropped whole: a slot whose store was skipped is
   // never read; the host reports the overflow at its next snapshot
   const int total = a.cnt->overflow ? 0 : span.total();
+  // the photon transport clears the generation the next iteration fills (no
+  // kernel of this iteration uses it; the next starts after this one); after
+  // the counter loads, which stay scalar loads
+  if (!kCharged && blockIdx.x == 0 && threadIdx.x < kStripes * kNumKinds)
+    a.cnt->ctr(gen_next(g1), threadIdx.x % kNumKinds, threadIdx.x / kNumKinds) = 0;
   // a workgroup without tracks (the grid is fixed, the count is on the
   // device) leaves before it stages anything
   if ((int64_t)blockIdx.x * blockDim.x >= total) return;
