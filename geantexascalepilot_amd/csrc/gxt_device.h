@@ -194,42 +194,43 @@ template <class R> __device__ __forceinline__ R rmax(R a, R b) { return a > b ? 
 
 // ------------------------------------------------------------------ MRG32k3a
 // L'Ecuyer's recurrence (random/MRG.hpp:35-45, random/src/MRG32k3a.cpp:123-163)
-// on FP64 registers.  a*s < 2^53 is exact; q = floor(p * (1/m)) is exact up to
-// one unit, which the two fix-ups absorb, so the residue is the exact
-// integer residue and the output (one FP64 multiply by norm) is bit-identical
-// to the reference's host branch.
+// in FP64 arithmetic; the state is held as 32-bit integers (half the registers
+// of doubles; each use converts exactly). a*s < 2^53 is exact; q = floor(p *
+// (1/m)) is exact up to one unit, which the two fix-ups absorb, so the residue
+// is the exact integer residue and the output (one FP64 multiply by norm) is
+// bit-identical to the reference's host branch.
 constexpr double kM1 = 4294967087.0, kM2 = 4294944443.0;
 constexpr double kInvM1 = 1.0 / 4294967087.0, kInvM2 = 1.0 / 4294944443.0;
 constexpr double kA12 = 1403580.0, kA13n = 810728.0, kA21 = 527612.0, kA23n = 1370589.0;
 constexpr double kNorm = 1.0 / (4294967087.0 + 1.0);
 
 struct Rng {
-  double s[6];
+  uint32_t s[6];
   __device__ __forceinline__ void load(const uint32_t *plane, int64_t cap, int64_t i) {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) s[k] = (double)plane[k * cap + i];
+    for (int k = 0; k < 6; ++k) s[k] = plane[k * cap + i];
   }
   __device__ __forceinline__ void store(uint32_t *plane, int64_t cap, int64_t i) const {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) plane[k * cap + i] = (uint32_t)s[k];
+    for (int k = 0; k < 6; ++k) plane[k * cap + i] = s[k];
   }
   __device__ __forceinline__ double uniform() {
-    double p1 = kA12 * s[1] - kA13n * s[0];
+    double p1 = kA12 * (double)s[1] - kA13n * (double)s[0];
     double k1 = floor(p1 * kInvM1);
     p1 -= k1 * kM1;
     if (p1 < 0.0) p1 += kM1;
     if (p1 >= kM1) p1 -= kM1;
     s[0] = s[1];
     s[1] = s[2];
-    s[2] = p1;
-    double p2 = kA21 * s[5] - kA23n * s[3];
+    s[2] = (uint32_t)p1;
+    double p2 = kA21 * (double)s[5] - kA23n * (double)s[3];
     double k2 = floor(p2 * kInvM2);
     p2 -= k2 * kM2;
     if (p2 < 0.0) p2 += kM2;
     if (p2 >= kM2) p2 -= kM2;
     s[3] = s[4];
     s[4] = s[5];
-    s[5] = p2;
+    s[5] = (uint32_t)p2;
     return (p1 > p2) ? (p1 - p2) * kNorm : (p1 - p2 + kM1) * kNorm;
   }
   template <class R> __device__ __forceinline__ R u() { return (R)uniform(); }
@@ -303,15 +304,15 @@ __device__ __forceinline__ void jump_substream(const uint32_t *T, const uint32_t
   }
 }
 
-__device__ __forceinline__ void substream_state(const DevParams *P, uint64_t idx, double out[6]) {
+__device__ __forceinline__ void substream_state(const DevParams *P, uint64_t idx, uint32_t out[6]) {
   uint32_t v1[3], v2[3];
   jump_substream(P->jump8, P->seed, idx, v1, v2);
-  out[0] = (double)v1[0];
-  out[1] = (double)v1[1];
-  out[2] = (double)v1[2];
-  out[3] = (double)v2[0];
-  out[4] = (double)v2[1];
-  out[5] = (double)v2[2];
+  out[0] = v1[0];
+  out[1] = v1[1];
+  out[2] = v1[2];
+  out[3] = v2[0];
+  out[4] = v2[1];
+  out[5] = v2[2];
 }
 
 // Substream index of a secondary: two draws of the creating track, 32 bits each.

@@ -446,7 +446,7 @@ __device__ __forceinline__ void inject_state(const DevParams *P, const StepArgs 
   t.vol = p.volume >= 0 ? p.volume : geo_locate(P->geo, t.pos, p.dir);
   if (t.vol >= 0) {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) t.rng.s[k] = (double)p.rng[k];
+    for (int k = 0; k < 6; ++k) t.rng.s[k] = p.rng[k];
     if (p.nsteps > 0) atomicAdd(&a.ev[t.event].nsteps, (unsigned long long)(-(long long)p.nsteps));
   }
 }
@@ -583,9 +583,12 @@ __device__ __forceinline__ NodesT<R, kLds> stage_tables(const DevParams *P, int 
   }
 }
 
+// Register budgets (waves per SIMD), as measured on the 10 GeV e- / PbWO4
+// batch: the plain FP32 e-/e+ transport at 3 (165 VGPRs); the FP64 one at
+// 2 (190 VGPRs), 3 % faster than at 3, where it spilled 14 registers.
 template <class R, bool kCharged, int kMode = 0, bool kLds = false>
-__global__ __launch_bounds__(kBlock, (kCharged && kMode == 0) ? 3 : GXT_STEP_WAVES) void k_transport(StepArgs a,
-                                                                                               PoolArgs<R> pa) {
+__global__ __launch_bounds__(kBlock, (kCharged && kMode == 0 && sizeof(R) == 4) ? 3 : GXT_STEP_WAVES) void k_transport(
+    StepArgs a, PoolArgs<R> pa) {
   const Pool<R> &cur = pa.cur;
   const Pool<R> &nxt = pa.nxt;
   // counts are clamped to the stripes so a corrupted counter cannot index past them
@@ -834,11 +837,11 @@ __device__ __forceinline__ int sec_fate(const DevParams *P, TrackRegs<R> &t, Dep
   return 1;
 }
 
-// FP64 interactions at 3 waves per SIMD (e-/e+: 160 VGPRs; measured 1.9 %
-// faster on the 10 GeV e- / PbWO4 batch); the FP32 form was 0.6 % slower at
-// 3 and keeps 2.
+// Interactions compiled for at least 2 waves per SIMD: FP64 takes 161 VGPRs
+// (3 waves fit); bounded to 3 it took 150 and measured 1 % slower beside the
+// 2-wave FP64 e-/e+ transport (the chains share the CUs).
 template <class R, bool kCharged>
-__global__ __launch_bounds__(kBlock, sizeof(R) == 8 ? 3 : GXT_STEP_WAVES) void k_interact(StepArgs a, PoolArgs<R> pa) {
+__global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a, PoolArgs<R> pa) {
   const Pool<R> &cur = pa.cur;
   const Pool<R> &nxt = pa.nxt;
   const int64_t sc = stripe_cap(cur);
@@ -1063,9 +1066,9 @@ __global__ void k_rng_draw(uint32_t *states, int n, int k, double *out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Rng r;
-  for (int j = 0; j < 6; ++j) r.s[j] = (double)states[6 * i + j];
+  for (int j = 0; j < 6; ++j) r.s[j] = states[6 * i + j];
   for (int j = 0; j < k; ++j) out[(int64_t)i * k + j] = r.uniform();
-  for (int j = 0; j < 6; ++j) states[6 * i + j] = (uint32_t)r.s[j];
+  for (int j = 0; j < 6; ++j) states[6 * i + j] = r.s[j];
 }
 
 // Straight-line navigation of the analytic geometry for n points (the
@@ -1124,11 +1127,11 @@ __global__ void k_sample(const DevParams *P, int slot, int model, SampleArgs s) 
   const int n = s.n;
   if (i >= n) return;
   Rng r;
-  for (int j = 0; j < 6; ++j) r.s[j] = (double)s.rng[6 * i + j];
+  for (int j = 0; j < 6; ++j) r.s[j] = s.rng[6 * i + j];
   const R d[3] = {(R)s.dir[i], (R)s.dir[n + i], (R)s.dir[2 * n + i]};
   Interaction<R> o;
   sample_model<R>(P, P->mat[slot], model, (R)s.ekin[i], d, r, o);
-  for (int j = 0; j < 6; ++j) s.rng[6 * i + j] = (uint32_t)r.s[j];
+  for (int j = 0; j < 6; ++j) s.rng[6 * i + j] = r.s[j];
   s.out_ekin[i] = (double)o.ekin;
   s.out_dir[i] = (double)o.d[0];
   s.out_dir[n + i] = (double)o.d[1];
