@@ -1303,6 +1303,10 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
   // its chain.
   hipEvent_t none[2 * kStepKernels] = {};
   hipEvent_t *E = ev ? ev : none;
+  // the RNG derivation (72 VGPRs, a chain of dependent modular products per
+  // lane) runs twice the others' workgroups per CU: more waves to hide that
+  // latency (measured +0.9 % FP64, +1.3 % FP32; half the grid -3.9 %)
+  const int dgrid = 2 * grid;
   const size_t l0 = a.lds_bytes[0], l1 = a.lds_bytes[1];
   if (a.transport_mode == 4 && l0 > 0)
     launch_k(k_transport<R, false, 4, true>, grid, l0, s.main, E[0], E[1], a, pa);
@@ -1313,7 +1317,7 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
   else
     launch_k(k_transport<R, false>, grid, 0, s.main, E[0], E[1], a, pa);
   launch_k(k_interact<R, false>, grid, 0, s.main, E[2], E[3], a, pa);
-  launch_k(k_derive<R, 0>, grid, 0, s.main, E[4], E[5], a, pa);
+  launch_k(k_derive<R, 0>, dgrid, 0, s.main, E[4], E[5], a, pa);
   hipError_t e = hipEventRecord(s.fork, s.main);
   if (e != hipSuccess) return e;
   if (a.transport_mode == 4 && l1 > 0)
@@ -1331,7 +1335,7 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
   else
     launch_k(k_transport<R, true>, grid, 0, s.side, E[6], E[7], a, pa);
   launch_k(k_interact<R, true>, grid, 0, s.side, E[8], E[9], a, pa);
-  launch_k(k_derive<R, 1>, grid, 0, s.side, E[10], E[11], a, pa);
+  launch_k(k_derive<R, 1>, dgrid, 0, s.side, E[10], E[11], a, pa);
   e = hipEventRecord(s.join, s.side);
   // each stream's wait is queued after its own chain: main's later work
   // (the next photon chain, snapshots, injections, synchronisation) follows
