@@ -615,8 +615,9 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0 && sizeof(R) == 4) 
   // the photon transport clears the generation the next iteration fills (no
   // kernel of this iteration uses it; the next starts after this one); after
   // the counter loads, which stay scalar loads
-  if (!kCharged && blockIdx.x == 0 && threadIdx.x < kStripes * kNumKinds)
-    a.cnt->ctr(gen_next(g1), threadIdx.x % kNumKinds, threadIdx.x / kNumKinds) = 0;
+  if (!kCharged && blockIdx.x == 0)
+    for (int i = threadIdx.x; i < kStripes * kNumKinds; i += blockDim.x)
+      a.cnt->ctr(gen_next(g1), i % kNumKinds, i / kNumKinds) = 0;
   // a workgroup without tracks (the grid is fixed, the count is on the
   // device) leaves before it stages anything
   if ((int64_t)blockIdx.x * blockDim.x >= total) return;
@@ -851,17 +852,26 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
   // of their lengths (clamped to the stripe capacity)
   constexpr int kQ0q = kCharged ? 3 : 0;
   constexpr int kNQ = kCharged ? kNumQueues - 3 : 3;
-  __shared__ int seg[kNQ * kStripes + 1];
-  if (threadIdx.x == 0) {
-    int acc = 0;
-    seg[0] = 0;
-    for (int k = 0; k < kNQ * kStripes; ++k) {
-      acc += (int)min((int64_t)max(a.cnt->ctr(gen_next(a.gen), kQ0 + kQ0q + k / kStripes, k % kStripes), 0), sc);
-      seg[k + 1] = acc;
+  constexpr int kSeg = kNQ * kStripes;
+  __shared__ int seg[kSeg + 1];
+  if (threadIdx.x < 64) {
+    // wave 0: one counter per lane, all loads in flight at once (a loop of
+    // dependent scalar loads took one memory round trip per eight), then a
+    // prefix over the lanes
+    int carry = 0;
+    for (int k0 = 0; k0 < kSeg; k0 += 64) {
+      const int k = k0 + (int)threadIdx.x;
+      int v = 0;
+      if (k < kSeg)
+        v = (int)min((int64_t)max(a.cnt->ctr(gen_next(a.gen), kQ0 + kQ0q + k / kStripes, k % kStripes), 0), sc);
+      const int incl = wave_incl_scan(v) + carry;
+      if (k < kSeg) seg[k + 1] = incl;
+      carry = __shfl(incl, 63, 64);
     }
+    if (threadIdx.x == 0) seg[0] = 0;
   }
   __syncthreads();
-  const int total = seg[kNQ * kStripes];
+  const int total = seg[kSeg];
   // a workgroup without items leaves before it stages anything
   if ((int64_t)blockIdx.x * blockDim.x >= total) return;
   __shared__ WaveHist whs[kBlock / 64];
@@ -888,7 +898,7 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
     Dep dp;  // every deposit of an interaction is at the track's position
     if (active) {
       // segment of item i: binary search over the prefix (seg[0] = 0 <= i < total)
-      int lo = 0, hi = kNQ * kStripes;
+      int lo = 0, hi = kSeg;
       while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
         if (i >= seg[mid])
