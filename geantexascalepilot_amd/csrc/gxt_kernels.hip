@@ -91,6 +91,18 @@ struct StripeSpan {
     }
     off = i - base;
   }
+  // pre[] from each lane k < kStripes's count v (lanes >= kStripes pass 0):
+  // a wave scan, read back into uniform registers.  The caller loads one
+  // count per lane, so the wave has all its counter loads in flight at once
+  // (a loop over the stripes' counters compiles to scalar loads that wait
+  // for each other in groups of eight).  Must be reached by all 64 lanes.
+  __device__ __forceinline__ void from_lanes(int v) {
+    static_assert(kStripes <= 64, "one stripe per lane");
+    const int incl = wave_incl_scan(v);
+    pre[0] = 0;
+#pragma unroll
+    for (int k = 0; k < kStripes; ++k) pre[k + 1] = __builtin_amdgcn_readlane(incl, k);
+  }
 };
 
 // Two-phase wave reservation.  issue() sends the wave's one atomic and get()
@@ -596,15 +608,19 @@ __global__ __launch_bounds__(kBlock, (kCharged && kMode == 0 && sizeof(R) == 4) 
   const int st = my_stripe();
   const int g1 = gen_next(a.gen);
   StripeSpan span;
-  span.pre[0] = 0;
-  bool bad = false;
-#pragma unroll
-  for (int k = 0; k < kStripes; ++k) {
-    const int64_t r0 = a.cnt->ctr(a.gen, kPool0, k), r1 = a.cnt->ctr(a.gen, kPool1, k);
-    bad = bad || r0 < 0 || r1 < 0 || r0 + r1 > sc;
+  bool bad;
+  {
+    // lane k < kStripes: stripe k's photon and e-/e+ counts
+    const int l = lane();
+    int64_t r0 = 0, r1 = 0;
+    if (l < kStripes) {
+      r0 = a.cnt->ctr(a.gen, kPool0, l);
+      r1 = a.cnt->ctr(a.gen, kPool1, l);
+    }
     const int64_t n0 = min(max(r0, (int64_t)0), sc);
     const int64_t n1 = min(max(r1, (int64_t)0), sc - n0);
-    span.pre[k + 1] = span.pre[k] + (int)(kCharged ? n1 : n0);
+    span.from_lanes((int)(kCharged ? n1 : n0));
+    bad = __ballot(r0 < 0 || r1 < 0 || r0 + r1 > sc) != 0ULL;
   }
   // injected primaries can fill a stripe from both ends: report, never overlap
   if (bad && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.cnt->overflow, 1);
@@ -1028,10 +1044,11 @@ __global__ __launch_bounds__(kBlock) void k_derive(StepArgs a, PoolArgs<R> pa) {
   const int64_t sc = stripe_cap(nxt);
   a.newlist += kList * nxt.cap;
   StripeSpan span;
-  span.pre[0] = 0;
-#pragma unroll
-  for (int k = 0; k < kStripes; ++k)
-    span.pre[k + 1] = span.pre[k] + (int)min((int64_t)max(a.cnt->ctr(gen_next(a.gen), kNew0 + kList, k), 0), sc);
+  {
+    const int l = lane();
+    const int64_t r = l < kStripes ? a.cnt->ctr(gen_next(a.gen), kNew0 + kList, l) : 0;
+    span.from_lanes((int)min(max(r, (int64_t)0), sc));
+  }
   const int n = span.total();
   // two adjacent lanes of a wave per track, one MRG32k3a component each;
   // both read the substream index (RNG words 1 and 2) before either writes
