@@ -1347,20 +1347,27 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
   launch_k(k_derive<R, 0>, dgrid, 0, s.main, E[4], E[5], a, pa);
   hipError_t e = hipEventRecord(s.fork, s.main);
   if (e != hipSuccess) return e;
+  // the FP64 e-/e+ transports without field sub-steps (plain, with msc) fit
+  // 2 workgroups per CU and launch only those: a second round of workgroups
+  // waiting for a slot had made the plain one 24 % slower (128.5 against
+  // 104 us per launch; +4 % events/s, +4 % with msc).  Where per-track work
+  // is irregular (field maps, RK45, the tree in a field) the spare
+  // workgroups balance it: 2 per CU measured 2-13 % slower there.
+  const int egrid = (sizeof(R) == 8 && a.transport_mode <= 1) ? grid / 2 : grid;
   if (a.transport_mode == 4 && l1 > 0)
-    launch_k(k_transport<R, true, 4, true>, grid, l1, s.side, E[6], E[7], a, pa);
+    launch_k(k_transport<R, true, 4, true>, egrid, l1, s.side, E[6], E[7], a, pa);
   else if (a.transport_mode == 4)
-    launch_k(k_transport<R, true, 4>, grid, 0, s.side, E[6], E[7], a, pa);
+    launch_k(k_transport<R, true, 4>, egrid, 0, s.side, E[6], E[7], a, pa);
   else if (a.transport_mode == 3)
-    launch_k(k_transport<R, true, 3>, grid, 0, s.side, E[6], E[7], a, pa);
+    launch_k(k_transport<R, true, 3>, egrid, 0, s.side, E[6], E[7], a, pa);
   else if (a.transport_mode == 2)
-    launch_k(k_transport<R, true, 2>, grid, 0, s.side, E[6], E[7], a, pa);
+    launch_k(k_transport<R, true, 2>, egrid, 0, s.side, E[6], E[7], a, pa);
   else if (a.transport_mode == 1)
-    launch_k(k_transport<R, true, 1>, grid, 0, s.side, E[6], E[7], a, pa);
+    launch_k(k_transport<R, true, 1>, egrid, 0, s.side, E[6], E[7], a, pa);
   else if (l1 > 0)
-    launch_k(k_transport<R, true, 0, true>, grid, l1, s.side, E[6], E[7], a, pa);
+    launch_k(k_transport<R, true, 0, true>, egrid, l1, s.side, E[6], E[7], a, pa);
   else
-    launch_k(k_transport<R, true>, grid, 0, s.side, E[6], E[7], a, pa);
+    launch_k(k_transport<R, true>, egrid, 0, s.side, E[6], E[7], a, pa);
   launch_k(k_interact<R, true>, grid, 0, s.side, E[8], E[9], a, pa);
   launch_k(k_derive<R, 1>, dgrid, 0, s.side, E[10], E[11], a, pa);
   e = hipEventRecord(s.join, s.side);
