@@ -559,22 +559,36 @@ def main():
     # beside it.)
     rec = int(ks.record_bytes)
     # The dominant kernel (the rocprofv3 summary in profiles/kernels_c1_fp64.md)
-    # is the e-/e+ transport; only its launches are bracketed by HIP events
-    # during the timed batches (profile_kernels = 2), which keeps the
+    # is the photon interaction kernel, then the e-/e+ transport; only their
+    # launches carry HIP events during the timed batches (profile_kernels = 2,
+    # the events taken from the kernels' own dispatches), which keeps the
     # instrumentation out of the measured throughput.
-    name = "k_transport<f64,e-/e+>"
-    ms, units = ks.transport_charged_ms, ks.charged_loaded
-    nbytes = units * 2 * rec
-    per_unit = ("2 x %d B per e-/e+ track loaded by a launch (one read and one write of its record), "
-                "%d tracks loaded for %d e-/e+ steps" % (rec, units, ks.charged_steps))
     launches = max(ks.launches, 1)
+    name = "k_interact<f64,photon>"
+    ms, units, made = ks.interact_neutral_ms, ks.neutral_interactions, ks.neutral_created
+    # a sampled photon's record read and written back (or its slot left:
+    # the record is read whole either way, and a survivor is written whole to
+    # the next pool), plus each created track's record written
+    nbytes = units * 2 * rec + made * rec
+    per_unit = ("2 x %d B per photon sampled (its record read, the survivor written) + %d B per track it "
+                "creates; %d photons sampled, %d tracks created" % (rec, rec, units, made))
     avg_us = 1e3 * ms / launches
     achieved = (nbytes / launches) / (avg_us * 1e-6) / 1e9
-    per_step_gbs = (ks.charged_steps * 2 * 148 / launches) / (avg_us * 1e-6) / 1e9
+    # the e-/e+ transport, the dominant kernel of earlier builds, for continuity
+    tname = "k_transport<f64,e-/e+>"
+    tms, tunits = ks.transport_charged_ms, ks.charged_loaded
+    tbytes = tunits * 2 * rec
+    tavg = 1e3 * tms / launches
+    tach = (tbytes / launches) / (tavg * 1e-6) / 1e9
+    transport = dict(kernel=tname, achieved=tach, frac=tach / PEAK_HBM_GBS, traffic=profiled_traffic(tname),
+                     avg_launch_us=tavg, bytes_per_launch=tbytes / launches, kernel_ms=tms,
+                     per_unit="2 x %d B per e-/e+ track loaded by a launch (one read and one write of its record), "
+                              "%d tracks loaded for %d e-/e+ steps" % (rec, tunits, ks.charged_steps),
+                     per_track_step_296b_gbs=(ks.charged_steps * 2 * 148 / launches) / (tavg * 1e-6) / 1e9)
     roofline = dict(bound="hbm", achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s", frac=achieved / PEAK_HBM_GBS,
                     traffic=profiled_traffic(name), kernel=name, launches=int(ks.launches), avg_launch_us=avg_us,
                     bytes_per_launch=nbytes / launches, per_unit=per_unit, record_bytes_in_pool=rec,
-                    kernel_ms=ms, per_track_step_296b_gbs=per_step_gbs,
+                    kernel_ms=ms, e_transport=transport,
                     # every track loaded by a transport launch (photons included) over the wall time
                     whole_run_gbs=(ks.charged_loaded + ks.neutral_loaded) * 2 * rec / r64["seconds"] / 1e9)
     e64 = r64["edep"]["edep"]

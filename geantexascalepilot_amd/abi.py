@@ -87,7 +87,8 @@ class KernelStats(C.Structure):
                 ("advance_ms", C.c_double), ("launches", C.c_int64), ("track_steps", C.c_int64),
                 ("neutral_steps", C.c_int64), ("charged_steps", C.c_int64), ("interactions", C.c_int64),
                 ("created", C.c_int64), ("record_bytes", C.c_int32), ("pad", C.c_int32),
-                ("neutral_loaded", C.c_int64), ("charged_loaded", C.c_int64)]
+                ("neutral_loaded", C.c_int64), ("charged_loaded", C.c_int64),
+                ("neutral_interactions", C.c_int64), ("neutral_created", C.c_int64)]
 
 
 class JobResult(C.Structure):

@@ -513,11 +513,13 @@ int iterate(gxt_ctx *c, int k) {
     hipEvent_t *evs = nullptr;
     if (c->prof) {
       if (c->ev_free.empty()) {
-        // level 2 times only the e-/e+ transport (kernel 3: events 6, 7);
-        // the events only time kernels, so they need no system-scope fence
+        // level 2 times only the photon interactions and the e-/e+ transport
+        // (kernels 1 and 3: events 2, 3, 6, 7); the events only time kernels,
+        // so they need no system-scope fence
         std::array<hipEvent_t, kStepEvents> q{};
         for (int k = 0; k < kStepEvents; ++k)
-          if (c->prof_level == 1 || k / 2 == 3) HIP_TRY(hipEventCreateWithFlags(&q[k], hipEventDisableSystemFence));
+          if (c->prof_level == 1 || k / 2 == 1 || k / 2 == 3)
+            HIP_TRY(hipEventCreateWithFlags(&q[k], hipEventDisableSystemFence));
         c->ev_free.push_back(q);
       }
       c->ev_pending.push_back(c->ev_free.back());
@@ -1346,6 +1348,8 @@ int gxt_get_kernel_stats(gxt_ctx *c, gxt_kernel_stats *o) {
   o->created = (int64_t)h.total(kStatCreated);
   o->neutral_loaded = (int64_t)h.total(kStatLoaded0);
   o->charged_loaded = (int64_t)h.total(kStatLoaded1);
+  o->neutral_interactions = (int64_t)h.total(kStatInteractions0);
+  o->neutral_created = (int64_t)h.total(kStatCreated0);
   // x, y, z (FP64) + dir, E, 3 interaction lengths (R) + 6 RNG words + event, meta, nsteps
   // + 3 FP64 per-track tallies
   const int r = c->precision == GXT_FP32 ? 4 : 8;

@@ -1025,9 +1025,13 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
       }
     }
     wave_add_u64(&a.cnt->stat[st][kStatInteractions], active);
+    if (!kCharged) wave_add_u64(&a.cnt->stat[st][kStatInteractions0], active);
     {
       const int tot = __shfl(rK.incl, 63, 64);
-      if (tot > 0 && lane() == 63) atomicAdd(&a.cnt->stat[st][kStatCreated], (unsigned long long)tot);
+      if (tot > 0 && lane() == 63) {
+        atomicAdd(&a.cnt->stat[st][kStatCreated], (unsigned long long)tot);
+        if (!kCharged) atomicAdd(&a.cnt->stat[st][kStatCreated0], (unsigned long long)tot);
+      }
     }
   }
   wh_flush(a, wh);

@@ -249,7 +249,7 @@ constexpr int kKindStride = 32;      // ints between the counters of one stripe
 constexpr int kStripeStride = 512;  // ints between stripes
 __host__ __device__ constexpr int gen_next(int g) { return g == kGens - 1 ? 0 : g + 1; }
 enum StatKind { kStatSteps = 0, kStatStepsCharged, kStatKilled, kStatInteractions, kStatCreated, kStatLoaded0,
-                kStatLoaded1, kNumStats };
+                kStatLoaded1, kStatInteractions0, kStatCreated0, kNumStats };
 struct alignas(256) Counters {
   int32_t c[kGens * kStripes * kStripeStride];
   unsigned long long stat[kStripes][16];  // 128 B per stripe

@@ -98,8 +98,9 @@ typedef struct gxt_config {
   int32_t trace_tracks;     /* >0: record the first trace_steps steps of tracks     */
   int32_t trace_steps;
   int32_t profile_kernels;  /* 1: time every step-loop launch with HIP events; 2: only the
-                               e-/e+ transport launches (the dominant kernel; 2 events per
-                               iteration instead of 10) */
+                               photon interaction and e-/e+ transport launches (the two
+                               largest kernels).  The events take each kernel's start and
+                               end from its own dispatch (no packets between kernels). */
   const char *data_dir;     /* directory holding the physics data files             */
   const char *field_map;    /* NULL: the uniform field above; else a cylindrical (r, z)
                                field-map file (see gxt_field_map_load), which replaces it */
@@ -177,6 +178,8 @@ typedef struct gxt_kernel_stats {
   int64_t neutral_loaded; /* track records read by the photon transport launches (a
                              track takes up to several steps per launch)          */
   int64_t charged_loaded; /* track records read by the e-/e+ transport launches   */
+  int64_t neutral_interactions; /* of interactions, photons sampled               */
+  int64_t neutral_created;      /* of created, tracks the photon interactions made */
 } gxt_kernel_stats;
 
 /* Completed job, as returned by gxt_poll. */

@@ -376,3 +376,31 @@ def test_pool_overflow_is_reported(gpu_lib):
             t.run(max_iterations=10000)
         assert e.value.code == -4
         assert "pool" in str(e.value)
+
+
+@pytest.mark.parametrize("level", [1, 2])
+def test_kernel_stats_counts(gpu_lib, level):
+    """The step loop's kernel timings and counts: every timed kernel has a
+    duration (level 2 times only the photon interactions and the e-/e+
+    transport), there is no counter-advance launch, the photon interactions
+    and the tracks they create are parts of the totals, and the steps add
+    up to the run's."""
+    _flush("kernel stats, profile level %d" % level)
+    n = 100
+    cfg = abi.config_lead_block(max_events=n, profile_kernels=level)
+    p = gpu_lib.make_primaries(cfg.seed, abi.GAMMA, 1000.0, 1000.0, 0.0, n)
+    with gpu_lib.Transport(cfg) as t:
+        t.submit(p, 1)
+        t.run()
+        ks = t.kernel_stats()
+        st = t.stats()
+    # timed iterations are those collected at a snapshot: the last look-ahead
+    # iteration of the run may not be
+    assert 0 < ks.launches <= st.iterations <= ks.launches + 1
+    assert ks.interact_neutral_ms > 0 and ks.transport_charged_ms > 0
+    timed = (ks.transport_neutral_ms, ks.interact_charged_ms, ks.derive_ms)
+    assert all(x > 0 for x in timed) if level == 1 else all(x == 0 for x in timed)
+    assert ks.advance_ms == 0
+    assert 0 < ks.neutral_interactions < ks.interactions
+    assert 0 < ks.neutral_created < ks.created
+    assert ks.neutral_steps + ks.charged_steps == ks.track_steps == st.track_steps
