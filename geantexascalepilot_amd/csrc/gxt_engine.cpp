@@ -264,7 +264,7 @@ struct gxt_ctx {
   std::string err;
   int device = 0;
   hipStream_t stream = nullptr;
-  StepStreams ss{};  // main = stream, side stream and fork/join events
+  StepStreams ss{};  // main = stream, side stream; fork, join and entry events
   HostData hd;
   DeviceData dd;
   int precision = GXT_FP64;

@@ -871,16 +871,21 @@ __global__ __launch_bounds__(kBlock, GXT_STEP_WAVES) void k_interact(StepArgs a,
   constexpr int kSeg = kNQ * kStripes;
   __shared__ int seg[kSeg + 1];
   if (threadIdx.x < 64) {
-    // wave 0: one counter per lane, all loads in flight at once (a loop of
-    // dependent scalar loads took one memory round trip per eight), then a
-    // prefix over the lanes
+    // wave 0: one counter per lane and 64-counter chunk, all loads in flight
+    // at once (a loop of dependent scalar loads took one memory round trip
+    // per eight), then a prefix over the lanes, chunk after chunk
+    constexpr int kChunks = (kSeg + 63) / 64;
+    int v[kChunks];
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+      const int k = c * 64 + (int)threadIdx.x;
+      v[c] = k < kSeg ? a.cnt->ctr(gen_next(a.gen), kQ0 + kQ0q + k / kStripes, k % kStripes) : 0;
+    }
     int carry = 0;
-    for (int k0 = 0; k0 < kSeg; k0 += 64) {
-      const int k = k0 + (int)threadIdx.x;
-      int v = 0;
-      if (k < kSeg)
-        v = (int)min((int64_t)max(a.cnt->ctr(gen_next(a.gen), kQ0 + kQ0q + k / kStripes, k % kStripes), 0), sc);
-      const int incl = wave_incl_scan(v) + carry;
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+      const int k = c * 64 + (int)threadIdx.x;
+      const int incl = wave_incl_scan((int)min((int64_t)max(v[c], 0), sc)) + carry;
       if (k < kSeg) seg[k + 1] = incl;
       carry = __shfl(incl, 63, 64);
     }
