@@ -1343,6 +1343,9 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
   // lane) runs twice the others' workgroups per CU: more waves to hide that
   // latency (measured +0.9 % FP64, +1.3 % FP32; half the grid -3.9 %)
   const int dgrid = 2 * grid;
+  // the FP64 interactions (168 VGPRs, 3 waves per SIMD) launch the 3
+  // workgroups per CU that fit (+0.5 % FP64 measured twice; FP32 -0.7 %)
+  const int igrid = sizeof(R) == 8 ? grid * 3 / 4 : grid;
   const size_t l0 = a.lds_bytes[0], l1 = a.lds_bytes[1];
   if (a.transport_mode == 4 && l0 > 0)
     launch_k(k_transport<R, false, 4, true>, grid, l0, s.main, E[0], E[1], a, pa);
@@ -1352,7 +1355,7 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
     launch_k(k_transport<R, false, 0, true>, grid, l0, s.main, E[0], E[1], a, pa);
   else
     launch_k(k_transport<R, false>, grid, 0, s.main, E[0], E[1], a, pa);
-  launch_k(k_interact<R, false>, grid, 0, s.main, E[2], E[3], a, pa);
+  launch_k(k_interact<R, false>, igrid, 0, s.main, E[2], E[3], a, pa);
   launch_k(k_derive<R, 0>, dgrid, 0, s.main, E[4], E[5], a, pa);
   hipError_t e = hipEventRecord(s.fork, s.main);
   if (e != hipSuccess) return e;
@@ -1377,7 +1380,7 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
     launch_k(k_transport<R, true, 0, true>, egrid, l1, s.side, E[6], E[7], a, pa);
   else
     launch_k(k_transport<R, true>, egrid, 0, s.side, E[6], E[7], a, pa);
-  launch_k(k_interact<R, true>, grid, 0, s.side, E[8], E[9], a, pa);
+  launch_k(k_interact<R, true>, igrid, 0, s.side, E[8], E[9], a, pa);
   launch_k(k_derive<R, 1>, dgrid, 0, s.side, E[10], E[11], a, pa);
   e = hipEventRecord(s.join, s.side);
   // each stream's wait is queued after its own chain: main's later work
