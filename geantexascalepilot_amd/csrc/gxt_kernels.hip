@@ -1339,12 +1339,13 @@ hipError_t launch_step(const StepArgs &a, const PoolArgs<R> &pa, int grid, const
   // its chain.
   hipEvent_t none[2 * kStepKernels] = {};
   hipEvent_t *E = ev ? ev : none;
-  // the RNG derivation (72 VGPRs, a chain of dependent modular products per
+  // the RNG derivation (80 VGPRs, a chain of dependent modular products per
   // lane) runs twice the others' workgroups per CU: more waves to hide that
   // latency (measured +0.9 % FP64, +1.3 % FP32; half the grid -3.9 %)
   const int dgrid = 2 * grid;
   // the FP64 interactions (168 VGPRs, 3 waves per SIMD) launch the 3
-  // workgroups per CU that fit (+0.5 % FP64 measured twice; FP32 -0.7 %)
+  // workgroups per CU that fit (+0.5 % FP64, measured three times; FP32
+  // -0.7 %, so it keeps 4)
   const int igrid = sizeof(R) == 8 ? grid * 3 / 4 : grid;
   const size_t l0 = a.lds_bytes[0], l1 = a.lds_bytes[1];
   if (a.transport_mode == 4 && l0 > 0)
