@@ -1325,7 +1325,9 @@ int gxt_get_spectra(gxt_ctx *c, int64_t *counts) {
 int gxt_get_stats(gxt_ctx *c, gxt_stats *o) {
   if (!c || !o) return set_err(GXT_EINVAL, "bad arguments");
   HIP_TRY(hipStreamSynchronize(c->stream));
-  Counters h;
+  // ~100 KB (three generations of striped counters): on the heap, not the caller's stack
+  auto hp = std::make_unique<Counters>();
+  Counters &h = *hp;
   HIP_TRY(hipMemcpy(&h, c->cnt, sizeof(h), hipMemcpyDeviceToHost));
   o->track_steps = (int64_t)h.total(kStatSteps);
   o->iterations = c->iterations;
@@ -1338,7 +1340,9 @@ int gxt_get_stats(gxt_ctx *c, gxt_stats *o) {
 int gxt_get_kernel_stats(gxt_ctx *c, gxt_kernel_stats *o) {
   if (!c || !o) return set_err(GXT_EINVAL, "bad arguments");
   HIP_TRY(hipStreamSynchronize(c->stream));
-  Counters h;
+  // ~100 KB (three generations of striped counters): on the heap, not the caller's stack
+  auto hp = std::make_unique<Counters>();
+  Counters &h = *hp;
   HIP_TRY(hipMemcpy(&h, c->cnt, sizeof(h), hipMemcpyDeviceToHost));
   *o = c->kst;
   o->track_steps = (int64_t)h.total(kStatSteps);
@@ -1363,7 +1367,9 @@ int gxt_reset_tallies(gxt_ctx *c) {
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipMemset(c->rows, 0, sizeof(double) * c->grid * c->row_stride));
   HIP_TRY(hipMemset(c->spec, 0, sizeof(unsigned long long) * c->grid * 3 * kSpecBins));
-  Counters h;
+  // ~100 KB (three generations of striped counters): on the heap, not the caller's stack
+  auto hp = std::make_unique<Counters>();
+  Counters &h = *hp;
   HIP_TRY(hipMemcpy(&h, c->cnt, sizeof(h), hipMemcpyDeviceToHost));
   std::memset(h.stat, 0, sizeof(h.stat));
   h.trace_n = 0;
@@ -1380,7 +1386,9 @@ int gxt_reset_tallies(gxt_ctx *c) {
 int gxt_get_trace(gxt_ctx *c, double *out, int64_t max_records, int64_t *n_out) {
   if (!c || !n_out) return set_err(GXT_EINVAL, "bad arguments");
   HIP_TRY(hipStreamSynchronize(c->stream));
-  Counters h;
+  // ~100 KB (three generations of striped counters): on the heap, not the caller's stack
+  auto hp = std::make_unique<Counters>();
+  Counters &h = *hp;
   HIP_TRY(hipMemcpy(&h, c->cnt, sizeof(h), hipMemcpyDeviceToHost));
   const int64_t n = std::min<int64_t>((int64_t)h.trace_n, c->trace_cap);
   *n_out = n;
